@@ -1,0 +1,199 @@
+"""Benchmark of the guide's hot path on one MI355X.
+
+Workload (the metric's configuration, one GPU): the full guided query —
+1,048,576 queries; per net a 16-level hash grid (F=2, T=2^19, base 16,
+scale 2) + SH(4) + OneBlob(4) (phi adds TriangleWave(12) of theta);
+theta/phi MLPs 64x3 -> 32 bins; NRC 64x5 -> RGB — fused into one launch
+(fp16 MFMA, fp32 accumulation, fp32 softmax/CDF/pdf).  A step is one query of
+the whole batch with inputs resident in HBM.  Beside it, in the same run:
+the plain FP32 per-batch GPU path on the same batch (the comparator of the
+>=10x target), the training step (262,144 samples: DF KL loss + NRC
+RelativeL2Luminance, backward, Adam+EMA), p50/p99 batch latency, the pdf
+error against the FP64 oracle on a sample, and the FP32 CPU restatement on the
+host cores (baseline only).
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "guiding-df_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "guided queries/s + train samples/s vs plain FP32 path; p99 batch latency"
+HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0
+
+
+def net_flops(enc_width, n_hidden, n_out_pad):
+    """Multiply-adds x2 of one MLP forward at the reference's padded widths."""
+    return 2 * (enc_width * 64 + (n_hidden - 1) * 64 * 64 + 64 * n_out_pad)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--train-batch", type=int, default=262144)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import dfguide as G
+    from gpu_util import df_features, uniforms, wi_init, train_targets
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    cfg = G.hash_config(n_bins_theta=32, n_bins_phi=32, with_nrc=True)
+    field = G.GuidingField(cfg)
+    info = field.info
+    n = args.n
+
+    feat_h = df_features(n, 1, nrc=True)
+    u_h = uniforms(n, 2)
+    feat = torch.from_numpy(feat_h).cuda()
+    u = torch.from_numpy(u_h).cuda()
+    wi = torch.from_numpy(wi_init(n)).cuda()
+    rgb = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+
+    def q_fused():
+        field.query(feat, u, wi, nrc_rgb=rgb, stream=stream)
+
+    def q_plain():
+        field.query_plain(feat, u, wi, nrc_rgb=rgb, stream=stream)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for s, e in evs:
+            s.record(stream)
+            fn()
+            e.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = np.array([s.elapsed_time(e) for s, e in evs])
+        return wall, ms
+
+    # ---- fused query (the measured value)
+    wall, ms = timed(q_fused, args.steps, args.warmup)
+    ms_step = wall * 1000.0 / args.steps
+    qps = n / (wall / args.steps)
+    kernel_ms = float(ms.mean())  # one kernel launch per step: its average duration
+
+    # ---- plain FP32 per-batch path on the same batch
+    psteps = max(3, args.steps // 5)
+    pwall, pms = timed(q_plain, psteps, 2)
+    plain_qps = n / (pwall / psteps)
+
+    # ---- training step (config 3 shape): DF + NRC on 262,144 samples
+    B = args.train_batch
+    tf = torch.from_numpy(df_features(B, 3, nrc=True)).cuda()
+    tt = torch.from_numpy(train_targets(B, 4)).cuda()
+    trgb = torch.from_numpy(np.exp(np.random.default_rng(5).normal(0, 2, (B, 3))).astype(np.float32)).cuda()
+    loss = torch.zeros(2, dtype=torch.float32, device="cuda")
+
+    def train():
+        field.train_step(tf, tt, loss[0:1], stream=stream)
+        field.nrc_train_step(tf, trgb, loss[1:2], stream=stream)
+
+    tsteps = max(3, args.steps // 5)
+    twall, tms = timed(train, tsteps, 2)
+    train_sps = B / (twall / tsteps)
+
+    # ---- accuracy of this run's weights against the FP64 oracle (sample)
+    err = None
+    try:
+        import oracle as O
+        th, ph, _ = O.config_layouts(cfg)
+        k = 4096
+        field.query(feat[:k].contiguous(), u[:k].contiguous(), wi[:k].contiguous(), stream=stream)
+        w0 = torch.from_numpy(wi_init(k)).cuda()
+        field.query(feat[:k].contiguous(), u[:k].contiguous(), w0, stream=stream)
+        torch.cuda.synchronize()
+        g = w0.cpu().numpy()
+        o = O.query_f64(th, ph, field.get_params(0, 2), field.get_params(1, 2), feat_h[:k], 9, u_h[:k], wi_init(k), 1)
+        rel = np.abs(g[:, 2] * g[:, 3] - o[:, 2] * o[:, 3]) / np.abs(o[:, 2] * o[:, 3])
+        err = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)), "sample": k}
+    except Exception as e:  # accuracy is reported, never required for the timing
+        err = {"error": str(e)}
+
+    # ---- CPU baseline: FP32 reference-order restatement on the host cores
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        th, ph, nr = O.config_layouts(cfg)
+        pt, pp, pn = (field.get_params(i, 2) for i in range(3))
+        k = 65536
+        fs, us, ws = feat_h[:k], u_h[:k], wi_init(k)
+        done, t0 = 0, time.perf_counter()
+        threads = 1
+        while True:
+            _, threads = O.query_f32_omp(th, ph, pt, pp, fs, 9, us, ws, 1)
+            O.nrc_f32_omp(nr, pn, fs)
+            done += k
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        cpu = {"value": done / el, "unit": "queries/s", "cores": int(threads), "kind": "port",
+               "sample": "%d passes over the first %d queries of the same batch (same weights), %.1f s" %
+                         (done // k, k, el)}
+
+    fl_q = (net_flops(info.enc_width[0], info.n_hidden[0], info.n_out_padded[0]) +
+            net_flops(info.enc_width[1], info.n_hidden[1], info.n_out_padded[1]) +
+            net_flops(info.enc_width[2], info.n_hidden[2], info.n_out_padded[2]))
+    bytes_q = 2 * 16 * 8 * 4 + (15 * 4 + 16 + 32 + 12)  # grid gathers + feat/u/wi(rw)/rgb
+    achieved_gbs = bytes_q * n / (kernel_ms * 1e-3) / 1e9
+    achieved_tf = fl_q * n / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16",
+        "data": "synthetic",
+        "config": {
+            "workload": "full guided query: 1,048,576 queries; 2 hash grids 16x2 T=2^19 + SH4 + OneBlob4 "
+                        "(+TriangleWave12 on phi); theta/phi 64x3 -> 32 bins; NRC 64x5 -> RGB; "
+                        "fp16 MFMA / fp32 accumulate, fp32 softmax/CDF/pdf",
+            "queries": n, "bins": [32, 32], "train_batch": B,
+        },
+        "plain_fp32_queries_per_s": plain_qps,
+        "speedup_vs_plain_fp32": qps / plain_qps,
+        "train_samples_per_s": train_sps,
+        "train_ms_per_step": float(twall * 1000.0 / tsteps),
+        "batch_latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
+        "pdf_rel_err_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "query_fused_kernel", "kernel_ms": kernel_ms,
+                     "bytes_per_query": bytes_q, "flops_per_query": fl_q,
+                     "mfma_tflops": achieved_tf, "mfma_frac": achieved_tf / MFMA_F16_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    field.close()
+
+
+if __name__ == "__main__":
+    main()

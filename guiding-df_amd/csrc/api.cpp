@@ -1,0 +1,981 @@
+// Host side of the guide: configuration -> network specs, parameter ownership
+// (fp32 master, fp16 training copy, fp16 EMA inference copy, Adam state), the
+// workspace, and every extern "C" entry point of include/dfg.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dfg.h"
+#include "internal.h"
+#include "json.h"
+
+namespace dfg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                               \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw Error(e_ == hipErrorOutOfMemory ? DFG_ERR_OOM : DFG_ERR_HIP,                    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+static void check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(DFG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+static inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+static inline uint32_t next_multiple(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+static std::string lower(std::string s) {
+    for (auto& c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+// PCG32 (O'Neill 2014, XSH-RR output), used to seed the weights.
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    Pcg32(uint64_t seed, uint64_t seq) {
+        inc = (seq << 1u) | 1u;
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    float next_float() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// ---------------------------------------------------------------------------
+// Configuration -> NetSpec.  Composite width / padding rules follow
+// tcnn encodings/composite.h:136-215 and encoding.h:85-87; grid level sizes
+// follow grid.h:959-1025; the MLP layout follows fully_fused_mlp.cu:636-688.
+// ---------------------------------------------------------------------------
+struct OptCfg {
+    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f, l2 = 1e-8f;
+    float ema_decay = 0.0f;  // 0 = no EMA (inference uses the training weights)
+};
+
+static OptCfg parse_optimizer(const Json& j) {
+    OptCfg o;
+    const Json* adam = &j;
+    std::string ot = lower(j.string_or("otype", "adam"));
+    if (ot == "ema") {
+        o.ema_decay = (float)j.number_or("decay", 0.99);
+        if (!j.contains("nested")) throw Error(DFG_ERR_CONFIG, "EMA optimizer needs a nested optimizer");
+        adam = &j.at("nested");
+        ot = lower(adam->string_or("otype", "adam"));
+    }
+    if (ot != "adam") throw Error(DFG_ERR_CONFIG, "unsupported optimizer '" + ot + "' (Adam or EMA(Adam))");
+    o.lr = (float)adam->number_or("learning_rate", 1e-3);
+    o.beta1 = (float)adam->number_or("beta1", 0.9);
+    o.beta2 = (float)adam->number_or("beta2", 0.999);
+    o.eps = (float)adam->number_or("epsilon", 1e-8);
+    o.l2 = (float)adam->number_or("l2_reg", 1e-8);
+    if (adam->bool_or("adabound", false)) throw Error(DFG_ERR_CONFIG, "adabound is not supported");
+    return o;
+}
+
+struct PartCfg {
+    const Json* j;
+    int type;
+    int dims;
+    int begin;
+};
+
+static int part_width(const Json& j, int type, int dims, int* param) {
+    switch (type) {
+        case ENC_IDENTITY: *param = 0; return dims;
+        case ENC_ONEBLOB: {
+            const int nb = (int)j.number_or("n_bins", 16);
+            if (nb <= 0 || (nb & (nb - 1))) throw Error(DFG_ERR_CONFIG, "OneBlob n_bins must be a power of two");
+            *param = nb;
+            return dims * nb;
+        }
+        case ENC_SH: {
+            const int deg = (int)j.number_or("degree", 4);
+            if (dims != 3) throw Error(DFG_ERR_CONFIG, "SphericalHarmonics encodes exactly 3 dims");
+            if (deg < 1 || deg > 4) throw Error(DFG_ERR_CONFIG, "SphericalHarmonics degree must be 1..4");
+            *param = deg;
+            return deg * deg;
+        }
+        case ENC_TRIWAVE: *param = (int)j.number_or("n_frequencies", 12); return dims * *param;
+        case ENC_FREQUENCY: *param = (int)j.number_or("n_frequencies", 12); return dims * *param * 2;
+        case ENC_GRID: {
+            const int F = (int)j.number_or("n_features_per_level", 2);
+            if (F != 1 && F != 2 && F != 4 && F != 8) throw Error(DFG_ERR_CONFIG, "grid n_features_per_level must be 1, 2, 4 or 8");
+            if (dims != 3) throw Error(DFG_ERR_CONFIG, "grid encodings encode exactly 3 dims here");
+            int L;
+            if (j.contains("n_features") || j.contains("n_grid_features")) {
+                const int nf = (int)(j.contains("n_features") ? j.at("n_features").num : j.at("n_grid_features").num);
+                L = (nf + F - 1) / F;
+            } else {
+                L = (int)j.number_or("n_levels", 16);
+            }
+            if (L < 1 || L > DFG_MAX_LEVELS) throw Error(DFG_ERR_CONFIG, "grid n_levels out of range");
+            *param = F;
+            return L * F;
+        }
+    }
+    throw Error(DFG_ERR_CONFIG, "bad encoding type");
+}
+
+static int enc_type(const std::string& ot) {
+    const std::string s = lower(ot);
+    if (s == "identity") return ENC_IDENTITY;
+    if (s == "oneblob") return ENC_ONEBLOB;
+    if (s == "sphericalharmonics") return ENC_SH;
+    if (s == "trianglewave") return ENC_TRIWAVE;
+    if (s == "frequency") return ENC_FREQUENCY;
+    if (s == "grid" || s == "hashgrid" || s == "densegrid" || s == "tiledgrid") return ENC_GRID;
+    throw Error(DFG_ERR_CONFIG, "unsupported encoding '" + ot + "'");
+}
+
+static NetSpec build_net(const Json& enc, const Json& net, int in_dims, int n_out) {
+    NetSpec ns;
+    std::memset(&ns, 0, sizeof(ns));
+    ns.in_dims = in_dims;
+    if (in_dims > DFG_MAX_IN) throw Error(DFG_ERR_CONFIG, "too many input dims");
+    // nested list
+    std::vector<const Json*> nested;
+    if (lower(enc.string_or("otype", "OneBlob")) == "composite") {
+        if (!enc.contains("nested") || !enc.at("nested").is_array()) throw Error(DFG_ERR_CONFIG, "Composite needs 'nested'");
+        for (const auto& e : enc.at("nested").arr) nested.push_back(&e);
+    } else {
+        nested.push_back(&enc);
+    }
+    int specified = 0, unspecified = 0;
+    for (auto* e : nested) {
+        if (e->contains("n_dims_to_encode")) specified += (int)e->at("n_dims_to_encode").num;
+        else ++unspecified;
+    }
+    if (unspecified > 1) throw Error(DFG_ERR_CONFIG, "only one nested encoding may omit n_dims_to_encode");
+    if (specified > in_dims) throw Error(DFG_ERR_CONFIG, "nested encodings encode more dims than the input has");
+    std::vector<PartCfg> parts;
+    int offset = 0;
+    for (auto* e : nested) {
+        int dims = e->contains("n_dims_to_encode") ? (int)e->at("n_dims_to_encode").num : in_dims - specified;
+        if (e->contains("dims_to_encode_begin")) offset = (int)e->at("dims_to_encode_begin").num;
+        if (dims > 0) parts.push_back({e, enc_type(e->string_or("otype", "OneBlob")), dims, offset});
+        offset += dims;
+    }
+    if (parts.empty() || (int)parts.size() > DFG_MAX_PARTS) throw Error(DFG_ERR_CONFIG, "1..6 nested encodings supported");
+    ns.n_parts = (int)parts.size();
+    // widths and padding (composite.h:188-199; last part padded to the network alignment 16)
+    std::vector<int> width(parts.size()), param(parts.size()), align(parts.size());
+    int n_grid = 0;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        width[i] = part_width(*parts[i].j, parts[i].type, parts[i].dims, &param[i]);
+        align[i] = parts[i].type == ENC_GRID ? param[i] : 1;
+        if (parts[i].type == ENC_GRID) ++n_grid;
+        if (parts[i].begin + parts[i].dims > in_dims) throw Error(DFG_ERR_CONFIG, "nested encoding reads past the input");
+    }
+    if (n_grid > 1) throw Error(DFG_ERR_CONFIG, "at most one grid encoding per network");
+    int so_far = 0;
+    int total_align = 16;
+    for (int a : align) total_align = std::lcm(total_align, a);
+    for (size_t i = 0; i < parts.size(); ++i) {
+        int padded;
+        if (i + 1 < parts.size()) {
+            padded = (int)next_multiple((uint32_t)(so_far + width[i]), (uint32_t)align[i + 1]) - so_far;
+        } else {
+            padded = (int)next_multiple((uint32_t)(so_far + width[i]), (uint32_t)total_align) - so_far;
+        }
+        EncPart& P = ns.parts[i];
+        P.type = parts[i].type;
+        P.dim_begin = parts[i].begin;
+        P.n_dims = parts[i].dims;
+        P.out_begin = so_far;
+        P.n_out = width[i];
+        P.n_pad = padded - width[i];
+        P.pad_front = P.type == ENC_SH ? 1 : 0;
+        P.param = param[i];
+        P.id_scale = (float)parts[i].j->number_or("scale", 1.0);
+        P.id_offset = (float)parts[i].j->number_or("offset", 0.0);
+        so_far += padded;
+    }
+    ns.enc_width = so_far;
+    ns.k_steps = (so_far + 31) / 32;
+    if (ns.k_steps * 32 > DFG_MAX_ENC) throw Error(DFG_ERR_CONFIG, "encoded width above 128");
+    // MLP
+    const int width_n = (int)net.number_or("n_neurons", 64);
+    if (width_n != DFG_WIDTH) throw Error(DFG_ERR_CONFIG, "only 64-neuron networks are supported");
+    ns.n_hidden = (int)net.number_or("n_hidden_layers", 3);
+    if (ns.n_hidden < 1 || ns.n_hidden > DFG_MAX_HIDDEN) throw Error(DFG_ERR_CONFIG, "n_hidden_layers must be 1..8");
+    if (lower(net.string_or("activation", "ReLU")) != "relu") throw Error(DFG_ERR_CONFIG, "only ReLU activations");
+    if (lower(net.string_or("output_activation", "None")) != "none") throw Error(DFG_ERR_CONFIG, "only a linear output");
+    ns.n_out = n_out;
+    ns.n_out_pad = (int)next_multiple((uint32_t)n_out, 16u);
+    if (ns.n_out_pad > 64) throw Error(DFG_ERR_CONFIG, "at most 64 outputs");
+    ns.n_layers = ns.n_hidden + 1;
+    uint32_t off = 0;
+    for (int l = 0; l <= ns.n_hidden; ++l) {
+        ns.layer_off[l] = off;
+        const uint32_t rows = l == ns.n_hidden ? (uint32_t)ns.n_out_pad : 64u;
+        const uint32_t cols = l == 0 ? (uint32_t)ns.enc_width : 64u;
+        off += rows * cols;
+    }
+    ns.layer_off[ns.n_hidden + 1] = off;
+    ns.n_matrix_params = off;
+    // grid
+    for (int i = 0; i < ns.n_parts; ++i) {
+        if (ns.parts[i].type != ENC_GRID) continue;
+        const Json& j = *parts[i].j;
+        GridSpec& g = ns.grid;
+        g.present = 1;
+        g.n_feat = ns.parts[i].param;
+        g.n_levels = ns.parts[i].n_out / g.n_feat;
+        const std::string ot = lower(j.string_or("otype", "Grid"));
+        const std::string def_type = ot == "tiledgrid" ? "tiled" : (ot == "densegrid" ? "dense" : "hash");
+        const std::string ty = lower(j.string_or("type", def_type));
+        g.type = ty == "hash" ? GRID_HASH : (ty == "dense" ? GRID_DENSE : (ty == "tiled" ? GRID_TILED : -1));
+        if (g.type < 0) throw Error(DFG_ERR_CONFIG, "grid type must be Hash, Dense or Tiled");
+        if (lower(j.string_or("hash", "CoherentPrime")) != "coherentprime")
+            throw Error(DFG_ERR_CONFIG, "only the CoherentPrime hash is supported");
+        const std::string it = lower(j.string_or("interpolation", "Linear"));
+        g.interp = it == "linear" ? INTERP_LINEAR : (it == "smoothstep" ? INTERP_SMOOTHSTEP : (it == "nearest" ? INTERP_NEAREST : -1));
+        if (g.interp < 0) throw Error(DFG_ERR_CONFIG, "grid interpolation must be Linear, Smoothstep or Nearest");
+        if (j.bool_or("stochastic_interpolation", false)) throw Error(DFG_ERR_CONFIG, "stochastic interpolation unsupported");
+        g.base_res = (int)j.number_or("base_resolution", 16);
+        g.per_level_scale = (float)j.number_or("per_level_scale", 2.0);
+        const uint32_t log2T = (uint32_t)j.number_or("log2_hashmap_size", 19);
+        const float l2s = std::log2(g.per_level_scale);
+        uint32_t o = 0;
+        for (int l = 0; l < g.n_levels; ++l) {
+            const float scale = exp2f((float)l * l2s) * (float)g.base_res - 1.0f;
+            const uint32_t res = (uint32_t)ceilf(scale) + 1;
+            const uint32_t max_params = 0xffffffffu / 2;
+            uint32_t pl = std::pow((float)res, 3.0f) > (float)max_params ? max_params : res * res * res;
+            pl = next_multiple(pl, 8u);
+            if (g.type == GRID_TILED) pl = std::min(pl, (uint32_t)(g.base_res * g.base_res * g.base_res));
+            else if (g.type == GRID_HASH) pl = std::min(pl, 1u << log2T);
+            g.scale[l] = scale;
+            g.res[l] = res;
+            g.offset[l] = o;
+            o += pl;
+        }
+        g.offset[g.n_levels] = o;
+        g.param_begin = ns.n_matrix_params;
+    }
+    ns.n_params = ns.n_matrix_params + (ns.grid.present ? ns.grid.offset[ns.grid.n_levels] * ns.grid.n_feat : 0);
+    // fragment packs: forward then backward
+    uint32_t p = 0;
+    for (int l = 0; l <= ns.n_hidden; ++l) {
+        ns.fwd_off[l] = p;
+        const uint32_t mt = l == ns.n_hidden ? (uint32_t)ns.n_out_pad / 16 : 4u;
+        const uint32_t ks = l == 0 ? (uint32_t)ns.k_steps : 2u;
+        p += mt * ks * 512;
+    }
+    ns.fwd_off[ns.n_hidden + 1] = p;
+    for (int l = 0; l <= ns.n_hidden; ++l) {
+        ns.bwd_off[l] = p;
+        const uint32_t mt = l == 0 ? (uint32_t)ns.enc_width / 16 : 4u;
+        const uint32_t ks = l == ns.n_hidden ? (uint32_t)(ns.n_out_pad + 31) / 32 : 2u;
+        p += mt * ks * 512;
+    }
+    ns.bwd_off[ns.n_hidden + 1] = p;
+    ns.pack_size = p;
+    return ns;
+}
+
+// ---------------------------------------------------------------------------
+// Device state
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void ensure(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        HIPX(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct NetState {
+    bool present = false;
+    NetSpec spec;
+    OptCfg opt;
+    uint32_t step = 0;
+    DevBuf<NetSpec> d_spec;
+    DevBuf<float> master, m, v, grad;
+    DevBuf<_Float16> w16, ema16, pack_train, pack_infer;
+    DevBuf<uint32_t> steps;
+    // training workspace
+    DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
+    DevBuf<float> dx, partial;
+    void release_all() {
+        d_spec.release(); master.release(); m.release(); v.release(); grad.release();
+        w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
+        x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();
+        dx.release(); partial.release();
+    }
+    const _Float16* tab_infer() const { return spec.grid.present ? ema16.p + spec.grid.param_begin : nullptr; }
+    const _Float16* tab_train() const { return spec.grid.present ? w16.p + spec.grid.param_begin : nullptr; }
+};
+
+}  // namespace dfg
+
+using namespace dfg;
+
+struct dfg_ctx {
+    NetState net[3];
+    int n_theta = 16, n_phi = 32, linear = 1, D = 9;
+    uint64_t seed = 7272;
+    std::string err;
+    // plain-path and loss workspace
+    DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
+    DevBuf<float> tmp;
+    ~dfg_ctx() {
+        for (auto& n : net) n.release_all();
+        p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
+    }
+};
+
+static std::string g_create_error = "no error";
+
+static int fail(dfg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define API_BEGIN try {
+#define API_END(ctx)                                                        \
+    }                                                                       \
+    catch (const Error& e) { return fail(ctx, e.code, e.what()); }          \
+    catch (const std::bad_alloc&) { return fail(ctx, DFG_ERR_OOM, "host out of memory"); } \
+    catch (const std::exception& e) { return fail(ctx, DFG_ERR_CONFIG, e.what()); }         \
+    return DFG_OK;
+
+static void init_net_params(dfg_ctx* c, int id) {
+    NetState& s = c->net[id];
+    const NetSpec& ns = s.spec;
+    std::vector<float> h(ns.n_params);
+    Pcg32 rng(c->seed, (uint64_t)id + 1);
+    for (int l = 0; l <= ns.n_hidden; ++l) {
+        const uint32_t rows = l == ns.n_hidden ? (uint32_t)ns.n_out_pad : 64u;
+        const uint32_t cols = l == 0 ? (uint32_t)ns.enc_width : 64u;
+        const float bound = std::sqrt(6.0f / (float)(rows + cols));  // xavier uniform, gpu_matrix.h:291-306
+        for (uint32_t i = 0; i < rows * cols; ++i) h[ns.layer_off[l] + i] = rng.next_float() * 2.0f * bound - bound;
+    }
+    for (size_t i = ns.n_matrix_params; i < ns.n_params; ++i)  // grid init U[-1e-4, 1e-4], grid.h:1364-1369
+        h[i] = rng.next_float() * 2e-4f - 1e-4f;
+    HIPX(hipMemcpy(s.master.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+static void sync_copies(dfg_ctx* c, int id, hipStream_t st) {
+    NetState& s = c->net[id];
+    launch_f32_to_f16(s.master.p, s.w16.p, s.spec.n_params, st);
+    HIPX(hipMemcpyAsync(s.ema16.p, s.w16.p, s.spec.n_params * sizeof(_Float16), hipMemcpyDeviceToDevice, st));
+    launch_pack(s.d_spec.p, s.spec, s.w16.p, s.pack_train.p, 3, st);
+    launch_pack(s.d_spec.p, s.spec, s.ema16.p, s.pack_infer.p, 1, st);
+    check_launch();
+}
+
+static void alloc_net(dfg_ctx* c, int id, const NetSpec& ns, const OptCfg& opt) {
+    NetState& s = c->net[id];
+    s.present = true;
+    s.spec = ns;
+    s.opt = opt;
+    s.d_spec.ensure(1);
+    HIPX(hipMemcpy(s.d_spec.p, &s.spec, sizeof(NetSpec), hipMemcpyHostToDevice));
+    const size_t P = ns.n_params;
+    s.master.ensure(P); s.m.ensure(P); s.v.ensure(P); s.grad.ensure(P);
+    s.w16.ensure(P); s.ema16.ensure(P); s.steps.ensure(P);
+    s.pack_train.ensure(ns.pack_size);
+    s.pack_infer.ensure(ns.fwd_off[ns.n_hidden + 1]);
+    HIPX(hipMemset(s.m.p, 0, P * sizeof(float)));
+    HIPX(hipMemset(s.v.p, 0, P * sizeof(float)));
+    HIPX(hipMemset(s.grad.p, 0, P * sizeof(float)));
+    HIPX(hipMemset(s.steps.p, 0, P * sizeof(uint32_t)));
+    init_net_params(c, id);
+    sync_copies(c, id, nullptr);
+    HIPX(hipDeviceSynchronize());
+}
+
+extern "C" {
+
+int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
+    if (!json_config || !out_ctx) return fail(nullptr, DFG_ERR_INVALID_ARG, "null argument");
+    *out_ctx = nullptr;
+    std::unique_ptr<dfg_ctx> c(new dfg_ctx());
+    try {
+        const Json root = parse_json(json_config);
+        if (!root.is_object()) throw Error(DFG_ERR_CONFIG, "config must be a JSON object");
+        Json empty;
+        empty.kind = Json::Object;
+        const Json& d = root.contains("dfg") ? root.at("dfg") : empty;
+        c->n_theta = (int)d.number_or("n_bins_theta", 16);
+        c->n_phi = (int)d.number_or("n_bins_phi", 32);
+        const std::string interp = lower(d.string_or("interpolation", "linear"));
+        if (interp != "linear" && interp != "nearest") throw Error(DFG_ERR_CONFIG, "interpolation must be linear or nearest");
+        c->linear = interp == "linear";
+        c->seed = (uint64_t)d.number_or("seed", 7272);
+        c->D = (int)d.number_or("n_input_dims", 9);
+        auto okb = [](int b) { return b == 16 || b == 32 || b == 64; };
+        if (!okb(c->n_theta) || !okb(c->n_phi)) throw Error(DFG_ERR_CONFIG, "bins must be 16, 32 or 64");
+        const bool pair_ok = (c->n_theta == 16 && (c->n_phi == 16 || c->n_phi == 32)) ||
+                             (c->n_theta == 32 && (c->n_phi == 32 || c->n_phi == 64)) ||
+                             (c->n_theta == 64 && c->n_phi == 64);
+        if (!pair_ok) throw Error(DFG_ERR_CONFIG, "unsupported (n_bins_theta, n_bins_phi) pair");
+        if (c->D < 1 || c->D + 1 > DFG_MAX_IN) throw Error(DFG_ERR_CONFIG, "n_input_dims out of range");
+        if (!root.contains("nn")) throw Error(DFG_ERR_CONFIG, "missing 'nn' section");
+        const Json& nn = root.at("nn");
+        // key lookup of learnedDistribution.cpp:84-90
+        auto pick = [&](const char* a, const char* b) -> const Json& { return nn.contains(a) ? nn.at(a) : nn.at(b); };
+        const NetSpec th = build_net(pick("theta_encoding", "encoding"), pick("theta_network", "network"), c->D, c->n_theta);
+        const NetSpec ph = build_net(pick("phi_encoding", "encoding"), pick("phi_network", "network"), c->D + 1, c->n_phi);
+        alloc_net(c.get(), DFG_NET_THETA, th, parse_optimizer(pick("theta_optimizer", "optimizer")));
+        alloc_net(c.get(), DFG_NET_PHI, ph, parse_optimizer(pick("phi_optimizer", "optimizer")));
+        if (root.contains("nrc")) {
+            const Json& nr = root.at("nrc");
+            const NetSpec n = build_net(nr.at("encoding"), nr.at("network"), 15, 3);
+            alloc_net(c.get(), DFG_NET_NRC, n, parse_optimizer(nr.at("optimizer")));
+        }
+    } catch (const Error& e) {
+        return fail(nullptr, e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail(nullptr, DFG_ERR_CONFIG, e.what());
+    }
+    *out_ctx = c.release();
+    return DFG_OK;
+}
+
+int dfg_destroy(dfg_ctx* ctx) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    delete ctx;
+    return DFG_OK;
+}
+
+const char* dfg_last_error(const dfg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int dfg_get_info(const dfg_ctx* ctx, dfg_info* info) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    if (!info) return DFG_ERR_INVALID_ARG;
+    std::memset(info, 0, sizeof(*info));
+    info->n_bins_theta = ctx->n_theta;
+    info->n_bins_phi = ctx->n_phi;
+    info->interpolation = ctx->linear;
+    info->n_input_dims = ctx->D;
+    info->has_nrc = ctx->net[DFG_NET_NRC].present ? 1 : 0;
+    for (int i = 0; i < 3; ++i) {
+        if (!ctx->net[i].present) continue;
+        const NetSpec& s = ctx->net[i].spec;
+        info->enc_width[i] = s.enc_width;
+        info->n_hidden[i] = s.n_hidden;
+        info->n_out_padded[i] = s.n_out_pad;
+        info->n_params[i] = s.n_params;
+        info->n_matrix_params[i] = s.n_matrix_params;
+    }
+    info->max_batch = (int64_t)1 << 30;
+    return DFG_OK;
+}
+
+static NetState& need_net(dfg_ctx* c, int net) {
+    if (net < 0 || net > 2) throw Error(DFG_ERR_INVALID_ARG, "unknown net id");
+    if (!c->net[net].present) throw Error(DFG_ERR_INVALID_ARG, "net not configured");
+    return c->net[net];
+}
+
+int dfg_get_params(dfg_ctx* ctx, int net, int which, float* host_out, size_t n) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    NetState& s = need_net(ctx, net);
+    if (!host_out) throw Error(DFG_ERR_INVALID_ARG, "null output");
+    if (n != s.spec.n_params) throw Error(DFG_ERR_BAD_SIZE, "parameter count mismatch");
+    if (which == DFG_PARAMS_MASTER) {
+        HIPX(hipMemcpy(host_out, s.master.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    } else if (which == DFG_PARAMS_TRAIN || which == DFG_PARAMS_INFERENCE) {
+        ctx->tmp.ensure(n);
+        launch_f16_to_f32(which == DFG_PARAMS_TRAIN ? s.w16.p : s.ema16.p, ctx->tmp.p, (int64_t)n, nullptr);
+        check_launch();
+        HIPX(hipMemcpy(host_out, ctx->tmp.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    } else {
+        throw Error(DFG_ERR_INVALID_ARG, "unknown parameter view");
+    }
+    API_END(ctx)
+}
+
+int dfg_get_grad(dfg_ctx* ctx, int net, float* host_out, size_t n) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    NetState& s = need_net(ctx, net);
+    if (!host_out) throw Error(DFG_ERR_INVALID_ARG, "null output");
+    if (n != s.spec.n_params) throw Error(DFG_ERR_BAD_SIZE, "parameter count mismatch");
+    HIPX(hipMemcpy(host_out, s.grad.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    API_END(ctx)
+}
+
+int dfg_set_params(dfg_ctx* ctx, int net, const float* host_in, size_t n) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    NetState& s = need_net(ctx, net);
+    if (!host_in) throw Error(DFG_ERR_INVALID_ARG, "null input");
+    if (n != s.spec.n_params) throw Error(DFG_ERR_BAD_SIZE, "parameter count mismatch");
+    HIPX(hipMemcpy(s.master.p, host_in, n * sizeof(float), hipMemcpyHostToDevice));
+    sync_copies(ctx, net, nullptr);
+    HIPX(hipDeviceSynchronize());
+    API_END(ctx)
+}
+
+static const char* kFileNames[3] = {"AR_network_theta.bin", "AR_network_phi.bin", "NRC_network.bin"};
+
+int dfg_save(dfg_ctx* ctx, const char* prefix) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    if (!prefix) return fail(ctx, DFG_ERR_INVALID_ARG, "null path");
+    API_BEGIN
+    for (int i = 0; i < 3; ++i) {
+        if (!ctx->net[i].present) continue;
+        std::vector<float> h(ctx->net[i].spec.n_params);
+        HIPX(hipMemcpy(h.data(), ctx->net[i].master.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const std::string path = std::string(prefix) + kFileNames[i];
+        std::ofstream f(path, std::ios::binary);
+        if (!f) throw Error(DFG_ERR_IO, "cannot open " + path);
+        f.write((const char*)h.data(), (std::streamsize)(h.size() * sizeof(float)));
+        if (!f) throw Error(DFG_ERR_IO, "write failed: " + path);
+    }
+    API_END(ctx)
+}
+
+int dfg_load(dfg_ctx* ctx, const char* prefix) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    if (!prefix) return fail(ctx, DFG_ERR_INVALID_ARG, "null path");
+    API_BEGIN
+    for (int i = 0; i < 3; ++i) {
+        if (!ctx->net[i].present) continue;
+        const std::string path = std::string(prefix) + kFileNames[i];
+        std::ifstream f(path, std::ios::binary | std::ios::ate);
+        if (!f) throw Error(DFG_ERR_IO, "cannot open " + path);
+        const size_t bytes = (size_t)f.tellg();
+        if (bytes != ctx->net[i].spec.n_params * sizeof(float))
+            throw Error(DFG_ERR_BAD_SIZE, path + ": size does not match the network's parameter count");
+        f.seekg(0);
+        std::vector<float> h(ctx->net[i].spec.n_params);
+        f.read((char*)h.data(), (std::streamsize)bytes);
+        if (!f) throw Error(DFG_ERR_IO, "read failed: " + path);
+        HIPX(hipMemcpy(ctx->net[i].master.p, h.data(), bytes, hipMemcpyHostToDevice));
+        sync_copies(ctx, i, nullptr);
+    }
+    HIPX(hipDeviceSynchronize());
+    API_END(ctx)
+}
+
+// --- query family ---------------------------------------------------------------
+static void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
+    int xs = 0;
+    if (with_df) {
+        xs = std::max(c->net[0].spec.k_steps, c->net[1].spec.k_steps) * 32 + 8;
+    }
+    if (with_nrc) xs = std::max(xs, c->net[2].spec.k_steps * 32 + 8);
+    a.xs = xs;
+    a.ls = std::max(std::max(c->n_theta, c->n_phi), 16) + 1;
+}
+
+static QueryArgs base_args(dfg_ctx* c) {
+    QueryArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.th = c->net[0].d_spec.p;
+    a.ph = c->net[1].d_spec.p;
+    a.pack_th = c->net[0].pack_infer.p;
+    a.pack_ph = c->net[1].pack_infer.p;
+    a.tab_th = c->net[0].tab_infer();
+    a.tab_ph = c->net[1].tab_infer();
+    if (c->net[2].present) {
+        a.nrc = c->net[2].d_spec.p;
+        a.pack_nrc = c->net[2].pack_infer.p;
+        a.tab_nrc = c->net[2].tab_infer();
+    }
+    a.linear = c->linear;
+    return a;
+}
+
+static void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride) {
+    if (n < 0 || n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "n out of range");
+    if (n > 0 && !feat) throw Error(DFG_ERR_INVALID_ARG, "null features");
+    if (stride < c->D) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below n_input_dims");
+}
+
+int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, const float* u,
+              const uint8_t* guided, float* wi, float* nrc_rgb) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, n, feat, feat_stride);
+    if (n > 0 && (!u || !wi)) throw Error(DFG_ERR_INVALID_ARG, "null u or wi");
+    if (nrc_rgb && !ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    if (nrc_rgb && feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "NRC output needs feat_stride >= 15");
+    if (n == 0) return DFG_OK;
+    QueryArgs a = base_args(ctx);
+    a.feat = feat; a.stride = feat_stride; a.n = n; a.u = u; a.guided = guided; a.wi = wi; a.rgb = nrc_rgb;
+    fill_lds_strides(ctx, a, nrc_rgb != nullptr, true);
+    launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, const float* wi2, float* pdf) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, n, feat, feat_stride);
+    if (n > 0 && (!wi2 || !pdf)) throw Error(DFG_ERR_INVALID_ARG, "null wi2 or pdf");
+    if (n == 0) return DFG_OK;
+    QueryArgs a = base_args(ctx);
+    a.feat = feat; a.stride = feat_stride; a.n = n; a.wi2 = wi2; a.pdf_out = pdf;
+    fill_lds_strides(ctx, a, false, true);
+    launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_logits(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, const float* theta_in,
+               float* logits_theta, float* logits_phi) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, n, feat, feat_stride);
+    if (n > 0 && (!theta_in || !logits_theta || !logits_phi)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return DFG_OK;
+    QueryArgs a = base_args(ctx);
+    a.feat = feat; a.stride = feat_stride; a.n = n; a.theta_in = theta_in;
+    a.logits_th = logits_theta; a.logits_ph = logits_phi;
+    fill_lds_strides(ctx, a, false, true);
+    launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_nrc_infer(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, float* rgb) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    if (n < 0 || n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "n out of range");
+    if (feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below 15");
+    if (n > 0 && (!feat || !rgb)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return DFG_OK;
+    QueryArgs a = base_args(ctx);
+    a.th = nullptr;
+    a.feat = feat; a.stride = feat_stride; a.n = n; a.rgb = rgb;
+    fill_lds_strides(ctx, a, true, false);
+    launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_query_plain(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, const float* u,
+                    const uint8_t* guided, float* wi, float* nrc_rgb) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, n, feat, feat_stride);
+    if (n > 0 && (!u || !wi)) throw Error(DFG_ERR_INVALID_ARG, "null u or wi");
+    if (nrc_rgb && !ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    if (nrc_rgb && feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "NRC output needs feat_stride >= 15");
+    if (n == 0) return DFG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    // the reference synchronises the device at the start of every query stage
+    HIPX(hipStreamSynchronize(st));
+    const int64_t ld = round_up(n, 64);
+    int kmax = std::max(ctx->net[0].spec.enc_width, ctx->net[1].spec.enc_width);
+    if (nrc_rgb) kmax = std::max(kmax, ctx->net[2].spec.enc_width);
+    ctx->p_enc.ensure((size_t)kmax * ld);
+    ctx->p_a.ensure((size_t)64 * ld);
+    ctx->p_b.ensure((size_t)64 * ld);
+    ctx->p_theta.ensure((size_t)ld);
+    auto run_mlp = [&](NetState& s, const float* extra) -> const float* {
+        const NetSpec& ns = s.spec;
+        launch_plain_encode(s.d_spec.p, s.tab_infer(), feat, feat_stride, extra, n, ld, ctx->p_enc.p, st);
+        const float* X = ctx->p_enc.p;
+        float* bufs[2] = {ctx->p_a.p, ctx->p_b.p};
+        int cur = 0;
+        for (int l = 0; l <= ns.n_hidden; ++l) {
+            const int M = l == ns.n_hidden ? ns.n_out_pad : 64;
+            const int K = l == 0 ? ns.enc_width : 64;
+            launch_plain_layer(s.ema16.p + ns.layer_off[l], M, K, K, X, bufs[cur], l < ns.n_hidden, n, ld, st);
+            X = bufs[cur];
+            cur ^= 1;
+        }
+        return X;
+    };
+    const float* lt = run_mlp(ctx->net[0], nullptr);
+    launch_plain_head(0, ctx->n_theta, ctx->n_theta, ctx->linear, lt, ld, n, u, guided, wi, ctx->p_theta.p, st);
+    const float* lp = run_mlp(ctx->net[1], ctx->p_theta.p);
+    launch_plain_head(1, ctx->n_phi, ctx->n_theta, ctx->linear, lp, ld, n, u, guided, wi, ctx->p_theta.p, st);
+    if (nrc_rgb) {
+        const float* o = run_mlp(ctx->net[2], nullptr);
+        launch_plain_rgb(o, ld, n, nrc_rgb, st);
+    }
+    check_launch();
+    API_END(ctx)
+}
+
+// --- training ----------------------------------------------------------------------
+static void ensure_train_ws(NetState& s, int64_t ld, int n_chunks) {
+    const NetSpec& ns = s.spec;
+    s.x.ensure((size_t)ns.k_steps * 32 * ld);
+    s.act.ensure((size_t)ns.n_hidden * 64 * ld);
+    s.g.ensure((size_t)ns.n_hidden * 64 * ld);
+    s.out.ensure((size_t)ns.n_out_pad * ld);
+    s.gout_rm.ensure((size_t)ns.n_out_pad * ld);
+    s.gout_fm.ensure((size_t)ns.n_out_pad * ld);
+    if (ns.grid.present) s.dx.ensure((size_t)ns.grid.n_levels * ns.grid.n_feat * ld);
+    s.partial.ensure((size_t)n_chunks * ns.n_matrix_params);
+}
+
+static int grid_part_index(const NetSpec& ns) {
+    for (int i = 0; i < ns.n_parts; ++i)
+        if (ns.parts[i].type == ENC_GRID) return i;
+    return -1;
+}
+
+static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int stride, const float* target,
+                              int mode, int64_t n, int64_t ld, hipStream_t st) {
+    TrainFwdArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ns = s.d_spec.p;
+    a.pack = s.pack_train.p;
+    a.tab = s.tab_train();
+    a.feat = feat;
+    a.stride = stride;
+    a.target = target;
+    a.theta_bins = c->n_theta;
+    a.linear = c->linear;
+    a.mode = mode;
+    a.n = n;
+    a.ld = ld;
+    a.x = s.x.p;
+    a.act = s.act.p;
+    a.out = s.out.p;
+    a.xs = s.spec.k_steps * 32 + 8;
+    a.ls = s.spec.n_out_pad + 1;
+    launch_train_fwd(a, s.spec, st);
+}
+
+static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
+                                int n_chunks, int chunk, float loss_scale, hipStream_t st) {
+    (void)c;
+    const NetSpec& ns = s.spec;
+    const int gp = grid_part_index(ns);
+    TrainBwdArgs b;
+    std::memset(&b, 0, sizeof(b));
+    b.ns = s.d_spec.p;
+    b.pack = s.pack_train.p;
+    b.gout = s.gout_rm.p;
+    b.act = s.act.p;
+    b.g = s.g.p;
+    b.n = n;
+    b.ld = ld;
+    if (gp >= 0) {
+        b.dx = s.dx.p;
+        b.dx_begin = ns.parts[gp].out_begin;
+        b.dx_n = ns.grid.n_levels * ns.grid.n_feat;
+    }
+    launch_train_bwd(b, ns, st);
+    WgradArgs w;
+    std::memset(&w, 0, sizeof(w));
+    w.ns = s.d_spec.p;
+    w.x = s.x.p;
+    w.act = s.act.p;
+    w.g = s.g.p;
+    w.gout_fm = s.gout_fm.p;
+    w.partial = s.partial.p;
+    w.grad = s.grad.p;
+    w.n = n;
+    w.ld = ld;
+    w.chunk = chunk;
+    w.n_chunks = n_chunks;
+    launch_wgrad(w, ns, st);
+    if (gp >= 0) {
+        const size_t ngrid = (size_t)ns.n_params - ns.grid.param_begin;
+        launch_fill_f32(s.grad.p + ns.grid.param_begin, 0.0f, (int64_t)ngrid, st);
+        GridBwdArgs gb;
+        std::memset(&gb, 0, sizeof(gb));
+        gb.ns = s.d_spec.p;
+        gb.feat = feat;
+        gb.stride = stride;
+        gb.pos_dim = ns.parts[gp].dim_begin;
+        gb.dx = s.dx.p;
+        gb.grad = s.grad.p;
+        gb.n = n;
+        gb.ld = ld;
+        launch_grid_bwd(gb, ns, st);
+    }
+    // Adam + EMA (EmaOptimizer::step, ema.h:102-136)
+    s.step += 1;
+    AdamArgs o;
+    std::memset(&o, 0, sizeof(o));
+    o.n_params = ns.n_params;
+    o.n_matrix = ns.n_matrix_params;
+    o.lr = s.opt.lr;
+    o.beta1 = s.opt.beta1;
+    o.beta2 = s.opt.beta2;
+    o.eps = s.opt.eps;
+    o.l2_reg = s.opt.l2;
+    o.loss_scale = loss_scale;
+    o.ema_decay = s.opt.ema_decay;
+    o.ema_debias_old = 1.0f - (float)std::pow((double)s.opt.ema_decay, (double)(s.step - 1));
+    o.ema_debias_new = 1.0f / (1.0f - (float)std::pow((double)s.opt.ema_decay, (double)s.step));
+    o.master = s.master.p;
+    o.w16 = s.w16.p;
+    o.ema16 = s.ema16.p;
+    o.grad = s.grad.p;
+    o.m = s.m.p;
+    o.v = s.v.p;
+    o.steps = s.steps.p;
+    launch_adam_ema(o, st);
+    launch_pack(s.d_spec.p, ns, s.w16.p, s.pack_train.p, 3, st);
+    launch_pack(s.d_spec.p, ns, s.ema16.p, s.pack_infer.p, 1, st);
+}
+
+static const int kChunk = 1024;
+
+int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride, const float* target,
+                   float* loss_out) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, B, feat, feat_stride);
+    if (B > 0 && !target) throw Error(DFG_ERR_INVALID_ARG, "null target");
+    if (B == 0) return DFG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t ld = round_up(B, 64);
+    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    NetState& th = ctx->net[0];
+    NetState& ph = ctx->net[1];
+    ensure_train_ws(th, ld, n_chunks);
+    ensure_train_ws(ph, ld, n_chunks);
+    ctx->loss.ensure((size_t)ld);
+    net_forward_train(ctx, th, feat, feat_stride, nullptr, 0, B, ld, st);
+    net_forward_train(ctx, ph, feat, feat_stride, target, 0, B, ld, st);
+    DfLossArgs l;
+    std::memset(&l, 0, sizeof(l));
+    l.target = target;
+    l.bins_th = th.out.p;
+    l.bins_ph = ph.out.p;
+    l.nt_pad = th.spec.n_out_pad;
+    l.np_pad = ph.spec.n_out_pad;
+    l.gout_th_rm = th.gout_rm.p;
+    l.gout_ph_rm = ph.gout_rm.p;
+    l.gout_th_fm = th.gout_fm.p;
+    l.gout_ph_fm = ph.gout_fm.p;
+    l.loss = ctx->loss.p;
+    l.n = B;
+    l.ld = ld;
+    l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE, parameters.h:45
+    l.linear = ctx->linear;
+    launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
+    net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                       const float* target_rgb, float* loss_out) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    if (B < 0 || B > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "B out of range");
+    if (feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below 15");
+    if (B > 0 && (!feat || !target_rgb)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    if (B == 0) return DFG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t ld = round_up(B, 64);
+    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    NetState& s = ctx->net[2];
+    ensure_train_ws(s, ld, n_chunks);
+    ctx->loss.ensure((size_t)ld);
+    net_forward_train(ctx, s, feat, feat_stride, nullptr, 1, B, ld, st);
+    NrcLossArgs l;
+    std::memset(&l, 0, sizeof(l));
+    l.pred = s.out.p;
+    l.target = target_rgb;
+    l.gout_rm = s.gout_rm.p;
+    l.gout_fm = s.gout_fm.p;
+    l.loss = ctx->loss.p;
+    l.n = B;
+    l.ld = ld;
+    l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE_NRC, parameters.h:59
+    launch_nrc_loss(l, st);
+    net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    check_launch();
+    API_END(ctx)
+}
+
+// --- stateless & debug ---------------------------------------------------------------
+int dfg_prepare_features(void* stream, int64_t n, const float* pos, const float* aabb, const float* wo,
+                         const float* normal, const float* roughness, float* feat) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (n == 0) return DFG_OK;
+    if (!pos || !aabb || !wo || !normal || !roughness || !feat) return DFG_ERR_INVALID_ARG;
+    launch_prepare_features(n, pos, aabb, wo, normal, roughness, feat, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+int dfg_df_head(void* stream, int64_t n, int n_theta, int n_phi, int interpolation, const float* logits_theta,
+                const float* logits_phi, const float* u, const uint8_t* guided, float* wi, float* bins_theta,
+                float* bins_phi, int32_t* seg) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    auto ok = [](int b) { return b == 16 || b == 32 || b == 64; };
+    if (!ok(n_theta) || !ok(n_phi)) return DFG_ERR_INVALID_ARG;
+    if (n == 0) return DFG_OK;
+    if (!logits_theta || !logits_phi || !u || !wi) return DFG_ERR_INVALID_ARG;
+    launch_df_head(n, n_theta, n_phi, interpolation ? 1 : 0, logits_theta, logits_phi, u, guided, wi, bins_theta,
+                   bins_phi, seg, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+int dfg_encode(dfg_ctx* ctx, void* stream, int net, int64_t n, const float* in, int in_stride, float* out) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    NetState& s = need_net(ctx, net);
+    if (n < 0) throw Error(DFG_ERR_BAD_SIZE, "n < 0");
+    if (in_stride < s.spec.in_dims) throw Error(DFG_ERR_BAD_SIZE, "in_stride below the net's input dims");
+    if (n == 0) return DFG_OK;
+    if (!in || !out) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    launch_encode_debug(s.d_spec.p, s.spec.enc_width, s.tab_infer(), in, in_stride, n, out, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_grid_index(dfg_ctx* ctx, void* stream, int net, int64_t n, const float* in, int in_stride, uint32_t* idx) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    NetState& s = need_net(ctx, net);
+    if (!s.spec.grid.present) throw Error(DFG_ERR_INVALID_ARG, "net has no grid encoding");
+    if (n < 0) throw Error(DFG_ERR_BAD_SIZE, "n < 0");
+    if (in_stride < s.spec.in_dims) throw Error(DFG_ERR_BAD_SIZE, "in_stride below the net's input dims");
+    if (n == 0) return DFG_OK;
+    if (!in || !idx) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    launch_grid_index_debug(s.d_spec.p, in, in_stride, n, idx, (hipStream_t)stream);
+    check_launch();
+    API_END(ctx)
+}
+
+}  // extern "C"

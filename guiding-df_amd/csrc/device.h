@@ -1,0 +1,624 @@
+// Device-side building blocks of the guide on CDNA4 (gfx950):
+//   * deterministic exp/log used by the softmax (so bins are reproducible),
+//   * the input encodings (grid, spherical harmonics, OneBlob, triangle wave,
+//     frequency, identity) evaluated one row per lane,
+//   * the Distribution-Factorization head: softmax x N, sequential CDF search,
+//     piecewise-linear inverse CDF, pdf evaluation, loss gradient,
+//   * 64-wide MLP layers on v_mfma_f32_16x16x32_f16 with fp32 accumulation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "spec.h"
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+namespace dfg {
+
+#define DFG_EPS 1e-5f                 // M_EPSILON, krrmath/constants.h:19
+#define DFG_INV_4PI 0.07957747154594766788f  // M_INV_4PI, krrmath/constants.h:17
+
+// ---------------------------------------------------------------------------
+// Deterministic exp / log.  Both use only IEEE-754 +, -, *, / and exact
+// bit manipulation, with FMA contraction disabled, so the host restatement of
+// the same steps produces bit-identical results.
+//   exp: Cody & Waite additive range reduction x = k ln2 + r with ln2 split
+//        into a 15-bit head (k*ln2_hi exact) and a tail, then the Taylor
+//        polynomial of e^r through r^7 (|r| <= ln2/2), then scaling by 2^k.
+//   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
+//        s = f/(2+f), log m = f - (f^2/2 - s (f^2/2 + R(s^2))) with R the
+//        atanh series 2s^2/3 + ... + 2s^10/11 (the reduction of fdlibm's
+//        e_log.c, Sun Microsystems 1993).
+// Both are within 2 ulp of the correctly rounded value over the normal range.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pow2i(int k) {  // 2^k for k in [-126, 127]
+    return __builtin_bit_cast(float, (uint32_t)(k + 127) << 23);
+}
+
+__device__ inline float det_expf(float x) {
+#pragma clang fp contract(off)
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return __builtin_inff();
+    if (x < -103.972084045410156f) return 0.0f;
+    const float k = __builtin_rintf(x * 1.44269502162933349609f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860676533018704504e-06f;
+    float p = 1.98412698412698412698e-04f;
+    p = p * r + 1.38888888888888888889e-03f;
+    p = p * r + 8.33333333333333333333e-03f;
+    p = p * r + 4.16666666666666666667e-02f;
+    p = p * r + 1.66666666666666666667e-01f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    const int ki = (int)k;
+    if (ki > 127) return (p * pow2i(127)) * pow2i(ki - 127);
+    if (ki < -126) return (p * pow2i(-126)) * pow2i(ki + 126);
+    return p * pow2i(ki);
+}
+
+__device__ inline float det_logf(float x) {
+#pragma clang fp contract(off)
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    if (x == __builtin_inff()) return x;
+    int e = 0;
+    uint32_t b = __builtin_bit_cast(uint32_t, x);
+    if (b < 0x00800000u) {  // subnormal: scale into the normal range
+        x = x * 16777216.0f;
+        b = __builtin_bit_cast(uint32_t, x);
+        e = -24;
+    }
+    e += (int)(b >> 23) - 127;
+    float m = __builtin_bit_cast(float, (b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984f) { m = m * 0.5f; e += 1; }
+    // fdlibm e_log.c reduction: log(1+f) = f - (hfsq - s (hfsq + R)), s = f/(2+f)
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float R = 1.81818181818181818182e-01f;  // 2/11
+    R = R * z + 2.22222222222222222222e-01f;  // 2/9
+    R = R * z + 2.85714285714285714286e-01f;  // 2/7
+    R = R * z + 0.4f;                          // 2/5
+    R = R * z + 6.66666666666666666667e-01f;  // 2/3
+    R = R * z;
+    const float hfsq = 0.5f * f * f;
+    const float ef = (float)e;
+    return ef * 0.693145751953125f - ((hfsq - (s * (hfsq + R) + ef * 1.42860676533018704504e-06f)) - f);
+}
+
+// ---------------------------------------------------------------------------
+// Encodings.  `in(d)` yields raw input d of the lane's row; `out(c, v)` stores
+// encoded column c.  Formulas follow tiny-cuda-nn v1.6 (vendored in the
+// reference under src/ext/tcnn).
+// ---------------------------------------------------------------------------
+
+// quartic_cdf, tcnn common_device.h:492-497
+__device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
+#pragma clang fp contract(off)
+    const float u = x * inv_radius;
+    const float u2 = u * u;
+    const float u4 = u2 * u2;
+    return fmaxf(0.0f, fminf(1.0f, 0.9375f * u * (1.0f - 0.666666686534881591797f * u2 + 0.200000002980232238770f * u4) + 0.5f));
+}
+
+// grid_index with the CoherentPrime hash, tcnn grid.h:111-128, :164-186.
+// The dense stride is a uint32 as in the reference; when res^2 wraps to 0 the
+// reference keeps the dense formula, and so does this.
+__device__ __forceinline__ uint32_t grid_index3(int type, uint32_t hsize, uint32_t res, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t stride = 1, index = 0;
+    if (stride <= hsize) { index += x * stride; stride *= res; }
+    if (stride <= hsize) { index += y * stride; stride *= res; }
+    if (stride <= hsize) { index += z * stride; stride *= res; }
+    if (type == GRID_HASH && hsize < stride) index = x ^ (y * 2654435761u) ^ (z * 805459861u);
+    return index % hsize;
+}
+
+// pos_fract, tcnn common_device.h:418-431 (identity or smoothstep)
+__device__ __forceinline__ void pos_fract(float in, float scale, int interp, float& frac, uint32_t& cell) {
+#pragma clang fp contract(off)
+    float p = in * scale + 0.5f;
+    const float t = floorf(p);
+    cell = (uint32_t)(int)t;
+    p = p - t;
+    if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0f - 2.0f * p);
+    frac = p;
+}
+
+template <int F>
+struct FeatVec { float v[F]; };
+
+template <int F>
+__device__ __forceinline__ void load_feats(const _Float16* __restrict__ t, uint32_t idx, float (&o)[F]) {
+    if constexpr (F == 1) {
+        o[0] = (float)t[idx];
+    } else if constexpr (F == 2) {
+        h2 v = *(const h2*)(t + idx * 2);
+        o[0] = (float)v[0]; o[1] = (float)v[1];
+    } else if constexpr (F == 4) {
+        h4 v = *(const h4*)(t + idx * 4);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) o[f] = (float)v[f];
+    } else {
+        h8 v = *(const h8*)(t + idx * 8);
+#pragma unroll
+        for (int f = 0; f < 8; ++f) o[f] = (float)v[f];
+    }
+}
+
+// One level of the grid for one row: the 8 corner indices and weights.
+struct GridCorners {
+    uint32_t idx[8];
+    float w[8];
+};
+
+__device__ __forceinline__ void grid_corners(const GridSpec& g, int level, float px, float py, float pz, GridCorners& c) {
+#pragma clang fp contract(off)
+    const float scale = g.scale[level];
+    const uint32_t res = g.res[level];
+    const uint32_t hsize = g.offset[level + 1] - g.offset[level];
+    float fx, fy, fz;
+    uint32_t cx, cy, cz;
+    pos_fract(px, scale, g.interp, fx, cx);
+    pos_fract(py, scale, g.interp, fy, cy);
+    pos_fract(pz, scale, g.interp, fz, cz);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float w = 1.0f;
+        uint32_t x = cx, y = cy, z = cz;
+        if (k & 1) { w *= fx; x += 1; } else { w *= 1.0f - fx; }
+        if (k & 2) { w *= fy; y += 1; } else { w *= 1.0f - fy; }
+        if (k & 4) { w *= fz; z += 1; } else { w *= 1.0f - fz; }
+        c.idx[k] = grid_index3(g.type, hsize, res, x, y, z);
+        c.w[k] = w;
+    }
+}
+
+// Grid level value for one row (N-linear / smoothstep / nearest).
+template <int F>
+__device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __restrict__ table, int level,
+                                           float px, float py, float pz, float (&res)[F]) {
+#pragma clang fp contract(off)
+    const _Float16* lt = table + (size_t)g.offset[level] * F;
+    if (g.interp == INTERP_NEAREST) {
+        const float scale = g.scale[level];
+        const uint32_t hsize = g.offset[level + 1] - g.offset[level];
+        float fx, fy, fz;
+        uint32_t cx, cy, cz;
+        pos_fract(px, scale, INTERP_LINEAR, fx, cx);
+        pos_fract(py, scale, INTERP_LINEAR, fy, cy);
+        pos_fract(pz, scale, INTERP_LINEAR, fz, cz);
+        load_feats<F>(lt, grid_index3(g.type, hsize, g.res[level], cx, cy, cz), res);
+        return;
+    }
+    GridCorners c;
+    grid_corners(g, level, px, py, pz, c);
+    float v[8][F];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) load_feats<F>(lt, c.idx[k], v[k]);
+#pragma unroll
+    for (int f = 0; f < F; ++f) res[f] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int f = 0; f < F; ++f) res[f] += c.w[k] * v[k][f];
+}
+
+// Spherical harmonics up to degree 4 of (2x-1, 2y-1, 2z-1), tcnn
+// spherical_harmonics.h:46-152 (the coefficients are the closed forms in its
+// comments, e.g. 1/(2 sqrt(pi))).
+template <typename Out>
+__device__ __forceinline__ void sh_encode(int degree, float ix, float iy, float iz, Out out) {
+#pragma clang fp contract(off)
+    const float x = ix * 2.f - 1.f, y = iy * 2.f - 1.f, z = iz * 2.f - 1.f;
+    const float xy = x * y, xz = x * z, yz = y * z, x2 = x * x, y2 = y * y, z2 = z * z;
+    out(0, 0.28209479177387814f);
+    if (degree <= 1) return;
+    out(1, -0.48860251190291987f * y);
+    out(2, 0.48860251190291987f * z);
+    out(3, -0.48860251190291987f * x);
+    if (degree <= 2) return;
+    out(4, 1.0925484305920792f * xy);
+    out(5, -1.0925484305920792f * yz);
+    out(6, 0.94617469575755997f * z2 - 0.31539156525251999f);
+    out(7, -1.0925484305920792f * xz);
+    out(8, 0.54627421529603959f * x2 - 0.54627421529603959f * y2);
+    if (degree <= 3) return;
+    out(9, 0.59004358992664352f * y * (-3.0f * x2 + y2));
+    out(10, 2.8906114426405538f * xy * z);
+    out(11, 0.45704579946446572f * y * (1.0f - 5.0f * z2));
+    out(12, 0.3731763325901154f * z * (5.0f * z2 - 3.0f));
+    out(13, 0.45704579946446572f * x * (1.0f - 5.0f * z2));
+    out(14, 1.4453057213202769f * z * (x2 - y2));
+    out(15, 0.59004358992664352f * x * (-x2 + 3.0f * y2));
+}
+
+// Whole composite encoding of one row.  In: callable in(d) -> float.
+// Out: callable out(col, float).  Writes enc_width columns and zero-fills up to
+// 32 * k_steps (the MFMA K padding, which has zero weights).
+template <typename In, typename Out>
+__device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out) {
+#pragma clang fp contract(off)
+    for (int pi = 0; pi < ns.n_parts; ++pi) {
+        const EncPart& P = ns.parts[pi];
+        const int ob = P.out_begin + (P.pad_front ? P.n_pad : 0);
+        const int pb = P.pad_front ? P.out_begin : P.out_begin + P.n_out;
+        for (int j = 0; j < P.n_pad; ++j) out(pb + j, 1.0f);
+        switch (P.type) {
+            case ENC_IDENTITY:
+                for (int d = 0; d < P.n_dims; ++d) out(ob + d, in(P.dim_begin + d) * P.id_scale + P.id_offset);
+                break;
+            case ENC_ONEBLOB: {
+                // kernel_one_blob_soa, tcnn oneblob.h:98-127
+                const int nb = P.param;
+                const float fnb = (float)nb;
+                for (int d = 0; d < P.n_dims; ++d) {
+                    const float x = in(P.dim_begin + d);
+                    float left = quartic_cdf(-x, fnb) + quartic_cdf(-x - 1.0f, fnb) + quartic_cdf(-x + 1.0f, fnb);
+                    for (int k = 0; k < nb; ++k) {
+                        const float rb = (float)(k + 1) / fnb;
+                        const float right = quartic_cdf(rb - x, fnb) + quartic_cdf(rb - x - 1.0f, fnb) + quartic_cdf(rb - x + 1.0f, fnb);
+                        out(ob + d * nb + k, right - left);
+                        left = right;
+                    }
+                }
+                break;
+            }
+            case ENC_SH:
+                sh_encode(P.param, in(P.dim_begin), in(P.dim_begin + 1), in(P.dim_begin + 2),
+                          [&](int j, float v) { out(ob + j, v); });
+                break;
+            case ENC_TRIWAVE: {
+                // triangle_wave_encoding, tcnn triangle_wave.h:46-82
+                const int nf = P.param;
+                for (int d = 0; d < P.n_dims; ++d) {
+                    const float xin = in(P.dim_begin + d);
+                    for (int f = 0; f < nf; ++f) {
+                        const float x = ldexpf(xin, f - 1);
+                        const float val = x + (float)f * 0.25f;
+                        out(ob + d * nf + f, fabsf(val - floorf(val) - 0.5f) * 4.0f - 1.0f);
+                    }
+                }
+                break;
+            }
+            case ENC_FREQUENCY: {
+                // frequency_encoding, tcnn frequency.h:46-94:
+                // sin(2^f pi x + (j%2) pi/2) evaluated in half-turns as
+                // sinpi(2 frac(x 2^(f-1) + (j%2)/4)), an exact argument reduction.
+                const int nf = P.param;
+                for (int d = 0; d < P.n_dims; ++d) {
+                    const float xin = in(P.dim_begin + d);
+                    for (int j = 0; j < 2 * nf; ++j) {
+                        const float v = ldexpf(xin, (j >> 1) - 1) + (float)(j & 1) * 0.25f;
+                        const float fr = v - floorf(v);
+                        out(ob + d * 2 * nf + j, sinpif(2.0f * fr));
+                    }
+                }
+                break;
+            }
+            case ENC_GRID: {
+                const GridSpec& g = ns.grid;
+                const float px = in(P.dim_begin), py = in(P.dim_begin + 1), pz = in(P.dim_begin + 2);
+                const int F = g.n_feat;
+                for (int l = 0; l < g.n_levels; ++l) {
+                    if (F == 2) {
+                        float r[2]; grid_level<2>(g, table, l, px, py, pz, r);
+                        out(ob + l * 2, r[0]); out(ob + l * 2 + 1, r[1]);
+                    } else if (F == 4) {
+                        float r[4]; grid_level<4>(g, table, l, px, py, pz, r);
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) out(ob + l * 4 + f, r[f]);
+                    } else if (F == 1) {
+                        float r[1]; grid_level<1>(g, table, l, px, py, pz, r);
+                        out(ob + l, r[0]);
+                    } else {
+                        float r[8]; grid_level<8>(g, table, l, px, py, pz, r);
+#pragma unroll
+                        for (int f = 0; f < 8; ++f) out(ob + l * 8 + f, r[f]);
+                    }
+                }
+                break;
+            }
+        }
+    }
+    for (int c = ns.enc_width; c < ns.k_steps * 32; ++c) out(c, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// DF head.  `P` is the lane's row of bins in LDS (stride 1).  All follow the
+// reference's operation order with FMA contraction disabled.
+// ---------------------------------------------------------------------------
+
+// softmax2 + softmaxMultFactor_F, train.h:844-893 (fp32 storage): in-place on
+// registers z[N] -> N * softmax(z), with the max initialised to 0.
+template <int N>
+__device__ __forceinline__ void softmax_times_n(float (&z)[N]) {
+#pragma clang fp contract(off)
+    float mx = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) mx = fmaxf(mx, z[i]);
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += det_expf(z[i] - mx);
+    const float ls = det_logf(s);
+#pragma unroll
+    for (int i = 0; i < N; ++i) z[i] = det_expf((z[i] - mx) - ls) * (float)N;
+}
+
+// Training variant: the reference stores the softmax in fp16 and then
+// multiplies by (half)N in half precision (train.h:874-880).
+template <int N>
+__device__ __forceinline__ void softmax_times_n_half(float (&z)[N]) {
+#pragma clang fp contract(off)
+    float mx = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) mx = fmaxf(mx, z[i]);
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += det_expf(z[i] - mx);
+    const float ls = det_logf(s);
+#pragma unroll
+    for (int i = 0; i < N; ++i) z[i] = (float)((_Float16)det_expf((z[i] - mx) - ls) * (_Float16)N);
+}
+
+struct Segment {
+    int i;        // segment index (0..N for linear, 0..N for nearest)
+    float c_lo;   // cdf at the segment start
+    float c_hi;   // cdf at the segment end
+};
+
+// search_sorted_cdf_linearinterpolation, train.h:361-397 — the sequential fp32
+// CDF of the piecewise-linear density; branch-free, same sums in the same order.
+template <int N>
+__device__ __forceinline__ Segment search_linear(const float (&p)[N], float u) {
+#pragma clang fp contract(off)
+    Segment s;
+    float c = p[0] * 0.5f / (float)N;
+    bool found = u < c;
+    s.i = 0; s.c_lo = 0.0f; s.c_hi = c;
+    float prev = c;
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+        c = prev + (p[i - 1] + p[i]) / (float)(2 * N);
+        if (!found && u < c) { found = true; s.i = i; s.c_lo = prev; s.c_hi = c; }
+        prev = c;
+    }
+    if (!found) { s.i = N; s.c_lo = prev; s.c_hi = prev + p[N - 1] * 0.5f / (float)N; }
+    return s;
+}
+
+// search_sorted_cdf_nearest, train.h:345-359
+template <int N>
+__device__ __forceinline__ Segment search_nearest(const float (&p)[N], float u) {
+#pragma clang fp contract(off)
+    Segment s;
+    float prev = 0.0f, c = 0.0f;
+    bool found = false;
+    s.i = N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        c = prev + p[i] / (float)N;
+        if (!found && u < c) { found = true; s.i = i; s.c_lo = prev; s.c_hi = c; }
+        prev = c;
+    }
+    if (!found) { s.c_lo = prev; s.c_hi = c; }
+    return s;
+}
+
+// Inverse CDF inside the found segment, sample_dir_guided_inputs_F,
+// train.h:415-520.  `P` reads bin k (dynamic index).
+template <int N, typename Bin>
+__device__ __forceinline__ float sample_linear(const Segment& sg, float u, float u_extra, Bin P) {
+#pragma clang fp contract(off)
+    const float invN = 1.0f / (float)N;
+    int vi = sg.i;
+    int vim1 = vi - 1;
+    float v = (float)vim1 + 0.5f;
+    float ecm = 1.0f;
+    float voff = 0.0f;
+    if (vim1 < 0) {
+        vim1 = vi; voff = 0.5f; ecm = 0.5f;
+    } else if (vi > N - 1) {
+        vi = vim1; ecm = 0.5f;
+    }
+    const float pa = P(vi), pb = P(vim1);
+    const float m = (pa - pb) * (float)N;
+    const float B = pb * (1.0f + v) - pa * v;
+    v += voff;
+    const float A = m * 0.5f;
+    float x;
+    if (fabsf(A) < DFG_EPS) {
+        const float vd = u - sg.c_lo;
+        const float cd = sg.c_hi - sg.c_lo;
+        const float w = cd > DFG_EPS ? vd / cd : u_extra;
+        x = (v + w * ecm) * invN;
+    } else {
+        const float C = -(m * 0.5f) * (v * v * invN * invN) - (B * v * invN) + sg.c_lo - u;
+        const float delta = B * B - 4.0f * A * C;
+        if (delta < DFG_EPS) {
+            x = (v + u_extra * ecm) * invN;
+        } else {
+            const float sd = __builtin_sqrtf(delta);
+            const float s0 = (-B + sd) / (2.0f * A);
+            const float s1 = (-B - sd) / (2.0f * A);
+            x = A >= 0.0f ? fmaxf(s0, s1) : fminf(s0, s1);
+        }
+    }
+    return fminf(x, 1.0f - DFG_EPS);
+}
+
+template <int N>
+__device__ __forceinline__ float sample_nearest(const Segment& sg, float u, float u_extra) {
+#pragma clang fp contract(off)
+    const float vd = u - sg.c_lo;
+    const float cd = sg.c_hi - sg.c_lo;
+    const float w = cd > DFG_EPS ? vd / cd : u_extra;
+    const float x = ((float)sg.i + w) / (float)N;
+    return fminf(x, 1.0f - DFG_EPS);
+}
+
+// sample_pdf, train.h:654-681 (x is the square coordinate; idx = x * N).
+template <int N, typename Bin>
+__device__ __forceinline__ float pdf_linear(float x, Bin P) {
+#pragma clang fp contract(off)
+    float idx = x * (float)N;
+    idx -= 0.5f;
+    int i0 = min((int)floorf(idx), N - 1);
+    int i1 = i0 + 1;
+    float w = idx - (float)i0;
+    if (i0 < 0) { i0 = N - 1; w = 1.0f; }
+    if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+    return (1.0f - w) * P(i0) + w * P(i1);
+}
+
+template <int N, typename Bin>
+__device__ __forceinline__ float pdf_nearest(float x, Bin P) {
+    int i = min((int)floorf(x * (float)N), N - 1);
+    i = max(i, 0);  // the reference reads out of bounds for x < 0; clamp instead
+    return P(i);
+}
+
+// add_theta_to_input_F, train.h:793-812: the phi net's theta input.
+__device__ __forceinline__ float theta_input(float theta, int n_theta, int linear) {
+#pragma clang fp contract(off)
+    const float t = theta * (float)n_theta;
+    const int fi = min((int)floorf(t), n_theta - 1);
+    if (linear) {
+        const float w = t - (float)fi;
+        return ((float)fi + w) / (float)n_theta;
+    }
+    return (float)fi / (float)n_theta;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA helpers.  v_mfma_f32_16x16x32_f16: A 16x32 (lane l: row l%16, K slots
+// 8 per lane), B 32x16 (lane l: column l%16, the same K slots), D 16x16 (lane
+// l: column l%16, rows 4*(l/16)+r).  A = weights (rows = output neurons),
+// B = activations (columns = rows of the batch), so each lane's D values are
+// 4 neurons of one batch row and can feed the next layer without a transpose:
+// the next layer's weight fragments are packed in the matching K order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ h8 ld_frag(const _Float16* __restrict__ pack, int t, int s, int ks, int lane) {
+    return *(const h8*)(pack + ((size_t)((t * ks + s) * 64 + lane) << 3));
+}
+
+// relu(C) of 4 M-tiles (64 neurons) -> the two K-step operands of the next layer.
+__device__ __forceinline__ void relu_to_act(const f4 (&c)[4], h8 (&a)[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[s][j] = (_Float16)fmaxf(c[2 * s + (j >> 2)][j & 3], 0.0f);
+}
+
+__device__ __forceinline__ void c_to_act(const f4 (&c)[4], h8 (&a)[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[s][j] = (_Float16)c[2 * s + (j >> 2)][j & 3];
+}
+
+// Optional per-layer activation store (training): feature-major [64][ld] fp16.
+struct ActStore {
+    _Float16* x;        // encoded input, [k_steps*32][ld] (NULL: not stored)
+    _Float16* act;      // hidden activations a_1..a_H, [H][64][ld]
+    int64_t ld;         // leading dimension (padded batch)
+    int64_t row0;       // first batch row of this tile
+};
+
+// Forward of a whole MLP for a 64-row tile.  xt: encoded input in LDS,
+// row-major with stride xs halves.  Writes the fp32 output of the last layer to
+// `lo` (LDS, row-major stride ls floats) for columns < n_out_pad.
+template <int NT_OUT>
+__device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
+                                                 const _Float16* xt, int xs, float* lo, int ls,
+                                                 const ActStore* st) {
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, g = lane >> 4;
+    f4 acc[4][4];
+    h8 act[4][2];
+    // layer 0: B operand from the LDS tile, natural K order
+    {
+        const _Float16* pk = fwd_pack + ns.fwd_off[0];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < ns.k_steps; ++s) {
+            h8 b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[q] = *(const h8*)(xt + (q * 16 + i16) * xs + s * 32 + g * 8);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const h8 a = ld_frag(pk, t, s, ns.k_steps, lane);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q][t] = mfma16(a, b[q], acc[q][t]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) relu_to_act(acc[q], act[q]);
+    }
+    if (st) {
+        _Float16* a1 = st->act;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
+                    a1[(size_t)n * st->ld + st->row0 + q * 16 + i16] = act[q][s][j];
+                }
+    }
+    // hidden layers 1..H-1
+    for (int l = 1; l < ns.n_hidden; ++l) {
+        const _Float16* pk = fwd_pack + ns.fwd_off[l];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const h8 a0 = ld_frag(pk, t, 0, 2, lane);
+            const h8 a1 = ld_frag(pk, t, 1, 2, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
+                acc[q][t] = mfma16(a1, act[q][1], c);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) relu_to_act(acc[q], act[q]);
+        if (st) {
+            _Float16* al = st->act + (size_t)l * 64 * st->ld;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
+                        al[(size_t)n * st->ld + st->row0 + q * 16 + i16] = act[q][s][j];
+                    }
+        }
+    }
+    // output layer
+    {
+        const _Float16* pk = fwd_pack + ns.fwd_off[ns.n_hidden];
+#pragma unroll
+        for (int t = 0; t < NT_OUT; ++t) {
+            const h8 a0 = ld_frag(pk, t, 0, 2, lane);
+            const h8 a1 = ld_frag(pk, t, 1, 2, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
+                c = mfma16(a1, act[q][1], c);
+                float* dst = lo + (q * 16 + i16) * ls + 16 * t + 4 * g;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) dst[r] = c[r];
+            }
+        }
+    }
+}
+
+}  // namespace dfg

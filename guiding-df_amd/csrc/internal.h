@@ -1,0 +1,169 @@
+// Internal host-side declarations shared by the kernel translation units and
+// the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "spec.h"
+
+namespace dfg {
+
+// Arguments of the fused query kernel (one launch per wavefront stage).
+struct QueryArgs {
+    const NetSpec* th;         // device-resident specs (see DevSpecs)
+    const NetSpec* ph;
+    const NetSpec* nrc;
+    const _Float16* pack_th;   // inference (EMA) forward fragment packs
+    const _Float16* pack_ph;
+    const _Float16* pack_nrc;
+    const _Float16* tab_th;    // inference (EMA) grid tables, NULL if no grid
+    const _Float16* tab_ph;
+    const _Float16* tab_nrc;
+    const float* feat;
+    int stride;
+    int linear;
+    int64_t n;
+    const float* u;
+    const uint8_t* guided;
+    float* wi;
+    float* rgb;          // NULL: no NRC output
+    const float* wi2;    // pdf mode: given directions [n][2]; NULL otherwise
+    float* pdf_out;      // pdf mode output [n]
+    float* logits_th;    // debug export [n][n_out_th] or NULL
+    float* logits_ph;
+    const float* theta_in;  // debug: phi net theta input [n] (logits mode)
+    int xs;              // LDS strides (halves / floats)
+    int ls;
+};
+
+// Arguments of the training forward kernel of one net.
+struct TrainFwdArgs {
+    const NetSpec* ns;
+    const _Float16* pack;   // training forward fragments
+    const _Float16* tab;    // training grid table or NULL
+    const float* feat;
+    int stride;
+    const float* target;    // [B][9] (phi net: theta input from dir[0]) or NULL
+    int theta_bins;         // phi net: n_theta for the theta input
+    int linear;
+    int mode;               // 0: DF bins (fp16 softmax x N), 1: raw output (NRC)
+    int64_t n;
+    int64_t ld;             // padded batch
+    _Float16* x;            // [k_steps*32][ld]
+    _Float16* act;          // [H][64][ld]
+    _Float16* out;          // row-major [ld][n_out_pad] (bins or prediction)
+    int xs, ls;
+};
+
+void launch_query(const QueryArgs& a, int n_theta, int n_phi, hipStream_t s);
+// Host copies of the specs are passed beside the args where launch shapes depend on them.
+void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s);
+
+struct TrainBwdArgs {
+    const NetSpec* ns;
+    const _Float16* pack;   // training packs (backward part at ns.bwd_off)
+    const _Float16* gout;   // row-major [ld][n_out_pad] fp16
+    const _Float16* act;    // [H][64][ld]
+    _Float16* g;            // [H][64][ld]  (g_1..g_H)
+    float* dx;              // [dx_n][ld] or NULL
+    int dx_begin, dx_n;
+    int64_t n, ld;
+};
+void launch_train_bwd(const TrainBwdArgs& a, const NetSpec& hs, hipStream_t s);
+
+struct WgradArgs {
+    const NetSpec* ns;
+    const _Float16* x;      // [k_steps*32][ld]
+    const _Float16* act;    // [H][64][ld]
+    const _Float16* g;      // [H][64][ld]
+    const _Float16* gout_fm;  // feature-major [n_out_pad][ld]
+    float* partial;         // [chunks][n_matrix_params]
+    float* grad;            // params gradient (matrix part)
+    int64_t n, ld;
+    int chunk;              // rows per chunk (multiple of 32)
+    int n_chunks;
+};
+void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s);
+
+struct DfLossArgs {
+    const float* target;    // [B][9]
+    const _Float16* bins_th;  // [ld][nt_pad]
+    const _Float16* bins_ph;
+    int nt_pad, np_pad;
+    _Float16* gout_th_rm;   // row-major [ld][nt_pad]
+    _Float16* gout_ph_rm;
+    _Float16* gout_th_fm;   // feature-major [nt_pad][ld]
+    _Float16* gout_ph_fm;
+    float* loss;            // [B]
+    int64_t n, ld;
+    float loss_scale;
+    int linear;
+};
+void launch_df_loss(const DfLossArgs& a, int n_theta, int n_phi, hipStream_t s);
+
+struct NrcLossArgs {
+    const _Float16* pred;   // [ld][16]
+    const float* target;    // [B][3]
+    _Float16* gout_rm;      // [ld][16]
+    _Float16* gout_fm;      // [16][ld]
+    float* loss;            // [B]
+    int64_t n, ld;
+    float loss_scale;
+};
+void launch_nrc_loss(const NrcLossArgs& a, hipStream_t s);
+
+struct GridBwdArgs {
+    const NetSpec* ns;
+    const float* feat;
+    int stride;
+    int pos_dim;            // first raw input dim of the grid
+    const float* dx;        // [L*F][ld] dL/d(grid features)
+    float* grad;            // parameter gradient base (grid part at ns.grid.param_begin)
+    int64_t n, ld;
+};
+void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s);
+
+struct AdamArgs {
+    int64_t n_params, n_matrix;
+    float lr, beta1, beta2, eps, l2_reg, loss_scale;
+    float ema_decay, ema_debias_old, ema_debias_new;
+    float* master;
+    _Float16* w16;
+    _Float16* ema16;
+    const float* grad;
+    float* m;
+    float* v;
+    uint32_t* steps;
+};
+void launch_adam_ema(const AdamArgs& a, hipStream_t s);
+
+// Pack fragments from fp16 parameters. which: 1 forward, 2 backward, 3 both.
+void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which, hipStream_t s);
+void launch_f32_to_f16(const float* src, _Float16* dst, int64_t n, hipStream_t s);
+void launch_f16_to_f32(const _Float16* src, float* dst, int64_t n, hipStream_t s);
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s);
+
+// Plain FP32 per-batch path.
+void launch_plain_encode(const NetSpec* dns, const _Float16* tab, const float* feat, int stride,
+                         const float* extra, int64_t n, int64_t ld, float* out, hipStream_t s);
+void launch_plain_layer(const _Float16* W, int M, int K, int K_alloc, const float* X, float* Y,
+                        int relu, int64_t n, int64_t ld, hipStream_t s);
+void launch_plain_head(int phase, int n_bins, int n_theta, int linear, const float* logits,
+                       int64_t ld, int64_t n, const float* u, const uint8_t* guided, float* wi,
+                       float* theta_in, hipStream_t s);
+void launch_plain_rgb(const float* out, int64_t ld, int64_t n, float* rgb, hipStream_t s);
+
+// Stateless DF head.
+void launch_df_head(int64_t n, int n_theta, int n_phi, int linear, const float* lt,
+                    const float* lp, const float* u, const uint8_t* guided, float* wi,
+                    float* bt, float* bp, int32_t* seg, hipStream_t s);
+void launch_prepare_features(int64_t n, const float* pos, const float* aabb, const float* wo,
+                             const float* normal, const float* rough, float* feat, hipStream_t s);
+// Debug encode / grid indices for one net.
+void launch_encode_debug(const NetSpec* dns, int enc_width, const _Float16* tab, const float* in, int stride,
+                         int64_t n, float* out, hipStream_t s);
+void launch_grid_index_debug(const NetSpec* dns, const float* in, int stride, int64_t n,
+                             uint32_t* idx, hipStream_t s);
+
+}  // namespace dfg

@@ -1,0 +1,312 @@
+// Fused guided query: for a 64-row tile per wavefront,
+//   theta encoding -> theta MLP (MFMA) -> theta head (softmax, inverse CDF,
+//   pdf) -> phi encoding with the sampled theta -> phi MLP -> phi head ->
+//   (optional) NRC encoding -> NRC MLP,
+// all in one launch with the activations in registers and the encoded inputs
+// and logits in LDS.  Replaces the ~10 launches and the device-wide sync per
+// bounce of LearnedDistribution::inferenceStep (learnedDistribution.cpp:167-206).
+#include "device.h"
+#include "internal.h"
+
+namespace dfg {
+
+struct RowIn {
+    const float* row;
+    int base_dims;
+    float extra;
+    __device__ float operator()(int d) const { return d < base_dims ? row[d] : extra; }
+};
+
+// Head of one net for the lane's row: reads N logits from the LDS row, writes
+// the bins back (for the dynamic reads of the sampler/pdf), optionally samples.
+template <int N>
+__device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, float u_extra, int linear,
+                                          float x_given, float& pdf_out) {
+    float z[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) z[i] = lrow[i];
+    softmax_times_n<N>(z);
+#pragma unroll
+    for (int i = 0; i < N; ++i) lrow[i] = z[i];
+    auto P = [&](int k) { return lrow[k]; };
+    float x = x_given;
+    if (do_sample) {
+        if (linear) {
+            const Segment sg = search_linear<N>(z, u);
+            x = sample_linear<N>(sg, u, u_extra, P);
+        } else {
+            const Segment sg = search_nearest<N>(z, u);
+            x = sample_nearest<N>(sg, u, u_extra);
+        }
+    }
+    pdf_out = linear ? pdf_linear<N>(x, P) : pdf_nearest<N>(x, P);
+    return x;
+}
+
+template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
+__global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    _Float16* xt = (_Float16*)smem;
+    float* lo = (float*)smem;
+    const int lane = threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = row < a.n;
+    const int64_t r = valid ? row : a.n - 1;
+    const float* frow = a.feat + r * a.stride;
+    _Float16* const xrow = xt + lane * a.xs;
+    auto wr = [xrow](int c, float v) { xrow[c] = (_Float16)v; };
+    float* lrow = lo + lane * a.ls;
+
+  if constexpr (WITH_DF) {
+    const NetSpec& th = *a.th;
+    const NetSpec& ph = *a.ph;
+    const int D = th.in_dims;
+
+    // ---- theta net
+    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr);
+    mlp_forward_tile<NTH / 16>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
+    if (a.logits_th && valid) {
+#pragma unroll
+        for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
+    }
+    const bool guided = a.wi2 == nullptr && a.u != nullptr && (a.guided == nullptr || a.guided[r] != 0);
+    float4 w = make_float4(0.f, 0.f, 1.f, 1.f);
+    if (a.wi2) {
+        w.x = a.wi2[2 * r];
+        w.y = a.wi2[2 * r + 1];
+    } else if (a.wi) {
+        w = *(const float4*)(a.wi + 4 * r);
+    }
+    float4 uu = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (guided) uu = *(const float4*)(a.u + 4 * r);
+    float p_th;
+    const float u0 = head_row<NTH>(lrow, guided, uu.x, uu.z, a.linear, w.x, p_th);
+    const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
+
+    // ---- phi net (its own encoding, theta appended as the last raw input)
+    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr);
+    mlp_forward_tile<NPH / 16>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
+    if (a.logits_ph && valid) {
+#pragma unroll
+        for (int i = 0; i < NPH; ++i) a.logits_ph[r * NPH + i] = lrow[i];
+    }
+    float p_ph;
+    const float u1 = head_row<NPH>(lrow, guided, uu.y, uu.w, a.linear, w.y, p_ph);
+    if (valid) {
+        if (a.wi2) {
+            // AutoRegressive::pdf, AR.h:41-43
+            a.pdf_out[r] = fabsf(p_th) * fabsf(p_ph) * DFG_INV_4PI;
+        } else if (a.wi) {
+            *(float4*)(a.wi + 4 * r) = make_float4(u0, u1, w.z * p_th, w.w * p_ph);
+        }
+    }
+  }
+
+    // ---- radiance cache
+    if constexpr (WITH_NRC) {
+        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, wr);
+        mlp_forward_tile<1>(*a.nrc, a.pack_nrc, xt, a.xs, lo, a.ls, nullptr);
+        if (valid) {
+            a.rgb[3 * r + 0] = lrow[0];
+            a.rgb[3 * r + 1] = lrow[1];
+            a.rgb[3 * r + 2] = lrow[2];
+        }
+    }
+}
+
+template <int NTH, int NPH>
+static void launch_q(const QueryArgs& a, hipStream_t s) {
+    const unsigned blocks = (unsigned)((a.n + 63) / 64);
+    size_t lds = (size_t)64 * a.xs * 2;
+    const size_t l2 = (size_t)64 * a.ls * 4;
+    if (l2 > lds) lds = l2;
+    if (a.th == nullptr)
+        hipLaunchKernelGGL((query_fused_kernel<16, 16, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else if (a.rgb)
+        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, true>), dim3(blocks), dim3(64), lds, s, a);
+    else
+        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, false>), dim3(blocks), dim3(64), lds, s, a);
+}
+
+void launch_query(const QueryArgs& a, int nt, int np, hipStream_t s) {
+    if (a.n <= 0) return;
+    if (a.th == nullptr) return launch_q<16, 16>(a, s);  // radiance cache only
+    if (nt == 16 && np == 32) return launch_q<16, 32>(a, s);
+    if (nt == 32 && np == 32) return launch_q<32, 32>(a, s);
+    if (nt == 64 && np == 64) return launch_q<64, 64>(a, s);
+    if (nt == 16 && np == 16) return launch_q<16, 16>(a, s);
+    if (nt == 32 && np == 64) return launch_q<32, 64>(a, s);
+}
+
+// ---------------------------------------------------------------------------
+// Stateless DF head on given logits (dfg_df_head).
+// ---------------------------------------------------------------------------
+template <int NTH, int NPH>
+__global__ __launch_bounds__(64) void df_head_kernel(int64_t n, int linear, const float* __restrict__ lt,
+                                                     const float* __restrict__ lp, const float* __restrict__ u,
+                                                     const uint8_t* __restrict__ guided, float* __restrict__ wi,
+                                                     float* __restrict__ bt, float* __restrict__ bp,
+                                                     int32_t* __restrict__ seg) {
+    __shared__ float srow[64][(NTH > NPH ? NTH : NPH) + 1];
+    const int lane = threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * 64 + lane;
+    if (row >= n) return;
+    float* P = srow[lane];
+    const bool g = guided == nullptr || guided[row] != 0;
+    float4 w = *(const float4*)(wi + 4 * row);
+    const float4 uu = *(const float4*)(u + 4 * row);
+    float xs[2] = {w.x, w.y};
+    float pd[2];
+    int sgi[2] = {-1, -1};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        constexpr int NA = NTH, NB = NPH;
+        const int N = h == 0 ? NA : NB;
+        const float uv = h == 0 ? uu.x : uu.y;
+        const float ue = h == 0 ? uu.z : uu.w;
+        auto Pf = [&](int k) { return P[k]; };
+        if (h == 0) {
+            float z[NA];
+#pragma unroll
+            for (int i = 0; i < NA; ++i) z[i] = lt[row * NA + i];
+            softmax_times_n<NA>(z);
+#pragma unroll
+            for (int i = 0; i < NA; ++i) P[i] = z[i];
+            if (bt) {
+#pragma unroll
+                for (int i = 0; i < NA; ++i) bt[row * NA + i] = z[i];
+            }
+            if (g) {
+                if (linear) { Segment sg = search_linear<NA>(z, uv); sgi[0] = sg.i; xs[0] = sample_linear<NA>(sg, uv, ue, Pf); }
+                else { Segment sg = search_nearest<NA>(z, uv); sgi[0] = sg.i; xs[0] = sample_nearest<NA>(sg, uv, ue); }
+            }
+            pd[0] = linear ? pdf_linear<NA>(xs[0], Pf) : pdf_nearest<NA>(xs[0], Pf);
+        } else {
+            float z[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) z[i] = lp[row * NB + i];
+            softmax_times_n<NB>(z);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) P[i] = z[i];
+            if (bp) {
+#pragma unroll
+                for (int i = 0; i < NB; ++i) bp[row * NB + i] = z[i];
+            }
+            if (g) {
+                if (linear) { Segment sg = search_linear<NB>(z, uv); sgi[1] = sg.i; xs[1] = sample_linear<NB>(sg, uv, ue, Pf); }
+                else { Segment sg = search_nearest<NB>(z, uv); sgi[1] = sg.i; xs[1] = sample_nearest<NB>(sg, uv, ue); }
+            }
+            pd[1] = linear ? pdf_linear<NB>(xs[1], Pf) : pdf_nearest<NB>(xs[1], Pf);
+        }
+        (void)N;
+    }
+    *(float4*)(wi + 4 * row) = make_float4(xs[0], xs[1], w.z * pd[0], w.w * pd[1]);
+    if (seg) { seg[2 * row] = sgi[0]; seg[2 * row + 1] = sgi[1]; }
+}
+
+void launch_df_head(int64_t n, int nt, int np, int linear, const float* lt, const float* lp, const float* u,
+                    const uint8_t* guided, float* wi, float* bt, float* bp, int32_t* seg, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+#define DFG_HEAD(A, B)                                                                                       \
+    if (nt == A && np == B) {                                                                                \
+        hipLaunchKernelGGL((df_head_kernel<A, B>), dim3(blocks), dim3(64), 0, s, n, linear, lt, lp, u, guided, \
+                           wi, bt, bp, seg);                                                                 \
+        return;                                                                                              \
+    }
+    DFG_HEAD(16, 16) DFG_HEAD(16, 32) DFG_HEAD(32, 32) DFG_HEAD(32, 64) DFG_HEAD(64, 64) DFG_HEAD(64, 32)
+    DFG_HEAD(16, 64) DFG_HEAD(32, 16) DFG_HEAD(64, 16)
+#undef DFG_HEAD
+}
+
+// ---------------------------------------------------------------------------
+// generate_inference_data_AR (train.cu:334-362): shading point -> DF features.
+// ---------------------------------------------------------------------------
+__global__ void prepare_features_kernel(int64_t n, const float* __restrict__ pos, const float* __restrict__ aabb,
+                                        const float* __restrict__ wo, const float* __restrict__ nrm,
+                                        const float* __restrict__ rough, float* __restrict__ feat) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // normalizeSpatialCoord (train.h:272-276): inflate the box by 0.5% of its diagonal
+    float mn[3], mx[3], d[3];
+    for (int k = 0; k < 3; ++k) { mn[k] = aabb[k]; mx[k] = aabb[3 + k]; d[k] = mx[k] - mn[k]; }
+    const float infl = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * 0.005f;
+    float* f = feat + i * 9;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = mn[k] - infl, hi = mx[k] + infl;
+        const float c = (lo + hi) * 0.5f;
+        f[k] = 0.5f + (pos[i * 3 + k] - c) / (hi - lo);
+    }
+    // warp_direction_for_sh (train.h:1243-1245) of the normalised wo
+    const float wx = wo[i * 3], wy = wo[i * 3 + 1], wz = wo[i * 3 + 2];
+    const float wl = sqrtf(wx * wx + wy * wy + wz * wz);
+    f[3] = (wx / wl + 1.0f) * 0.5f;
+    f[4] = (wy / wl + 1.0f) * 0.5f;
+    f[5] = (wz / wl + 1.0f) * 0.5f;
+    // cartesianToSphericalNormalized (math_utils.h:183-195)
+    const float nx = nrm[i * 3], ny = nrm[i * 3 + 1], nz = nrm[i * 3 + 2];
+    const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+    const float cz = fminf(fmaxf(nz / nl, -1.0f), 1.0f);
+    float ph = atan2f(ny / nl, nx / nl);
+    if (ph < 0.0f) ph += 6.28318530717958647693f;
+    f[6] = acosf(cz) / 3.14159265358979323846f;
+    f[7] = ph / 6.28318530717958647693f;
+    // warp_roughness_for_ob (train.h:1247-1249)
+    f[8] = 1.0f - expf(-rough[i]);
+}
+
+void launch_prepare_features(int64_t n, const float* pos, const float* aabb, const float* wo, const float* normal,
+                             const float* rough, float* feat, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(prepare_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, pos, aabb, wo,
+                       normal, rough, feat);
+}
+
+// ---------------------------------------------------------------------------
+// Debug exports of one net's encoding.
+// ---------------------------------------------------------------------------
+__global__ void encode_debug_kernel(const NetSpec* __restrict__ dns, const _Float16* tab, const float* in, int stride,
+                                    int64_t n, float* out) {
+    const NetSpec& ns = *dns;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* row = in + i * stride;
+    float* o = out + i * ns.enc_width;
+    encode_row(ns, tab, RowIn{row, 64, 0.0f}, [&](int c, float v) {
+        if (c < ns.enc_width) o[c] = v;
+    });
+}
+
+void launch_encode_debug(const NetSpec* dns, int enc_width, const _Float16* tab, const float* in, int stride,
+                         int64_t n, float* out, hipStream_t s) {
+    (void)enc_width;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(encode_debug_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dns, tab, in, stride, n,
+                       out);
+}
+
+__global__ void grid_index_debug_kernel(const NetSpec* __restrict__ dns, const float* in, int stride, int64_t n,
+                                        uint32_t* idx) {
+    const NetSpec& ns = *dns;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GridSpec& g = ns.grid;
+    int pd = 0;
+    for (int p = 0; p < ns.n_parts; ++p)
+        if (ns.parts[p].type == ENC_GRID) pd = ns.parts[p].dim_begin;
+    const float* row = in + i * stride;
+    for (int l = 0; l < g.n_levels; ++l) {
+        GridCorners c;
+        grid_corners(g, l, row[pd], row[pd + 1], row[pd + 2], c);
+        for (int k = 0; k < 8; ++k) idx[(i * g.n_levels + l) * 8 + k] = c.idx[k];
+    }
+}
+
+void launch_grid_index_debug(const NetSpec* dns, const float* in, int stride, int64_t n, uint32_t* idx,
+                             hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(grid_index_debug_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dns, in, stride, n,
+                       idx);
+}
+
+}  // namespace dfg

@@ -1,0 +1,559 @@
+// Training step kernels: forward with stored activations, DF / NRC loss heads,
+// MFMA backward (data gradients), MFMA weight gradients with a deterministic
+// chunked reduction, grid scatter-add, fused Adam + EMA, fragment packing.
+// Semantics: LearnedDistribution::trainStep (learnedDistribution.cpp:230-290),
+// tcnn fully_fused_mlp.cu:754-854, grid.h:394-520, adam.h:47-120, ema.h:62-77.
+#include "device.h"
+#include "internal.h"
+
+namespace dfg {
+
+struct RowInT {
+    const float* row;
+    int base_dims;
+    float extra;
+    __device__ float operator()(int d) const { return d < base_dims ? row[d] : extra; }
+};
+
+// ---------------------------------------------------------------------------
+// Forward of one net for a 64-row tile, storing the encoded input (x) and the
+// hidden activations feature-major, and the output row-major as fp16
+// (DF: softmax x N stored in half like softmaxMultFactor, train.h:874-893;
+//  NRC: the raw network output).
+// ---------------------------------------------------------------------------
+template <int NT_OUT, int NBINS>
+__global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    _Float16* xt = (_Float16*)smem;
+    float* lo = (float*)smem;
+    const NetSpec& ns = *a.ns;
+    const int lane = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * 64;
+    const int64_t row = row0 + lane;
+    const int64_t r = row < a.n ? row : a.n - 1;
+    const float* frow = a.feat + r * a.stride;
+    float th_in = 0.0f;
+    int base = ns.in_dims;
+    if (a.target) {
+        // generate_training_data_AR, train.cu:52-60: phi input theta from the record's direction
+        th_in = theta_input(a.target[r * 9], a.theta_bins, a.linear);
+        base = ns.in_dims - 1;
+    }
+    _Float16* const xrow = xt + lane * a.xs;
+    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (_Float16)v; });
+    const int kc = ns.k_steps * 32;
+    for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = xrow[c];
+    ActStore st{nullptr, a.act, a.ld, row0};
+    mlp_forward_tile<NT_OUT>(ns, a.pack, xt, a.xs, lo, a.ls, &st);
+    const float* lrow = lo + lane * a.ls;
+    _Float16* o = a.out + row * (NT_OUT * 16);
+    if (a.mode == 0) {
+        float z[NBINS];
+#pragma unroll
+        for (int i = 0; i < NBINS; ++i) z[i] = lrow[i];
+        softmax_times_n_half<NBINS>(z);
+#pragma unroll
+        for (int i = 0; i < NBINS; ++i) o[i] = (_Float16)z[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NT_OUT * 16; ++i) o[i] = (_Float16)lrow[i];
+    }
+}
+
+void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s) {
+    if (a.n <= 0) return;
+    const unsigned blocks = (unsigned)(a.ld / 64);
+    size_t lds = (size_t)64 * a.xs * 2;
+    const size_t l2 = (size_t)64 * a.ls * 4;
+    if (l2 > lds) lds = l2;
+    const int nt = hs.n_out_pad / 16;
+    if (a.mode == 1) {
+        hipLaunchKernelGGL((train_fwd_kernel<1, 16>), dim3(blocks), dim3(64), lds, s, a);
+        return;
+    }
+    if (nt == 1) hipLaunchKernelGGL((train_fwd_kernel<1, 16>), dim3(blocks), dim3(64), lds, s, a);
+    else if (nt == 2) hipLaunchKernelGGL((train_fwd_kernel<2, 32>), dim3(blocks), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((train_fwd_kernel<4, 64>), dim3(blocks), dim3(64), lds, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// DF loss head: combine_AR_pdf_and_dl_doutput (train.h:931-965) with the
+// hand-written softmax Jacobian of AR.h:50-104.
+// ---------------------------------------------------------------------------
+template <int N, int NPAD>
+__device__ __forceinline__ void df_grad(float x, float pdf_other, float prefix, const _Float16* __restrict__ bins,
+                                        int linear, _Float16* rm, _Float16* fm, int64_t ld, int64_t row) {
+#pragma clang fp contract(off)
+    const float inv_res = 1.0f / (float)N;
+    const float multiplier = prefix * pdf_other * (float)N;
+    float g[N];
+    if (linear) {
+        const float idx = x * (float)N - 0.5f;
+        int i0 = min((int)floorf(idx), N - 1);
+        int i1 = i0 + 1;
+        float w = idx - (float)i0;
+        if (i0 < 0) { i0 = N - 1; w = 1.0f; }
+        if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+        const float p0 = (float)bins[i0] * inv_res;
+        const float p1 = (float)bins[i1] * inv_res;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float pi = (float)bins[i] * inv_res;
+            const float j0 = i0 == i ? p0 * (1.0f - p0) : -p0 * pi;
+            const float j1 = i1 == i ? p1 * (1.0f - p1) : -p1 * pi;
+            const float j = (1.0f - w) * j0 + w * j1;
+            g[i] = multiplier * j;
+        }
+    } else {
+        int i0 = min((int)(x * (float)N), N - 1);
+        i0 = max(i0, 0);
+        const float p0 = (float)bins[i0] * inv_res;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float pi = (float)bins[i] * inv_res;
+            g[i] = multiplier * (i0 == i ? p0 * (1.0f - p0) : -p0 * pi);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NPAD; ++i) {
+        const _Float16 h = i < N ? (_Float16)g[i < N ? i : 0] : (_Float16)0.0f;
+        rm[row * NPAD + i] = h;
+        fm[(size_t)i * ld + row] = h;
+    }
+}
+
+template <int NTH, int NPH>
+__global__ void df_loss_kernel(const DfLossArgs a) {
+#pragma clang fp contract(off)
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.ld) return;
+    if (row >= a.n) {
+        for (int i = 0; i < NTH; ++i) { a.gout_th_rm[row * NTH + i] = (_Float16)0.0f; a.gout_th_fm[(size_t)i * a.ld + row] = (_Float16)0.0f; }
+        for (int i = 0; i < NPH; ++i) { a.gout_ph_rm[row * NPH + i] = (_Float16)0.0f; a.gout_ph_fm[(size_t)i * a.ld + row] = (_Float16)0.0f; }
+        return;
+    }
+    const float* t = a.target + row * 9;
+    const _Float16* bt = a.bins_th + row * NTH;
+    const _Float16* bp = a.bins_ph + row * NPH;
+    auto Pt = [&](int k) { return (float)bt[k]; };
+    auto Pp = [&](int k) { return (float)bp[k]; };
+    const float x0 = t[0], x1 = t[1];
+    const float pt = a.linear ? pdf_linear<NTH>(x0, Pt) : pdf_nearest<NTH>(x0, Pt);
+    const float pp = a.linear ? pdf_linear<NPH>(x1, Pp) : pdf_nearest<NPH>(x1, Pp);
+    // AutoRegressive(guidingData): mPdfTheta = |p_theta|, mPdfPhi = |p_phi| (AR.h:25-35)
+    const float apt = fabsf(pt), app = fabsf(pp);
+    const float Li = (t[2] + t[3] + t[4]) / 3.0f;
+    const float wiPdf = t[5] + DFG_EPS;
+    const float guided = ((apt * app) * DFG_INV_4PI) * (1.0f / DFG_INV_4PI) + DFG_EPS;
+    const float ls = a.loss_scale / (float)a.n;
+    const float prefix = -Li / wiPdf / guided * ls;
+    a.loss[row] = -Li / wiPdf * det_logf(guided);
+    df_grad<NTH, NTH>(x0, app, prefix, bt, a.linear, a.gout_th_rm, a.gout_th_fm, a.ld, row);
+    df_grad<NPH, NPH>(x1, apt, prefix, bp, a.linear, a.gout_ph_rm, a.gout_ph_fm, a.ld, row);
+}
+
+void launch_df_loss(const DfLossArgs& a, int nt, int np, hipStream_t s) {
+    if (a.n <= 0) return;
+    const unsigned blocks = (unsigned)((a.ld + 255) / 256);
+#define DFG_LOSS(A, B)                                                                    \
+    if (nt == A && np == B) {                                                             \
+        hipLaunchKernelGGL((df_loss_kernel<A, B>), dim3(blocks), dim3(256), 0, s, a);     \
+        return;                                                                           \
+    }
+    DFG_LOSS(16, 32) DFG_LOSS(32, 32) DFG_LOSS(64, 64) DFG_LOSS(16, 16) DFG_LOSS(32, 64)
+#undef DFG_LOSS
+}
+
+// relative_l2_luminance_loss, tcnn relative_l2_luminance.h:39-88 (stride 16, dims 3)
+__global__ void nrc_loss_kernel(const NrcLossArgs a) {
+#pragma clang fp contract(off)
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.ld) return;
+    if (row >= a.n) {
+        for (int c = 0; c < 16; ++c) { a.gout_rm[row * 16 + c] = (_Float16)0.0f; a.gout_fm[(size_t)c * a.ld + row] = (_Float16)0.0f; }
+        return;
+    }
+    const _Float16* p = a.pred + row * 16;
+    const float r = (float)p[0], g = (float)p[1], b = (float)p[2];
+    const float lum = 0.299f * r + 0.587f * g + 0.114f * b;
+    const float den = lum * lum + 0.01f;
+    const float n_total = (float)(a.n * 3);
+    float lsum = 0.0f;
+    for (int c = 0; c < 16; ++c) {
+        _Float16 gh = (_Float16)0.0f;
+        if (c < 3) {
+            const float diff = (float)p[c] - a.target[row * 3 + c];
+            lsum += diff * diff / den / 1.0f / n_total;
+            const float grad = 2.0f * diff / den / 1.0f;
+            gh = (_Float16)(a.loss_scale * grad / n_total);
+        }
+        a.gout_rm[row * 16 + c] = gh;
+        a.gout_fm[(size_t)c * a.ld + row] = gh;
+    }
+    a.loss[row] = lsum;
+}
+
+void launch_nrc_loss(const NrcLossArgs& a, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(nrc_loss_kernel, dim3((unsigned)((a.ld + 255) / 256)), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Backward data gradients for a 64-row tile (kernel_mlp_fused_backward,
+// fully_fused_mlp.cu:150-259, plus the last-layer and input-gradient GEMMs).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mask_store(f4 (&c)[4], const _Float16* __restrict__ act, _Float16* __restrict__ gst,
+                                           int64_t ld, int64_t row, int g) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = 16 * t + 4 * g + r;
+            const float av = (float)act[(size_t)n * ld + row];
+            const float v = av > 0.0f ? c[t][r] : 0.0f;
+            c[t][r] = v;
+            gst[(size_t)n * ld + row] = (_Float16)v;
+        }
+}
+
+template <int KS_OUT>
+__global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
+    const NetSpec& ns = *a.ns;
+    const int lane = threadIdx.x;
+    const int i16 = lane & 15, g = lane >> 4;
+    const int64_t row0 = (int64_t)blockIdx.x * 64;
+    const int H = ns.n_hidden;
+    const int npad = ns.n_out_pad;
+    const size_t lay = (size_t)64 * a.ld;
+    f4 acc[4][4];
+    h8 act[4][2];
+    // dL/da_H = W_H^T g_out (K = outputs, natural order)
+    {
+        const _Float16* pk = a.pack + ns.bwd_off[H];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS_OUT; ++s) {
+            h8 b[4];
+            const int col = s * 32 + g * 8;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (col < npad) b[q] = *(const h8*)(a.gout + (row0 + q * 16 + i16) * npad + col);
+                else b[q] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const h8 w = ld_frag(pk, t, s, KS_OUT, lane);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q][t] = mfma16(w, b[q], acc[q][t]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            mask_store(acc[q], a.act + (size_t)(H - 1) * lay, a.g + (size_t)(H - 1) * lay, a.ld, row0 + q * 16 + i16, g);
+            c_to_act(acc[q], act[q]);
+        }
+    }
+    // hidden layers
+    for (int l = H - 1; l >= 1; --l) {
+        const _Float16* pk = a.pack + ns.bwd_off[l];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const h8 w0 = ld_frag(pk, t, 0, 2, lane);
+            const h8 w1 = ld_frag(pk, t, 1, 2, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f4 c = mfma16(w0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
+                acc[q][t] = mfma16(w1, act[q][1], c);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            mask_store(acc[q], a.act + (size_t)(l - 1) * lay, a.g + (size_t)(l - 1) * lay, a.ld, row0 + q * 16 + i16, g);
+            c_to_act(acc[q], act[q]);
+        }
+    }
+    // dL/dx = W_0^T g_1, only the requested input columns (the grid features)
+    if (a.dx) {
+        const _Float16* pk = a.pack + ns.bwd_off[0];
+        const int t_lo = a.dx_begin / 16;
+        const int t_hi = (a.dx_begin + a.dx_n + 15) / 16;
+        for (int t = t_lo; t < t_hi; ++t) {
+            const h8 w0 = ld_frag(pk, t, 0, 2, lane);
+            const h8 w1 = ld_frag(pk, t, 1, 2, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                f4 c = mfma16(w0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
+                c = mfma16(w1, act[q][1], c);
+                const int64_t row = row0 + q * 16 + i16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * t + 4 * g + r - a.dx_begin;
+                    if (col >= 0 && col < a.dx_n) a.dx[(size_t)col * a.ld + row] = c[r];
+                }
+            }
+        }
+    }
+}
+
+void launch_train_bwd(const TrainBwdArgs& a, const NetSpec& hs, hipStream_t s) {
+    if (a.n <= 0) return;
+    const unsigned blocks = (unsigned)(a.ld / 64);
+    const int ks = (hs.n_out_pad + 31) / 32;
+    if (ks == 1) hipLaunchKernelGGL((train_bwd_kernel<1>), dim3(blocks), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((train_bwd_kernel<2>), dim3(blocks), dim3(64), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradients dW_l = G_{l+1} A_l^T summed over the batch, on MFMA with the
+// batch as the K dimension.  Block (chunk c, layer l): wave w owns output rows
+// 16w..16w+15 and every column tile; partial sums go to partial[c] and are
+// reduced in chunk order, so the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void wgrad_kernel(const WgradArgs a) {
+    const NetSpec& ns = *a.ns;
+    const int l = blockIdx.y;
+    const int c = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, g = lane >> 4;
+    const int H = ns.n_hidden;
+    const int M = l == H ? ns.n_out_pad : 64;
+    const int N = l == 0 ? ns.enc_width : 64;
+    if (wave * 16 >= M) return;
+    const size_t lay = (size_t)64 * a.ld;
+    const _Float16* G = l == H ? a.gout_fm : a.g + (size_t)l * lay;        // g_{l+1}
+    const _Float16* A = l == 0 ? a.x : a.act + (size_t)(l - 1) * lay;      // a_l
+    const int nt_n = N / 16;
+    f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const int64_t k0 = (int64_t)c * a.chunk;
+    int64_t k1 = k0 + a.chunk;
+    if (k1 > a.ld) k1 = a.ld;
+    const _Float16* gr = G + (size_t)(wave * 16 + i16) * a.ld;
+    for (int64_t kb = k0; kb < k1; kb += 32) {
+        const h8 af = *(const h8*)(gr + kb + g * 8);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (t < nt_n) {
+                const h8 bf = *(const h8*)(A + (size_t)(t * 16 + i16) * a.ld + kb + g * 8);
+                acc[t] = mfma16(af, bf, acc[t]);
+            }
+        }
+    }
+    float* out = a.partial + (size_t)c * ns.n_matrix_params + ns.layer_off[l];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        if (t < nt_n) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = wave * 16 + 4 * g + r;
+                const int n = t * 16 + i16;
+                out[(size_t)m * N + n] = acc[t][r];
+            }
+        }
+    }
+}
+
+__global__ void wgrad_reduce_kernel(const float* __restrict__ partial, int n_chunks, int64_t n_matrix,
+                                    float* __restrict__ grad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_matrix) return;
+    float s = 0.0f;
+    for (int c = 0; c < n_chunks; ++c) s += partial[(size_t)c * n_matrix + i];
+    grad[i] = s;
+}
+
+void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(wgrad_kernel, dim3((unsigned)a.n_chunks, (unsigned)(hs.n_hidden + 1)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((hs.n_matrix_params + 255) / 256)), dim3(256), 0, s,
+                       a.partial, a.n_chunks, (int64_t)hs.n_matrix_params, a.grad);
+}
+
+// ---------------------------------------------------------------------------
+// Grid backward: scatter w * dL/dy into the table gradient (kernel_grid_backward,
+// grid.h:394-520), fp32 atomics.
+// ---------------------------------------------------------------------------
+__global__ void grid_bwd_kernel(const GridBwdArgs a) {
+    const NetSpec& ns = *a.ns;
+    const GridSpec& gs = ns.grid;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int level = blockIdx.y;
+    if (row >= a.n) return;
+    const float* fr = a.feat + row * a.stride + a.pos_dim;
+    const int F = gs.n_feat;
+    float* gg = a.grad + gs.param_begin + (size_t)gs.offset[level] * F;
+    float d[8];
+    for (int f = 0; f < F; ++f) d[f] = a.dx[(size_t)(level * F + f) * a.ld + row];
+    if (gs.interp == INTERP_NEAREST) {
+        float fx, fy, fz;
+        uint32_t cx, cy, cz;
+        pos_fract(fr[0], gs.scale[level], INTERP_LINEAR, fx, cx);
+        pos_fract(fr[1], gs.scale[level], INTERP_LINEAR, fy, cy);
+        pos_fract(fr[2], gs.scale[level], INTERP_LINEAR, fz, cz);
+        const uint32_t idx = grid_index3(gs.type, gs.offset[level + 1] - gs.offset[level], gs.res[level], cx, cy, cz);
+        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)idx * F + f, d[f]);
+        return;
+    }
+    GridCorners c;
+    grid_corners(gs, level, fr[0], fr[1], fr[2], c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[k] * F + f, c.w[k] * d[f]);
+}
+
+void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s) {
+    if (a.n <= 0 || !hs.grid.present) return;
+    hipLaunchKernelGGL(grid_bwd_kernel, dim3((unsigned)((a.n + 255) / 256), (unsigned)hs.grid.n_levels), dim3(256), 0,
+                       s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Adam (adam.h:47-120) fused with the half-precision EMA (ema.h:62-77).
+// ---------------------------------------------------------------------------
+__global__ void adam_ema_kernel(const AdamArgs a) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_params) return;
+    float gr = a.grad[i] / a.loss_scale;
+    const bool matrix = i < a.n_matrix;
+    if (matrix || gr != 0.0f) {
+        const float wf = a.master[i];
+        if (matrix) gr += a.l2_reg * wf;
+        const float gsq = gr * gr;
+        const float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * gr;
+        const float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * gsq;
+        a.m[i] = m;
+        a.v[i] = v;
+        const uint32_t step = a.steps[i] + 1;
+        a.steps[i] = step;
+        float lr = a.lr;
+        lr *= sqrtf(1.0f - powf(a.beta2, (float)step)) / (1.0f - powf(a.beta1, (float)step));
+        const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
+        const float nw = wf - elr * m;
+        a.master[i] = nw;
+        a.w16[i] = (_Float16)nw;
+    }
+    const float e = (float)a.ema16[i];
+    const float w = (float)a.w16[i];
+    a.ema16[i] = (_Float16)((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
+}
+
+void launch_adam_ema(const AdamArgs& a, hipStream_t s) {
+    if (a.n_params <= 0) return;
+    hipLaunchKernelGGL(adam_ema_kernel, dim3((unsigned)((a.n_params + 255) / 256)), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Fragment packing.  Forward layer l holds W_l (rows = outputs); backward
+// layer l holds W_l^T (rows = inputs).  K order: natural (k = 32s + 8g + j)
+// when the operand comes from memory, permuted
+// (k = 16(2s + j/4) + 4g + j%4) when it comes from the previous layer's
+// accumulators.
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(const NetSpec* __restrict__ dns, const _Float16* __restrict__ w16,
+                            _Float16* __restrict__ pack, int which) {
+    const NetSpec& ns = *dns;
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one h8 per thread
+    const int L = ns.n_hidden + 1;
+    const uint32_t h = (uint32_t)slot * 8;
+    const uint32_t end = (which & 2) ? ns.bwd_off[L] : ns.fwd_off[L];
+    const uint32_t begin = (which & 1) ? 0u : ns.bwd_off[0];
+    if (h + begin >= end) return;
+    const uint32_t pos = h + begin;
+    bool bwd = pos >= ns.bwd_off[0];
+    int l = 0;
+    const uint32_t* off = bwd ? ns.bwd_off : ns.fwd_off;
+    while (l + 1 < L && pos >= off[l + 1]) ++l;
+    // logical matrix of this pack: rows x cols, value(m, k)
+    const int H = ns.n_hidden;
+    const int out_l = l == H ? ns.n_out_pad : 64;
+    const int in_l = l == 0 ? ns.enc_width : 64;
+    int ks, natural;
+    if (!bwd) {
+        ks = l == 0 ? ns.k_steps : 2;
+        natural = l == 0;
+    } else {
+        ks = l == H ? (ns.n_out_pad + 31) / 32 : 2;
+        natural = l == H;
+    }
+    const uint32_t local = (pos - off[l]) / 8;   // frag*64 + lane
+    const int lane = local & 63;
+    const int frag = local >> 6;
+    const int t = frag / ks, s = frag % ks;
+    const int m = 16 * t + (lane & 15);
+    const int gq = lane >> 4;
+    const _Float16* W = w16 + ns.layer_off[l];
+    h8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = natural ? 32 * s + 8 * gq + j : 16 * (2 * s + (j >> 2)) + 4 * gq + (j & 3);
+        _Float16 val = (_Float16)0.0f;
+        if (!bwd) {
+            if (m < out_l && k < in_l) val = W[(size_t)m * in_l + k];
+        } else {
+            if (m < in_l && k < out_l) val = W[(size_t)k * in_l + m];
+        }
+        v[j] = val;
+    }
+    *(h8*)(pack + pos) = v;
+}
+
+void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which,
+                 hipStream_t s) {
+    const int L = hs.n_hidden + 1;
+    const uint32_t end = (which & 2) ? hs.bwd_off[L] : hs.fwd_off[L];
+    const uint32_t begin = (which & 1) ? 0u : hs.bwd_off[0];
+    const int64_t slots = (int64_t)(end - begin) / 8;
+    if (slots <= 0) return;
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, dns, w16, pack, which);
+}
+
+// ---------------------------------------------------------------------------
+// Small utilities.
+// ---------------------------------------------------------------------------
+__global__ void f32_to_f16_kernel(const float* __restrict__ a, _Float16* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (_Float16)a[i];
+}
+__global__ void f16_to_f32_kernel(const _Float16* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+// Deterministic mean: one block, fixed-order strided partial sums then a tree.
+__global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ v, int64_t n, float scale,
+                                                    float* __restrict__ out) {
+    __shared__ float red[1024];
+    float s = 0.0f;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+void launch_f32_to_f16(const float* a, _Float16* b, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(f32_to_f16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+}
+void launch_f16_to_f32(const _Float16* a, float* b, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(f16_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+}
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, s, v, n, scale, out);
+}
+
+}  // namespace dfg

@@ -1,0 +1,327 @@
+"""Python mirror of the guide's operator interface over the C ABI (libdfg.so).
+
+The reference drives the guide through the C++ class LearnedDistribution
+(src/render/guided/learnedDistribution.h:37-168) and NRC (nrc.h:30-97); this
+module exposes the same operations — reset/create, query (inferenceStep),
+pdf (AutoRegressive::pdf), train step, NRC infer/train, save/load — on device
+tensors (torch is only used for device memory and streams).  Every call goes
+through the HIP library; there is no CPU fallback: loading fails loudly when
+libdfg.so is missing.
+"""
+import ctypes
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdfg.so")
+
+DFG_NET_THETA, DFG_NET_PHI, DFG_NET_NRC = 0, 1, 2
+DFG_PARAMS_MASTER, DFG_PARAMS_TRAIN, DFG_PARAMS_INFERENCE = 0, 1, 2
+
+STATUS = {
+    0: "DFG_OK",
+    1: "DFG_ERR_INVALID_ARG",
+    2: "DFG_ERR_BAD_SIZE",
+    3: "DFG_ERR_CONFIG",
+    4: "DFG_ERR_HIP",
+    5: "DFG_ERR_OOM",
+    6: "DFG_ERR_NOT_INIT",
+    7: "DFG_ERR_IO",
+    8: "DFG_ERR_OVER_BUDGET",
+}
+
+# Every symbol include/dfg.h declares.
+EXPORTS = [
+    "dfg_create", "dfg_destroy", "dfg_last_error", "dfg_get_info", "dfg_get_params", "dfg_get_grad",
+    "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
+    "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_prepare_features",
+    "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
+]
+
+
+class DfgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class DfgInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_bins_theta", ctypes.c_int), ("n_bins_phi", ctypes.c_int),
+        ("interpolation", ctypes.c_int), ("n_input_dims", ctypes.c_int),
+        ("has_nrc", ctypes.c_int), ("enc_width", ctypes.c_int * 3),
+        ("n_hidden", ctypes.c_int * 3), ("n_out_padded", ctypes.c_int * 3),
+        ("n_params", ctypes.c_size_t * 3), ("n_matrix_params", ctypes.c_size_t * 3),
+        ("max_batch", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libdfg.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            "libdfg.so not built (%s); run python guiding-df_amd/build.py" % path)
+    lib = ctypes.CDLL(path)
+    P, I, I64, SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t
+    sig = {
+        "dfg_create": ([ctypes.c_char_p, ctypes.POINTER(P)], I),
+        "dfg_destroy": ([P], I),
+        "dfg_last_error": ([P], ctypes.c_char_p),
+        "dfg_get_info": ([P, ctypes.POINTER(DfgInfo)], I),
+        "dfg_get_params": ([P, I, I, P, SZ], I),
+        "dfg_set_params": ([P, I, P, SZ], I),
+        "dfg_get_grad": ([P, I, P, SZ], I),
+        "dfg_save": ([P, ctypes.c_char_p], I),
+        "dfg_load": ([P, ctypes.c_char_p], I),
+        "dfg_query": ([P, P, I64, P, I, P, P, P, P], I),
+        "dfg_query_plain": ([P, P, I64, P, I, P, P, P, P], I),
+        "dfg_pdf": ([P, P, I64, P, I, P, P], I),
+        "dfg_train_step": ([P, P, I64, P, I, P, P], I),
+        "dfg_nrc_infer": ([P, P, I64, P, I, P], I),
+        "dfg_nrc_train_step": ([P, P, I64, P, I, P, P], I),
+        "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
+        "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
+        "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
+        "dfg_encode": ([P, P, I, I64, P, I, P], I),
+        "dfg_grid_index": ([P, P, I, I64, P, I, P], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _check(lib, ctx, code):
+    if code != 0:
+        msg = lib.dfg_last_error(ctx)
+        raise DfgError(code, msg.decode() if msg else "")
+
+
+# ----------------------------------------------------------------------------
+# Configurations.  reference_config() is the reference's own pair of files
+# (common/configs/nn/base_product_df.json and base_hashgrid_nrc_df.json).
+# ----------------------------------------------------------------------------
+def _ema(decay, lr, eps, beta2=0.99):
+    return {"otype": "EMA", "decay": decay, "nested": {
+        "otype": "Adam", "learning_rate": lr, "beta1": 0.9, "beta2": beta2,
+        "epsilon": eps, "l2_reg": 1e-6}}
+
+
+def _nrc_section():
+    return {
+        "loss": {"otype": "RelativeL2Luminance"},
+        "optimizer": _ema(0.99, 8e-3, 1e-8),
+        "encoding": {"otype": "Composite", "nested": [
+            {"n_dims_to_encode": 3, "otype": "Frequency", "n_frequencies": 12},
+            {"otype": "OneBlob", "n_bins": 4, "n_dims_to_encode": 6},
+            {"n_dims_to_encode": 6, "otype": "Identity"}]},
+        "network": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+                    "n_neurons": 64, "n_hidden_layers": 5},
+    }
+
+
+def reference_config(with_nrc=True, seed=7272):
+    """base_product_df.json (DenseGrid 8x4 smoothstep + SH4 + OneBlob4, phi adds
+    TriangleWave 12) with the reference's compile-time bins 16/32."""
+    grid = {"otype": "DenseGrid", "n_levels": 8, "n_features_per_level": 4, "base_resolution": 8,
+            "per_level_scale": 1.405, "interpolation": "Smoothstep", "n_dims_to_encode": 3}
+    th = {"otype": "Composite", "nested": [
+        grid, {"n_dims_to_encode": 3, "otype": "SphericalHarmonics", "degree": 4},
+        {"otype": "OneBlob", "n_bins": 4, "n_dims_to_encode": 3}]}
+    ph = {"otype": "Composite", "nested": th["nested"] + [
+        {"n_dims_to_encode": 1, "otype": "TriangleWave", "n_frequencies": 12}]}
+    cfg = {
+        "dfg": {"n_bins_theta": 16, "n_bins_phi": 32, "interpolation": "linear", "seed": seed,
+                "n_input_dims": 9},
+        "nn": {"loss": {"otype": "L2"}, "optimizer": _ema(0.95, 3e-2, 1e-15),
+               "encoding": th, "phi_encoding": ph,
+               "network": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+                           "n_neurons": 64, "n_hidden_layers": 3}},
+    }
+    if with_nrc:
+        cfg["nrc"] = _nrc_section()
+    return cfg
+
+
+def plain_head_config(n_bins=32, seed=7272, interpolation="linear"):
+    """Config 1: 8-d features, identity encoding; phi adds OneBlob(16) of theta."""
+    net = {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+           "n_neurons": 64, "n_hidden_layers": 3}
+    return {
+        "dfg": {"n_bins_theta": n_bins, "n_bins_phi": n_bins, "interpolation": interpolation,
+                "seed": seed, "n_input_dims": 8},
+        "nn": {"optimizer": _ema(0.95, 1e-2, 1e-15),
+               "encoding": {"otype": "Identity"},
+               "phi_encoding": {"otype": "Composite", "nested": [
+                   {"otype": "Identity", "n_dims_to_encode": 8},
+                   {"otype": "OneBlob", "n_bins": 16, "n_dims_to_encode": 1}]},
+               "network": net},
+    }
+
+
+def hash_config(n_bins_theta=32, n_bins_phi=32, with_nrc=True, seed=7272, n_levels=16,
+                log2_hashmap_size=19, lr=1e-2):
+    """Configs 2-4: 16-level hash grid (F=2, T=2^19, base 16, scale 2) + SH4 +
+    OneBlob4 per net; phi adds TriangleWave 12; 64 x 3 MLPs."""
+    grid = {"otype": "HashGrid", "n_levels": n_levels, "n_features_per_level": 2,
+            "log2_hashmap_size": log2_hashmap_size, "base_resolution": 16, "per_level_scale": 2.0,
+            "interpolation": "Linear", "n_dims_to_encode": 3}
+    th = {"otype": "Composite", "nested": [
+        grid, {"n_dims_to_encode": 3, "otype": "SphericalHarmonics", "degree": 4},
+        {"otype": "OneBlob", "n_bins": 4, "n_dims_to_encode": 3}]}
+    ph = {"otype": "Composite", "nested": th["nested"] + [
+        {"n_dims_to_encode": 1, "otype": "TriangleWave", "n_frequencies": 12}]}
+    cfg = {
+        "dfg": {"n_bins_theta": n_bins_theta, "n_bins_phi": n_bins_phi, "interpolation": "linear",
+                "seed": seed, "n_input_dims": 9},
+        "nn": {"optimizer": _ema(0.95, lr, 1e-15), "encoding": th, "phi_encoding": ph,
+               "network": {"otype": "FullyFusedMLP", "activation": "ReLU", "output_activation": "None",
+                           "n_neurons": 64, "n_hidden_layers": 3}},
+    }
+    if with_nrc:
+        cfg["nrc"] = _nrc_section()
+    return cfg
+
+
+class GuidingField:
+    """Owner of one dfg context: the theta/phi nets (+ optional NRC)."""
+
+    def __init__(self, config):
+        self.lib = load_library()
+        if isinstance(config, dict):
+            config = json.dumps(config)
+        ctx = ctypes.c_void_p()
+        code = self.lib.dfg_create(config.encode(), ctypes.byref(ctx))
+        if code != 0:
+            raise DfgError(code, self.lib.dfg_last_error(None).decode())
+        self.ctx = ctx
+        self.info = DfgInfo()
+        _check(self.lib, self.ctx, self.lib.dfg_get_info(self.ctx, ctypes.byref(self.info)))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.dfg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, code):
+        _check(self.lib, self.ctx, code)
+
+    # --- parameters ---------------------------------------------------------
+    def n_params(self, net):
+        return int(self.info.n_params[net])
+
+    def get_params(self, net, which=DFG_PARAMS_MASTER):
+        import numpy as np
+        n = self.n_params(net)
+        out = np.empty(n, dtype=np.float32)
+        self._c(self.lib.dfg_get_params(self.ctx, net, which, out.ctypes.data_as(ctypes.c_void_p), n))
+        return out
+
+    def get_grad(self, net):
+        """Last step's gradient, loss-scaled by 1024."""
+        import numpy as np
+        n = self.n_params(net)
+        out = np.empty(n, dtype=np.float32)
+        self._c(self.lib.dfg_get_grad(self.ctx, net, out.ctypes.data_as(ctypes.c_void_p), n))
+        return out
+
+    def set_params(self, net, params):
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._c(self.lib.dfg_set_params(self.ctx, net, p.ctypes.data_as(ctypes.c_void_p), p.size))
+
+    def save(self, prefix):
+        self._c(self.lib.dfg_save(self.ctx, prefix.encode()))
+
+    def load(self, prefix):
+        self._c(self.lib.dfg_load(self.ctx, prefix.encode()))
+
+    # --- operators (device tensors) -----------------------------------------
+    def query(self, feat, u, wi, guided=None, nrc_rgb=None, stream=None):
+        n = feat.shape[0]
+        self._c(self.lib.dfg_query(self.ctx, _stream(stream), n, _ptr(feat), feat.shape[1], _ptr(u),
+                                   _ptr(guided), _ptr(wi), _ptr(nrc_rgb)))
+
+    def query_plain(self, feat, u, wi, guided=None, nrc_rgb=None, stream=None):
+        n = feat.shape[0]
+        self._c(self.lib.dfg_query_plain(self.ctx, _stream(stream), n, _ptr(feat), feat.shape[1], _ptr(u),
+                                         _ptr(guided), _ptr(wi), _ptr(nrc_rgb)))
+
+    def pdf(self, feat, wi2, out, stream=None):
+        self._c(self.lib.dfg_pdf(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
+                                 _ptr(wi2), _ptr(out)))
+
+    def train_step(self, feat, target, loss_out=None, stream=None):
+        self._c(self.lib.dfg_train_step(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
+                                        _ptr(target), _ptr(loss_out)))
+
+    def nrc_infer(self, feat, rgb, stream=None):
+        self._c(self.lib.dfg_nrc_infer(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
+                                       _ptr(rgb)))
+
+    def nrc_train_step(self, feat, target_rgb, loss_out=None, stream=None):
+        self._c(self.lib.dfg_nrc_train_step(self.ctx, _stream(stream), feat.shape[0], _ptr(feat),
+                                            feat.shape[1], _ptr(target_rgb), _ptr(loss_out)))
+
+    def logits(self, feat, theta_in, lt, lp, stream=None):
+        self._c(self.lib.dfg_logits(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
+                                    _ptr(theta_in), _ptr(lt), _ptr(lp)))
+
+    def encode(self, net, inp, out, stream=None):
+        self._c(self.lib.dfg_encode(self.ctx, _stream(stream), net, inp.shape[0], _ptr(inp), inp.shape[1],
+                                    _ptr(out)))
+
+    def grid_index(self, net, inp, idx, stream=None):
+        self._c(self.lib.dfg_grid_index(self.ctx, _stream(stream), net, inp.shape[0], _ptr(inp), inp.shape[1],
+                                        _ptr(idx)))
+
+
+def df_head(logits_theta, logits_phi, u, wi, guided=None, interpolation=1, bins_theta=None,
+            bins_phi=None, seg=None, stream=None):
+    lib = load_library()
+    n = logits_theta.shape[0]
+    code = lib.dfg_df_head(_stream(stream), n, logits_theta.shape[1], logits_phi.shape[1], interpolation,
+                           _ptr(logits_theta), _ptr(logits_phi), _ptr(u), _ptr(guided), _ptr(wi),
+                           _ptr(bins_theta), _ptr(bins_phi), _ptr(seg))
+    if code != 0:
+        raise DfgError(code, "dfg_df_head")
+
+
+def prepare_features(pos, aabb, wo, normal, roughness, feat, stream=None):
+    lib = load_library()
+    code = lib.dfg_prepare_features(_stream(stream), pos.shape[0], _ptr(pos), _ptr(aabb), _ptr(wo),
+                                    _ptr(normal), _ptr(roughness), _ptr(feat))
+    if code != 0:
+        raise DfgError(code, "dfg_prepare_features")

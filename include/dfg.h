@@ -1,0 +1,185 @@
+/*
+ * dfg.h — C ABI of the MI355X-native Distribution-Factorization guiding field.
+ *
+ * The library owns the three networks of the guide (theta-marginal net,
+ * phi|theta-conditional net, radiance cache "NRC"), their fp32 master
+ * weights, fp16 training weights, fp16 EMA inference weights and Adam state,
+ * all resident in device memory.  Every batch pointer below is a DEVICE
+ * pointer owned by the caller; every call is ordered on the hipStream_t passed
+ * as `stream` (NULL = default stream) and returns a dfg_status.  No C++
+ * exception crosses this boundary.  A context is single-stream serialised:
+ * concurrent use needs one context per host thread.
+ *
+ * Reference surface replaced (pedrovfigueiredo/guiding-df):
+ *   LearnedDistribution   src/render/guided/learnedDistribution.h:37-168
+ *   NRC                   src/render/guided/nrc.h:30-97
+ *   tcnn module API model src/ext/tcnn/include/tiny-cuda-nn/cpp_api.h:54-92
+ */
+#ifndef DFG_H
+#define DFG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfg_ctx dfg_ctx;
+
+typedef enum dfg_status {
+    DFG_OK = 0,
+    DFG_ERR_INVALID_ARG = 1,   /* null pointer, unknown net id, bad enum        */
+    DFG_ERR_BAD_SIZE = 2,      /* n < 0, n above the context maximum, size mismatch */
+    DFG_ERR_CONFIG = 3,        /* JSON parse error or unsupported configuration */
+    DFG_ERR_HIP = 4,           /* a HIP runtime call failed                     */
+    DFG_ERR_OOM = 5,           /* device allocation failed                      */
+    DFG_ERR_NOT_INIT = 6,      /* context is NULL / not initialised             */
+    DFG_ERR_IO = 7,            /* file open/read/write failed                   */
+    DFG_ERR_OVER_BUDGET = 8    /* request exceeds the configured memory budget  */
+} dfg_status;
+
+/* Net identifiers for the parameter entry points. */
+#define DFG_NET_THETA 0
+#define DFG_NET_PHI 1
+#define DFG_NET_NRC 2
+
+/* Parameter views for dfg_get_params. */
+#define DFG_PARAMS_MASTER 0     /* fp32 master weights (what the optimizer updates) */
+#define DFG_PARAMS_TRAIN 1      /* fp16 weights used by the training forward, as fp32 */
+#define DFG_PARAMS_INFERENCE 2  /* fp16 EMA weights used by every query, as fp32    */
+
+/* Row layouts (AoS float32) — src/render/guided/guideditem.h:53-110.
+ *   DF features  [n][9]  : pos(3) in [0,1], (wo+1)/2 (3), normal theta/pi,
+ *                          normal phi/2pi, 1-exp(-roughness)      (GuidedInput)
+ *   NRC features [n][15] : the 9 above, diffuse(3), specular(3)  (GuidedInputNRC)
+ *   u            [n][4]  : u_theta, u_phi, extra uniform for a degenerate theta
+ *                          segment, extra uniform for a degenerate phi segment
+ *                          (the reference draws those from sampler.get1D(),
+ *                          src/render/guided/train.h:482,494)
+ *   wi           [n][4]  : (u0, u1, p_theta, p_phi) — parameters.h:44
+ *   target       [B][9]  : dir(2) square coords, L(3), wiPdf, bsdfVal(3) (GuidedOutput)
+ */
+
+/* Create a context from a JSON configuration (the keys of the reference's nn JSON files,
+ * see the "dfg" and "nn"/"nrc" sections documented in INTEGRATION.md), build
+ * the networks and initialise parameters from the configured seed.
+ * Replaces LearnedDistribution::resetNetwork (learnedDistribution.cpp:64-165)
+ * and NRC::resetNetwork (nrc.cpp:35-60). */
+int dfg_create(const char* json_config, dfg_ctx** out_ctx);
+int dfg_destroy(dfg_ctx* ctx);
+/* Last error message of the context (or of the last failed dfg_create when ctx
+ * is NULL).  Never NULL. */
+const char* dfg_last_error(const dfg_ctx* ctx);
+
+/* Static description of the context. */
+typedef struct dfg_info {
+    int n_bins_theta, n_bins_phi;   /* bins of the two 1-D PDFs           */
+    int interpolation;              /* 1 = linear (AR_LINEAR_INTERPOLATION), 0 = nearest */
+    int n_input_dims;               /* DF feature dims (theta net input)  */
+    int has_nrc;                    /* 1 if an NRC net is configured      */
+    int enc_width[3];               /* padded encoded width per net       */
+    int n_hidden[3];                /* hidden layers per net              */
+    int n_out_padded[3];            /* padded output width per net        */
+    size_t n_params[3];             /* parameters per net (MLP then encoding) */
+    size_t n_matrix_params[3];      /* leading MLP (matrix) parameters    */
+    int64_t max_batch;              /* largest n accepted by query/train  */
+} dfg_info;
+int dfg_get_info(const dfg_ctx* ctx, dfg_info* info);
+
+/* Parameters in the reference order: MLP matrices (first, hidden..., last
+ * padded to 16 rows, each row-major [out][in]) followed by the encoding's
+ * parameters by level offset table
+ * (network_with_input_encoding.h:113-155, fully_fused_mlp.cu:873-883,
+ *  grid.h:959-1025).  Host pointers, n = dfg_info.n_params[net]. */
+int dfg_get_params(dfg_ctx* ctx, int net, int which, float* host_out, size_t n);
+/* Gradient of the last training step of `net` (fp32, parameter order), as the
+ * optimizer consumed it: loss-scaled by 1024 (TRAIN_LOSS_SCALE, parameters.h:45),
+ * i.e. 1024 * d(mean loss)/d(param). */
+int dfg_get_grad(dfg_ctx* ctx, int net, float* host_out, size_t n);
+/* Sets the fp32 master, the fp16 training copy and the fp16 EMA copy, like
+ * Trainer::set_params_full_precision (trainer.h:238-250); resets nothing else. */
+int dfg_set_params(dfg_ctx* ctx, int net, const float* host_in, size_t n);
+/* Raw little-endian fp32 files <prefix>AR_network_theta.bin,
+ * <prefix>AR_network_phi.bin, <prefix>NRC_network.bin — the reference's
+ * format (learnedDistribution.cpp:295-382, nrc.cpp:62-112). */
+int dfg_save(dfg_ctx* ctx, const char* path_prefix);
+int dfg_load(dfg_ctx* ctx, const char* path_prefix);
+
+/* Guided query for one wavefront stage (replaces
+ * LearnedDistribution::inferenceStep, learnedDistribution.cpp:167-206, and the
+ * thetaPostInference/phiPostInference kernels, train.h:895-929).
+ * For each i < n: if guided == NULL or guided[i] != 0, sample (u0, u1) from the
+ * guide with u[i]; otherwise keep the direction already in wi[i][0..1] (a BSDF
+ * sample).  Then multiply wi[i][2] by p_theta(u0) and wi[i][3] by p_phi(u1|u0)
+ * (the reference keeps the sign of its initial +-1 there, train.h:741-750).
+ * feat has `feat_stride` floats per row (>= n_input_dims).  When nrc_rgb is
+ * non-NULL and an NRC net exists, also writes the NRC radiance [n][3]
+ * (feat_stride must then be >= 15).  Fused MFMA path: one launch. */
+int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+              const float* u, const uint8_t* guided, float* wi, float* nrc_rgb);
+
+/* Same contract as dfg_query computed by the plain FP32 per-batch path: one
+ * kernel per encoding and per layer, fp32 activations, no MFMA, a host
+ * synchronisation per call — the comparator the throughput target is quoted
+ * against. */
+int dfg_query_plain(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+                    const float* u, const uint8_t* guided, float* wi, float* nrc_rgb);
+
+/* Solid-angle pdf of given directions (AutoRegressive::pdf, AR.h:41-43):
+ * pdf[i] = p_theta(wi2[i][0]) * p_phi(wi2[i][1] | theta) / (4 pi). */
+int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+            const float* wi2, float* pdf);
+
+/* One DF training step on B records (LearnedDistribution::trainStep inner
+ * batch, learnedDistribution.cpp:230-290): forward of both nets with the
+ * training weights, softmax, KL loss and hand-written logits gradient
+ * (train.h:931-965, AR.h:50-116), backward, Adam + EMA.  loss_out (device, may
+ * be NULL) receives the mean per-record loss. */
+int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                   const float* target, float* loss_out);
+
+/* NRC inference (NRC::inferenceStep, nrc.cpp:172-181): feat [n][15] -> rgb [n][3]. */
+int dfg_nrc_infer(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+                  float* rgb);
+/* NRC training step with RelativeL2Luminance (nrc.cpp:114-170):
+ * feat [B][15], target_rgb [B][3]; loss_out (device, may be NULL) receives the
+ * summed loss divided by B, as the reference reports it. */
+int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                       const float* target_rgb, float* loss_out);
+
+/* Feature preparation (generate_inference_data_AR, train.cu:334-362):
+ * shading points -> DF feature rows [n][9].
+ *   pos [n][3] world space, aabb[6] = (min xyz, max xyz), wo [n][3] (any length),
+ *   normal [n][3] (any length), roughness [n]. */
+int dfg_prepare_features(void* stream, int64_t n, const float* pos, const float* aabb,
+                         const float* wo, const float* normal, const float* roughness,
+                         float* feat);
+
+/* Stateless DF head on given logits (softmax x N, inverse-CDF sample, pdf):
+ * the thetaPostInference / phiPostInference math (train.h:844-929) without the
+ * networks.  logits_theta [n][n_theta], logits_phi [n][n_phi]; interpolation
+ * 1 = linear, 0 = nearest.  Outputs (each may be NULL): bins [n][N] (softmax x N),
+ * seg [n][2] (sampled CDF segment of theta and phi; -1 when not sampled).
+ * The phi logits are taken as given (not recomputed from the sampled theta). */
+int dfg_df_head(void* stream, int64_t n, int n_theta, int n_phi, int interpolation,
+                const float* logits_theta, const float* logits_phi, const float* u,
+                const uint8_t* guided, float* wi, float* bins_theta, float* bins_phi,
+                int32_t* seg);
+
+/* Debug exports.  dfg_logits: raw logits of both nets (inference weights) for
+ * rows feat[n] with the phi net fed theta_in[n] (square u0 of theta).
+ * dfg_encode: encoded (padded) input of net `net` [n][enc_width] for raw inputs
+ * in [n][in_dims] (phi: n_input_dims + 1, the last one the theta input).
+ * dfg_grid_index: hash/dense grid table indices [n][levels][8] of net `net`. */
+int dfg_logits(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+               const float* theta_in, float* logits_theta, float* logits_phi);
+int dfg_encode(dfg_ctx* ctx, void* stream, int net, int64_t n, const float* in, int in_stride,
+               float* out);
+int dfg_grid_index(dfg_ctx* ctx, void* stream, int net, int64_t n, const float* in,
+                   int in_stride, uint32_t* idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFG_H */

@@ -1,0 +1,379 @@
+"""ORACLE — test infrastructure only (tests/, the smoke check and the bench's
+CPU-baseline leg use it as the checker; the product never imports it).
+
+Python half of the CPU restatement of the reference's DF guiding math:
+  * layout(): an independent derivation of tiny-cuda-nn's encoding layout
+    (composite padding, encodings/composite.h:136-215, encoding.h:85-87;
+    grid level offsets, encodings/grid.h:959-1025) and of the FullyFusedMLP
+    parameter layout (src/fully_fused_mlp.cu:636-688,
+    network_with_input_encoding.h:113-155);
+  * numpy wrappers around liboracle.so (dfg_oracle.cpp: FP64 oracle and the
+    FP32 reference-order mode).
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ENC = {"identity": 0, "oneblob": 1, "sphericalharmonics": 2, "grid": 3, "hashgrid": 3, "densegrid": 3,
+       "tiledgrid": 3, "trianglewave": 4, "frequency": 5}
+INTERP = {"nearest": 0, "linear": 1, "smoothstep": 2}
+GRID_TYPE = {"dense": 0, "hash": 1, "tiled": 2}
+
+
+class OrcPart(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("type", "dim_begin", "n_dims", "out_begin", "n_out", "n_pad", "pad_front", "param")] + \
+               [("id_scale", ctypes.c_float), ("id_offset", ctypes.c_float)]
+
+
+class OrcNet(ctypes.Structure):
+    _fields_ = [("in_dims", ctypes.c_int), ("n_parts", ctypes.c_int), ("parts", OrcPart * 6),
+                ("enc_width", ctypes.c_int), ("n_hidden", ctypes.c_int), ("n_out", ctypes.c_int),
+                ("n_out_pad", ctypes.c_int), ("grid_present", ctypes.c_int), ("grid_levels", ctypes.c_int),
+                ("grid_F", ctypes.c_int), ("grid_interp", ctypes.c_int), ("grid_type", ctypes.c_int),
+                ("grid_offset", ctypes.c_uint32 * 25), ("grid_res", ctypes.c_uint32 * 24),
+                ("grid_scale", ctypes.c_float * 24), ("layer_off", ctypes.c_uint32 * 10),
+                ("n_matrix", ctypes.c_uint32), ("n_params", ctypes.c_uint32),
+                ("grid_param_begin", ctypes.c_uint32)]
+
+
+def _next_multiple(v, m):
+    return (v + m - 1) // m * m
+
+
+_libm = ctypes.CDLL("libm.so.6")
+_libm.exp2f.argtypes = [ctypes.c_float]
+_libm.exp2f.restype = ctypes.c_float
+_libm.log2f.argtypes = [ctypes.c_float]
+_libm.log2f.restype = ctypes.c_float
+
+
+def grid_levels(n_levels, base_resolution, per_level_scale, log2_hashmap_size, grid_type):
+    """Per-level (scale, resolution, params_in_level) of tcnn's GridEncodingTemplated
+    constructor for 3-D positions (grid.h:984-1013), in fp32 like the reference:
+    scale = exp2f(level * log2f(s)) * base - 1 (grid.h:194-203)."""
+    l2s = np.float32(_libm.log2f(float(np.float32(per_level_scale))))
+    out = []
+    for l in range(n_levels):
+        e = np.float32(_libm.exp2f(float(np.float32(l) * l2s)))
+        scale = np.float32(e * np.float32(base_resolution)) - np.float32(1.0)
+        scale = np.float32(scale)
+        res = int(math.ceil(float(scale))) + 1
+        max_params = 0xffffffff // 2
+        params = max_params if float(res) ** 3 > max_params else res ** 3
+        params = _next_multiple(params, 8)
+        if grid_type == "tiled":
+            params = min(params, base_resolution ** 3)
+        elif grid_type == "hash":
+            params = min(params, 1 << log2_hashmap_size)
+        out.append((scale, res, params))
+    return out
+
+
+def layout(enc, net, in_dims, n_out):
+    """Layout of one NetworkWithInputEncoding(in_dims -> n_out) as a dict."""
+    nested = enc["nested"] if enc.get("otype", "OneBlob").lower() == "composite" else [enc]
+    specified = sum(e.get("n_dims_to_encode", 0) for e in nested)
+    parts = []
+    off = 0
+    for e in nested:
+        dims = e["n_dims_to_encode"] if "n_dims_to_encode" in e else in_dims - specified
+        off = e.get("dims_to_encode_begin", off)
+        if dims > 0:
+            parts.append({"cfg": e, "type": ENC[e.get("otype", "OneBlob").lower()], "n_dims": dims,
+                          "dim_begin": off})
+        off += dims
+    for p in parts:
+        e, t, d = p["cfg"], p["type"], p["n_dims"]
+        if t == 0:
+            p["param"], p["width"] = 0, d
+        elif t == 1:
+            p["param"] = e.get("n_bins", 16)
+            p["width"] = d * p["param"]
+        elif t == 2:
+            p["param"] = e.get("degree", 4)
+            p["width"] = p["param"] ** 2
+        elif t == 4:
+            p["param"] = e.get("n_frequencies", 12)
+            p["width"] = d * p["param"]
+        elif t == 5:
+            p["param"] = e.get("n_frequencies", 12)
+            p["width"] = 2 * d * p["param"]
+        elif t == 3:
+            p["param"] = e.get("n_features_per_level", 2)
+            p["width"] = e.get("n_levels", 16) * p["param"]
+        p["align"] = p["param"] if t == 3 else 1
+    total_align = 16
+    for p in parts:
+        total_align = total_align * p["align"] // math.gcd(total_align, p["align"])
+    so_far = 0
+    for i, p in enumerate(parts):
+        a = parts[i + 1]["align"] if i + 1 < len(parts) else total_align
+        padded = _next_multiple(so_far + p["width"], a) - so_far
+        p["out_begin"], p["n_pad"] = so_far, padded - p["width"]
+        so_far += padded
+    L = {"in_dims": in_dims, "parts": parts, "enc_width": so_far,
+         "n_hidden": net.get("n_hidden_layers", 3), "n_out": n_out, "n_out_pad": _next_multiple(n_out, 16)}
+    offs, o = [], 0
+    for l in range(L["n_hidden"] + 1):
+        rows = L["n_out_pad"] if l == L["n_hidden"] else 64
+        cols = L["enc_width"] if l == 0 else 64
+        offs.append(o)
+        o += rows * cols
+    offs.append(o)
+    L["layer_off"], L["n_matrix"] = offs, o
+    L["grid"] = None
+    for p in parts:
+        if p["type"] != 3:
+            continue
+        e = p["cfg"]
+        ot = e.get("otype", "Grid").lower()
+        gtype = e.get("type", "Tiled" if ot == "tiledgrid" else ("Dense" if ot == "densegrid" else "Hash")).lower()
+        lv = grid_levels(e.get("n_levels", 16), e.get("base_resolution", 16), e.get("per_level_scale", 2.0),
+                         e.get("log2_hashmap_size", 19), gtype)
+        offsets = [0]
+        for (_, _, pl) in lv:
+            offsets.append(offsets[-1] + pl)
+        L["grid"] = {"type": gtype, "F": p["param"], "levels": len(lv), "scale": [s for s, _, _ in lv],
+                     "res": [r for _, r, _ in lv], "offset": offsets,
+                     "interp": e.get("interpolation", "Linear").lower(), "param_begin": o, "dim_begin": p["dim_begin"],
+                     "out_begin": p["out_begin"]}
+    L["n_params"] = o + (L["grid"]["offset"][-1] * L["grid"]["F"] if L["grid"] else 0)
+    return L
+
+
+def config_layouts(cfg):
+    """(theta, phi, nrc-or-None) layouts of a full guide configuration dict."""
+    d = cfg.get("dfg", {})
+    nt, npb, D = d.get("n_bins_theta", 16), d.get("n_bins_phi", 32), d.get("n_input_dims", 9)
+    nn = cfg["nn"]
+    pick = lambda a, b: nn[a] if a in nn else nn[b]
+    th = layout(pick("theta_encoding", "encoding"), pick("theta_network", "network"), D, nt)
+    ph = layout(pick("phi_encoding", "encoding"), pick("phi_network", "network"), D + 1, npb)
+    nrc = None
+    if "nrc" in cfg:
+        nrc = layout(cfg["nrc"]["encoding"], cfg["nrc"]["network"], 15, 3)
+    return th, ph, nrc
+
+
+def to_struct(L):
+    s = OrcNet()
+    s.in_dims = L["in_dims"]
+    s.n_parts = len(L["parts"])
+    for i, p in enumerate(L["parts"]):
+        q = s.parts[i]
+        q.type, q.dim_begin, q.n_dims = p["type"], p["dim_begin"], p["n_dims"]
+        q.out_begin, q.n_out, q.n_pad = p["out_begin"], p["width"], p["n_pad"]
+        q.pad_front = 1 if p["type"] == 2 else 0
+        q.param = p["param"]
+        q.id_scale = p["cfg"].get("scale", 1.0)
+        q.id_offset = p["cfg"].get("offset", 0.0)
+    s.enc_width, s.n_hidden, s.n_out, s.n_out_pad = L["enc_width"], L["n_hidden"], L["n_out"], L["n_out_pad"]
+    for i, o in enumerate(L["layer_off"]):
+        s.layer_off[i] = o
+    s.n_matrix, s.n_params = L["n_matrix"], L["n_params"]
+    g = L["grid"]
+    if g:
+        s.grid_present, s.grid_levels, s.grid_F = 1, g["levels"], g["F"]
+        s.grid_interp = INTERP[g["interp"]]
+        s.grid_type = GRID_TYPE[g["type"]]
+        for i, o in enumerate(g["offset"]):
+            s.grid_offset[i] = o
+        for i in range(g["levels"]):
+            s.grid_res[i] = g["res"][i]
+            s.grid_scale[i] = float(g["scale"][i])
+        s.grid_param_begin = g["param_begin"]
+    else:
+        s.grid_param_begin = L["n_params"]
+    return s
+
+
+# ----------------------------------------------------------------------------
+# liboracle.so
+# ----------------------------------------------------------------------------
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, I64, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        F = ctypes.c_float
+        U32 = ctypes.c_uint32
+        sig = {
+            "orc_det_expf": ([F], F), "orc_det_logf": ([F], F), "orc_half_round": ([F], F),
+            "orc_grid_index": ([P, P, I64, I, I, P], None),
+            "orc_grid_index3": ([I, U32, U32, U32, U32, U32], U32),
+            "orc_encode_f64": ([P, P, P, I64, I, P], None),
+            "orc_mlp_f64": ([P, P, P, I64, P], None),
+            "orc_head_f64": ([P, I64, I, I, P, P, P, P, P, P, P, P], None),
+            "orc_head_f32": ([P, I64, I, I, P, P, P, P, P, P, P, P], None),
+            "orc_pdf_linear_f64": ([P, I, D], D),
+            "orc_theta_input_f64": ([D, I, I], D),
+            "orc_theta_input_f32": ([F, I, I], F),
+            "orc_query_f64": ([P, P, P, P, P, I, I, I64, P, P, P, I, P, P], None),
+            "orc_query_f32_omp": ([P, P, P, P, P, I, I, I64, P, P, P, I, P], I),
+            "orc_nrc_f64": ([P, P, P, I, I64, P], None),
+            "orc_nrc_f32_omp": ([P, P, P, I, I64, P], None),
+            "orc_df_train_f64": ([P, P, P, P, P, I, I, P, I64, I, P, P], D),
+            "orc_nrc_train_f64": ([P, P, P, I, P, I64, P], D),
+            "orc_adam_ema_f64": ([I64, I64, D, D, D, D, D, D, I, P, P, P, P, P, P, P], None),
+        }
+        for name, (a, r) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = a
+            fn.restype = r
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def det_expf(x):
+    return np.float32(lib().orc_det_expf(float(x)))
+
+
+def det_logf(x):
+    return np.float32(lib().orc_det_logf(float(x)))
+
+
+def half_round(x):
+    return np.float32(lib().orc_half_round(float(x)))
+
+
+def grid_index(L, inp, pos_dim=None):
+    s = to_struct(L)
+    inp = _c(inp, np.float32)
+    n = inp.shape[0]
+    g = L["grid"]
+    out = np.zeros((n, g["levels"], 8), dtype=np.uint32)
+    lib().orc_grid_index(ctypes.byref(s), _p(inp), n, inp.shape[1], g["dim_begin"] if pos_dim is None else pos_dim,
+                         _p(out))
+    return out
+
+
+def encode_f64(L, params, inp):
+    s = to_struct(L)
+    params = _c(params, np.float64)
+    inp = _c(inp, np.float64)
+    out = np.zeros((inp.shape[0], L["enc_width"]), dtype=np.float64)
+    lib().orc_encode_f64(ctypes.byref(s), _p(params), _p(inp), inp.shape[0], inp.shape[1], _p(out))
+    return out
+
+
+def mlp_f64(L, params, x):
+    s = to_struct(L)
+    params = _c(params, np.float64)
+    x = _c(x, np.float64)
+    out = np.zeros((x.shape[0], L["n_out_pad"]), dtype=np.float64)
+    lib().orc_mlp_f64(ctypes.byref(s), _p(params), _p(x), x.shape[0], _p(out))
+    return out
+
+
+def head(logits, linear, u, ue, sample=None, x_given=None, fp32=False):
+    """Softmax x N, inverse-CDF sample (where sample[i]), pdf.  Returns
+    (x, pdf, bins, seg)."""
+    dt = np.float32 if fp32 else np.float64
+    logits = _c(logits, dt)
+    n, N = logits.shape
+    u, ue = _c(u, dt), _c(ue, dt)
+    x = np.zeros(n, dt)
+    pdf = np.zeros(n, dt)
+    bins = np.zeros((n, N), dt)
+    seg = np.zeros(n, np.int32)
+    smp = None if sample is None else _c(sample, np.uint8)
+    xg = None if x_given is None else _c(x_given, dt)
+    fn = lib().orc_head_f32 if fp32 else lib().orc_head_f64
+    fn(_p(logits), n, N, int(linear), _p(u), _p(ue), _p(smp), _p(xg), _p(x), _p(pdf), _p(bins), _p(seg))
+    return x, pdf, bins, seg
+
+
+def query_f64(Lt, Lp, pt, pp, feat, D, u, wi_in, linear, guided=None, with_logits=False):
+    st, sp = to_struct(Lt), to_struct(Lp)
+    pt, pp = _c(pt, np.float32), _c(pp, np.float32)
+    feat, u, wi_in = _c(feat, np.float32), _c(u, np.float32), _c(wi_in, np.float32)
+    n = feat.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    lg = np.zeros((n, 128), np.float64) if with_logits else None
+    g = None if guided is None else _c(guided, np.uint8)
+    lib().orc_query_f64(ctypes.byref(st), ctypes.byref(sp), _p(pt), _p(pp), _p(feat), feat.shape[1], D, n, _p(u),
+                        _p(g), _p(wi_in), int(linear), _p(out), _p(lg))
+    return (out, lg) if with_logits else out
+
+
+def query_f32_omp(Lt, Lp, pt, pp, feat, D, u, wi_in, linear, guided=None):
+    """FP32 reference-order query on all host cores; returns (wi, threads)."""
+    st, sp = to_struct(Lt), to_struct(Lp)
+    pt, pp = _c(pt, np.float32), _c(pp, np.float32)
+    feat, u, wi_in = _c(feat, np.float32), _c(u, np.float32), _c(wi_in, np.float32)
+    n = feat.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    g = None if guided is None else _c(guided, np.uint8)
+    thr = lib().orc_query_f32_omp(ctypes.byref(st), ctypes.byref(sp), _p(pt), _p(pp), _p(feat), feat.shape[1], D, n,
+                                  _p(u), _p(g), _p(wi_in), int(linear), _p(out))
+    return out, thr
+
+
+def nrc_f64(L, params, feat):
+    s = to_struct(L)
+    params, feat = _c(params, np.float32), _c(feat, np.float32)
+    out = np.zeros((feat.shape[0], 3), np.float64)
+    lib().orc_nrc_f64(ctypes.byref(s), _p(params), _p(feat), feat.shape[1], feat.shape[0], _p(out))
+    return out
+
+
+def nrc_f32_omp(L, params, feat):
+    s = to_struct(L)
+    params, feat = _c(params, np.float32), _c(feat, np.float32)
+    out = np.zeros((feat.shape[0], 3), np.float32)
+    lib().orc_nrc_f32_omp(ctypes.byref(s), _p(params), _p(feat), feat.shape[1], feat.shape[0], _p(out))
+    return out
+
+
+def df_train_f64(Lt, Lp, pt, pp, feat, D, target, linear, grads=True):
+    st, sp = to_struct(Lt), to_struct(Lp)
+    pt, pp = _c(pt, np.float64), _c(pp, np.float64)
+    feat, target = _c(feat, np.float32), _c(target, np.float32)
+    gt = np.zeros(Lt["n_params"], np.float64) if grads else None
+    gp = np.zeros(Lp["n_params"], np.float64) if grads else None
+    loss = lib().orc_df_train_f64(ctypes.byref(st), ctypes.byref(sp), _p(pt), _p(pp), _p(feat), feat.shape[1], D,
+                                  _p(target), feat.shape[0], int(linear), _p(gt), _p(gp))
+    return loss, gt, gp
+
+
+def nrc_train_f64(L, params, feat, target, grads=True):
+    s = to_struct(L)
+    params, feat, target = _c(params, np.float64), _c(feat, np.float32), _c(target, np.float32)
+    g = np.zeros(L["n_params"], np.float64) if grads else None
+    loss = lib().orc_nrc_train_f64(ctypes.byref(s), _p(params), _p(feat), feat.shape[1], _p(target), feat.shape[0],
+                                   _p(g))
+    return loss, g
+
+
+def adam_ema_f64(n_matrix, lr, b1, b2, eps, l2, decay, step, grad, master, m, v, steps, w_lowp=None, ema=None):
+    grad, master = _c(grad, np.float64), _c(master, np.float64).copy()
+    m, v, steps = _c(m, np.float64).copy(), _c(v, np.float64).copy(), _c(steps, np.uint32).copy()
+    ema = None if ema is None else _c(ema, np.float64).copy()
+    wl = None if w_lowp is None else _c(w_lowp, np.float64)
+    lib().orc_adam_ema_f64(master.size, n_matrix, lr, b1, b2, eps, l2, decay, step, _p(grad), _p(master), _p(m),
+                           _p(v), _p(steps), _p(wl), _p(ema))
+    return master, m, v, steps, ema

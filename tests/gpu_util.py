@@ -1,0 +1,66 @@
+"""Shared helpers for the GPU parity tests and the bench: synthetic inputs of
+the shapes the guide consumes (seeded), and device-tensor plumbing."""
+import math
+
+import numpy as np
+
+
+def df_features(n, seed, nrc=False):
+    """DF feature rows [n][9] (or [n][15] with the NRC extras) as the reference's
+    generate_inference_data_AR produces them (train.cu:334-362):
+    pos U[0,1]^3; wo from dir2 ~ U[0,1]^2 read as (theta/pi, phi/2pi), warped
+    (v+1)/2; normal2 ~ U[0,1]^2 (already theta/pi, phi/2pi); 1-exp(-r),
+    r ~ U[0,1]; NRC extras diffuse3, specular3 ~ U[0,1]."""
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3), dtype=np.float32)
+    d2 = rng.random((n, 2), dtype=np.float32)
+    th, ph = d2[:, 0] * np.float32(math.pi), d2[:, 1] * np.float32(2 * math.pi)
+    wo = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1).astype(np.float32)
+    wo = (wo + 1.0) * 0.5
+    nrm = rng.random((n, 2), dtype=np.float32)
+    rough = 1.0 - np.exp(-rng.random((n, 1), dtype=np.float32))
+    cols = [pos, wo, nrm, rough]
+    if nrc:
+        cols.append(rng.random((n, 6), dtype=np.float32))
+    return np.ascontiguousarray(np.concatenate(cols, 1).astype(np.float32))
+
+
+def plain_features(n, seed, dims=8):
+    return np.random.default_rng(seed).random((n, dims), dtype=np.float32)
+
+
+def uniforms(n, seed):
+    return np.random.default_rng(seed).random((n, 4), dtype=np.float32)
+
+
+def wi_init(n):
+    w = np.zeros((n, 4), np.float32)
+    w[:, 2:] = 1.0
+    return w
+
+
+def train_targets(n, seed):
+    """GuidedOutput rows [B][9]: dir (u0,u1) of a direction uniform on the
+    sphere via uniformSphereToSquare (math_utils.h:233-240), L ~ logNormal(0,2)
+    per channel, wiPdf = 1/(4pi), bsdfVal = 1."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    phi = np.arctan2(v[:, 1], v[:, 0]) / (2 * math.pi)
+    phi = np.where(phi < 0, phi + 1.0, phi)
+    u0 = (1.0 - v[:, 2]) * 0.5
+    L = np.exp(rng.normal(0.0, 2.0, (n, 3)))
+    t = np.zeros((n, 9), np.float32)
+    t[:, 0], t[:, 1] = u0, phi
+    t[:, 2:5] = L
+    t[:, 5] = 1.0 / (4 * math.pi)
+    t[:, 6:9] = 1.0
+    return t
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()

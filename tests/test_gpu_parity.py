@@ -1,0 +1,442 @@
+"""GPU parity: the HIP library (through the C ABI) against the oracle.
+
+Bit-exact where the work is integer or a fixed fp32 sequence (grid indices,
+softmax bins, CDF segments, samples and pdfs from identical logits); within a
+stated tolerance against the FP64 oracle where the path computes in fp16/fp32
+(MLP on MFMA with fp16 activations and fp32 accumulation)."""
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import dfguide as G
+import oracle as O
+from gpu_util import df_features, plain_features, uniforms, wi_init, train_targets, dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+# ----------------------------------------------------------------------------
+# DF head on given logits: bit-exact against the FP32 reference-order oracle
+# ----------------------------------------------------------------------------
+def _adversarial_logits(rng, n, N):
+    z = rng.normal(0.0, 1.0, (n, N)).astype(np.float32)
+    k = n // 8
+    z[:k] = 0.0                                   # uniform bins: A == 0 branch everywhere
+    z[k:2 * k] = -60.0                            # all far below the max's init of 0
+    hot = rng.integers(0, N, k)
+    z[2 * k:3 * k] = -5.0
+    z[np.arange(2 * k, 3 * k), hot] = 25.0        # one-hot: empty segments (delta < eps)
+    z[3 * k:4 * k] *= 8.0                         # peaked
+    z[4 * k:5 * k] = np.round(z[4 * k:5 * k])     # equal neighbours
+    return z
+
+
+@pytest.mark.parametrize("nt,np_,linear", [(16, 32, 1), (32, 32, 1), (64, 64, 1), (32, 32, 0), (16, 32, 0)])
+def test_df_head_bitexact(torch_cuda, nt, np_, linear):
+    torch = torch_cuda
+    n = 8192
+    rng = np.random.default_rng(100 + nt + np_ + linear)
+    lt = _adversarial_logits(rng, n, nt)
+    lp = _adversarial_logits(rng, n, np_)[rng.permutation(n)]
+    u = rng.random((n, 4), dtype=np.float32)
+    u[:64, :2] = 0.0
+    u[64:128, :2] = np.float32(1.0) - np.float32(2.0 ** -24)
+    guided = (rng.random(n) < 0.8).astype(np.uint8)
+    wi = np.concatenate([rng.random((n, 2), dtype=np.float32), np.ones((n, 2), np.float32)], 1)
+    d_wi = dev(torch, wi)
+    d_bt = torch.zeros((n, nt), dtype=torch.float32, device="cuda")
+    d_bp = torch.zeros((n, np_), dtype=torch.float32, device="cuda")
+    d_seg = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    G.df_head(dev(torch, lt), dev(torch, lp), dev(torch, u), d_wi, guided=dev(torch, guided), interpolation=linear,
+              bins_theta=d_bt, bins_phi=d_bp, seg=d_seg)
+    torch.cuda.synchronize()
+    g_wi, g_bt, g_bp, g_seg = host(d_wi), host(d_bt), host(d_bp), host(d_seg)
+    x0, p0, b0, s0 = O.head(lt, linear, u[:, 0], u[:, 2], sample=guided, x_given=wi[:, 0], fp32=True)
+    x1, p1, b1, s1 = O.head(lp, linear, u[:, 1], u[:, 3], sample=guided, x_given=wi[:, 1], fp32=True)
+    assert np.array_equal(g_bt, b0), "theta bins differ in %d rows" % (g_bt != b0).any(1).sum()
+    assert np.array_equal(g_bp, b1), "phi bins differ in %d rows" % (g_bp != b1).any(1).sum()
+    s0 = np.where(guided == 1, s0, -1)
+    s1 = np.where(guided == 1, s1, -1)
+    assert np.array_equal(g_seg[:, 0], s0), "theta segments differ in %d rows" % (g_seg[:, 0] != s0).sum()
+    assert np.array_equal(g_seg[:, 1], s1), "phi segments differ in %d rows" % (g_seg[:, 1] != s1).sum()
+    assert np.array_equal(g_wi[:, 0], x0) and np.array_equal(g_wi[:, 1], x1), "samples differ"
+    assert np.array_equal(g_wi[:, 2], p0) and np.array_equal(g_wi[:, 3], p1), "pdfs differ"
+
+
+# ----------------------------------------------------------------------------
+# Encodings and grid indices
+# ----------------------------------------------------------------------------
+def _edge_positions(rng, n, in_dims):
+    x = rng.random((n, in_dims), dtype=np.float32)
+    edges = np.array([0.0, 1.0, 0.5, 1.0 - 2 ** -24, 2 ** -20, 0.25, 0.999], np.float32)
+    x[: 7 * 7, 0] = np.repeat(edges, 7)
+    x[: 7 * 7, 1] = np.tile(edges, 7)
+    return x
+
+
+@pytest.mark.parametrize("cfgname", ["reference", "hash"])
+def test_grid_index_bitexact(torch_cuda, cfgname):
+    torch = torch_cuda
+    cfg = G.reference_config(with_nrc=False) if cfgname == "reference" else G.hash_config(with_nrc=False)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    for net, L in ((G.DFG_NET_THETA, th), (G.DFG_NET_PHI, ph)):
+        n = 4096
+        x = _edge_positions(np.random.default_rng(7 + net), n, L["in_dims"])
+        d_idx = torch.zeros((n, L["grid"]["levels"], 8), dtype=torch.int32, device="cuda")
+        f.grid_index(net, dev(torch, x), d_idx)
+        torch.cuda.synchronize()
+        g = host(d_idx).view(np.uint32)
+        o = O.grid_index(L, x)
+        assert np.array_equal(g, o), "grid indices differ at %d entries" % (g != o).sum()
+    f.close()
+
+
+@pytest.mark.parametrize("cfgname", ["reference", "hash", "plain"])
+def test_encode_matches_fp64(torch_cuda, cfgname):
+    torch = torch_cuda
+    cfg = {"reference": G.reference_config(), "hash": G.hash_config(), "plain": G.plain_head_config()}[cfgname]
+    f = G.GuidingField(cfg)
+    layouts = O.config_layouts(cfg)
+    for net, L in enumerate(layouts):
+        if L is None:
+            continue
+        n = 2048
+        x = _edge_positions(np.random.default_rng(11 + net), n, L["in_dims"])
+        d_out = torch.zeros((n, L["enc_width"]), dtype=torch.float32, device="cuda")
+        f.encode(net, dev(torch, x), d_out)
+        torch.cuda.synchronize()
+        params = f.get_params(net, G.DFG_PARAMS_INFERENCE).astype(np.float64)
+        o = O.encode_f64(L, params, x.astype(np.float64))
+        err = np.abs(host(d_out) - o)
+        # sin/cos arguments up to 2^11 pi carry fp32 argument rounding (~6e-4)
+        tol = 2e-3 if net == G.DFG_NET_NRC else 2e-5
+        assert err.max() < tol, "net %d encode max err %.3g" % (net, err.max())
+    f.close()
+
+
+# ----------------------------------------------------------------------------
+# Full query against the FP64 oracle on the same (inference) weights
+# ----------------------------------------------------------------------------
+def _query_case(cfgname):
+    if cfgname == "plain":
+        cfg = G.plain_head_config()
+        return cfg, 8, lambda n, s: plain_features(n, s)
+    cfg = G.hash_config(with_nrc=True)
+    return cfg, 9, lambda n, s: df_features(n, s, nrc=True)
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+def test_query_vs_fp64(torch_cuda, cfgname):
+    torch = torch_cuda
+    cfg, D, feats = _query_case(cfgname)
+    f = G.GuidingField(cfg)
+    th, ph, nrc = O.config_layouts(cfg)
+    n = 4096
+    feat = feats(n, 1)
+    u = uniforms(n, 2)
+    guided = (np.random.default_rng(3).random(n) < 0.75).astype(np.uint8)
+    wi0 = wi_init(n)
+    wi0[:, :2] = np.random.default_rng(4).random((n, 2), dtype=np.float32) * np.float32(0.99998)
+    d_wi = dev(torch, wi0)
+    d_rgb = torch.zeros((n, 3), dtype=torch.float32, device="cuda") if nrc else None
+    f.query(dev(torch, feat), dev(torch, u), d_wi, guided=dev(torch, guided), nrc_rgb=d_rgb)
+    torch.cuda.synchronize()
+    g = host(d_wi)
+    pt = f.get_params(0, G.DFG_PARAMS_INFERENCE)
+    pp = f.get_params(1, G.DFG_PARAMS_INFERENCE)
+    o = O.query_f64(th, ph, pt, pp, feat, D, u, wi0, 1, guided=guided)
+    pdf_g, pdf_o = g[:, 2] * g[:, 3], o[:, 2] * o[:, 3]
+    rel = _rel(pdf_g, pdf_o)
+    assert np.all(np.isfinite(g)) and np.all(pdf_g > 0)
+    assert np.median(rel) <= 1e-3, "median rel pdf err %.3g" % np.median(rel)
+    assert np.percentile(rel, 99) <= 3e-2, "p99 rel pdf err %.3g" % np.percentile(rel, 99)
+    du = np.abs(g[:, :2] - o[:, :2])
+    assert np.median(du) <= 1e-3 and np.percentile(du, 99) <= 2e-2
+    # non-guided rows keep the caller's direction exactly
+    ng = guided == 0
+    assert np.array_equal(g[ng, :2], wi0[ng, :2])
+    if nrc:
+        orgb = O.nrc_f64(nrc, f.get_params(2, G.DFG_PARAMS_INFERENCE), feat)
+        err = np.abs(host(d_rgb) - orgb)
+        assert err.max() <= 2e-2 * max(1.0, np.abs(orgb).max()), "NRC max err %.3g" % err.max()
+    f.close()
+
+
+@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+def test_fused_head_bitexact_on_own_logits(torch_cuda, cfgname):
+    """The fused kernel's head equals the FP32 oracle head on the kernel's own
+    logits: same bins, segments, samples and pdfs bit for bit."""
+    torch = torch_cuda
+    cfg, D, feats = _query_case(cfgname)
+    f = G.GuidingField(cfg)
+    n = 4096
+    feat = feats(n, 5)
+    u = uniforms(n, 6)
+    d_wi = dev(torch, wi_init(n))
+    d_feat = dev(torch, feat)
+    f.query(d_feat, dev(torch, u), d_wi)
+    torch.cuda.synchronize()
+    g = host(d_wi)
+    nt, np_ = f.info.n_bins_theta, f.info.n_bins_phi
+    th_in = np.array([O.lib().orc_theta_input_f32(float(x), nt, 1) for x in g[:, 0]], np.float32)
+    d_lt = torch.zeros((n, nt), dtype=torch.float32, device="cuda")
+    d_lp = torch.zeros((n, np_), dtype=torch.float32, device="cuda")
+    f.logits(d_feat, dev(torch, th_in), d_lt, d_lp)
+    torch.cuda.synchronize()
+    lt, lp = host(d_lt), host(d_lp)
+    x0, p0, _, _ = O.head(lt, 1, u[:, 0], u[:, 2], fp32=True)
+    x1, p1, _, _ = O.head(lp, 1, u[:, 1], u[:, 3], fp32=True)
+    assert np.array_equal(g[:, 0], x0), "theta samples differ in %d rows" % (g[:, 0] != x0).sum()
+    assert np.array_equal(g[:, 1], x1), "phi samples differ in %d rows" % (g[:, 1] != x1).sum()
+    assert np.array_equal(g[:, 2], p0) and np.array_equal(g[:, 3], p1)
+    f.close()
+
+
+@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+def test_plain_path_matches(torch_cuda, cfgname):
+    torch = torch_cuda
+    cfg, D, feats = _query_case(cfgname)
+    f = G.GuidingField(cfg)
+    th, ph, nrc = O.config_layouts(cfg)
+    n = 3000
+    feat = feats(n, 8)
+    u = uniforms(n, 9)
+    d_feat, d_u = dev(torch, feat), dev(torch, u)
+    w1, w2 = dev(torch, wi_init(n)), dev(torch, wi_init(n))
+    r1 = torch.zeros((n, 3), dtype=torch.float32, device="cuda") if nrc else None
+    r2 = torch.zeros((n, 3), dtype=torch.float32, device="cuda") if nrc else None
+    f.query(d_feat, d_u, w1, nrc_rgb=r1)
+    f.query_plain(d_feat, d_u, w2, nrc_rgb=r2)
+    torch.cuda.synchronize()
+    gf, gp = host(w1), host(w2)
+    o = O.query_f64(th, ph, f.get_params(0, 2), f.get_params(1, 2), feat, D, u, wi_init(n), 1)
+    rel_p = _rel(gp[:, 2] * gp[:, 3], o[:, 2] * o[:, 3])
+    rel_f = _rel(gf[:, 2] * gf[:, 3], o[:, 2] * o[:, 3])
+    assert np.median(rel_p) <= 1e-4, "plain FP32 path median rel pdf err %.3g" % np.median(rel_p)
+    assert np.median(rel_f) <= 1e-3
+    if nrc:
+        assert np.abs(host(r1) - host(r2)).max() < 5e-2
+    f.close()
+
+
+def test_pdf_entry(torch_cuda):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    n = 2048
+    feat = df_features(n, 12)
+    wi2 = np.random.default_rng(13).random((n, 2), dtype=np.float32)
+    d_pdf = torch.zeros(n, dtype=torch.float32, device="cuda")
+    f.pdf(dev(torch, feat), dev(torch, wi2), d_pdf)
+    torch.cuda.synchronize()
+    wi0 = wi_init(n)
+    wi0[:, :2] = wi2
+    o = O.query_f64(th, ph, f.get_params(0, 2), f.get_params(1, 2), feat, 9, uniforms(n, 1), wi0, 1,
+                    guided=np.zeros(n, np.uint8))
+    ref = o[:, 2] * o[:, 3] * np.float32(0.07957747154594766788)
+    rel = _rel(host(d_pdf), ref)
+    assert np.median(rel) <= 1e-3 and np.percentile(rel, 99) <= 3e-2
+    f.close()
+
+
+# ----------------------------------------------------------------------------
+# Training step: loss, gradient vs the FP64 oracle's analytic gradient (which
+# test_oracle pins by finite differences), Adam + EMA on the GPU's gradient.
+# ----------------------------------------------------------------------------
+def _grad_rel(g, o):
+    return np.linalg.norm(g - o) / max(np.linalg.norm(o), 1e-30)
+
+
+@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+def test_train_step_vs_fp64(torch_cuda, cfgname):
+    torch = torch_cuda
+    if cfgname == "plain":
+        cfg = G.plain_head_config()
+        D, feat = 8, plain_features(1000, 20)
+    else:
+        cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+        D, feat = 9, df_features(1000, 20)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    B = feat.shape[0]
+    tgt = train_targets(B, 21)
+    before = {k: f.get_params(k, G.DFG_PARAMS_MASTER) for k in (0, 1)}
+    wtrain = {k: f.get_params(k, G.DFG_PARAMS_TRAIN) for k in (0, 1)}
+    ema_before = {k: f.get_params(k, G.DFG_PARAMS_INFERENCE) for k in (0, 1)}
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    f.train_step(dev(torch, feat), dev(torch, tgt), d_loss)
+    torch.cuda.synchronize()
+    loss_g = float(host(d_loss)[0])
+    loss_o, gt_o, gp_o = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1)
+    assert abs(loss_g - loss_o) <= 2e-3 * abs(loss_o) + 1e-6, (loss_g, loss_o)
+    for k, go, L in ((0, gt_o, th), (1, gp_o, ph)):
+        gg = f.get_grad(k).astype(np.float64) / 1024.0
+        nm = L["n_matrix"]
+        assert _grad_rel(gg[:nm], go[:nm]) <= 3e-2, "net %d matrix grad rel err %.3g" % (k, _grad_rel(gg[:nm], go[:nm]))
+        if L["grid"]:
+            assert _grad_rel(gg[nm:], go[nm:]) <= 5e-2, "net %d grid grad rel err %.3g" % (k, _grad_rel(gg[nm:], go[nm:]))
+        # Adam + EMA applied to the GPU's own gradient
+        opt = cfg["nn"]["optimizer"]
+        ad = opt["nested"]
+        master, _, _, _, _ = O.adam_ema_f64(nm, ad["learning_rate"], ad["beta1"], ad["beta2"], ad["epsilon"],
+                                            ad["l2_reg"], opt["decay"], 1, gg, before[k], np.zeros(gg.size),
+                                            np.zeros(gg.size), np.zeros(gg.size, np.uint32))
+        after = f.get_params(k, G.DFG_PARAMS_MASTER).astype(np.float64)
+        assert np.allclose(after, master, rtol=1e-5, atol=1e-7), "net %d Adam update differs" % k
+        # first EMA step: debias makes the EMA equal the new fp16 weights
+        ema = f.get_params(k, G.DFG_PARAMS_INFERENCE)
+        assert np.array_equal(ema, f.get_params(k, G.DFG_PARAMS_TRAIN))
+        assert not np.array_equal(ema, ema_before[k])
+    f.close()
+
+
+def test_nrc_train_step_vs_fp64(torch_cuda):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
+    f = G.GuidingField(cfg)
+    _, _, L = O.config_layouts(cfg)
+    B = 1000
+    feat = df_features(B, 30, nrc=True)
+    tgt = np.exp(np.random.default_rng(31).normal(0, 2, (B, 3))).astype(np.float32)
+    w = f.get_params(2, G.DFG_PARAMS_TRAIN)
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    f.nrc_train_step(dev(torch, feat), dev(torch, tgt), d_loss)
+    torch.cuda.synchronize()
+    loss_o, g_o = O.nrc_train_f64(L, w, feat, tgt)
+    loss_g = float(host(d_loss)[0])
+    assert abs(loss_g - loss_o) <= 1e-2 * abs(loss_o), (loss_g, loss_o)
+    gg = f.get_grad(2).astype(np.float64) / 1024.0
+    assert _grad_rel(gg, g_o) <= 3e-2, "NRC grad rel err %.3g" % _grad_rel(gg, g_o)
+    f.close()
+
+
+# ----------------------------------------------------------------------------
+# Sizes, errors, formats
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000])
+def test_ragged_sizes_match_single_rows(torch_cuda, n):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False)
+    f = G.GuidingField(cfg)
+    feat = df_features(max(n, 1), 40)[:n]
+    u = uniforms(max(n, 1), 41)[:n]
+    d_wi = dev(torch, wi_init(n)) if n else torch.zeros((0, 4), dtype=torch.float32, device="cuda")
+    f.query(dev(torch, feat) if n else torch.zeros((0, 9), device="cuda"), dev(torch, u) if n else
+            torch.zeros((0, 4), device="cuda"), d_wi)
+    torch.cuda.synchronize()
+    if n:
+        # batch invariance: each row equals the row queried inside a bigger batch
+        big = 2048
+        featb = np.concatenate([feat, df_features(big - n, 42)], 0)
+        ub = np.concatenate([u, uniforms(big - n, 43)], 0)
+        d_wib = dev(torch, wi_init(big))
+        f.query(dev(torch, featb), dev(torch, ub), d_wib)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(d_wi), host(d_wib)[:n])
+    f.close()
+
+
+def test_errors(torch_cuda):
+    torch = torch_cuda
+    lib = G.load_library()
+    with pytest.raises(G.DfgError) as e:
+        G.GuidingField("{not json")
+    assert e.value.code == 3
+    with pytest.raises(G.DfgError) as e:
+        bad = G.hash_config()
+        bad["dfg"]["n_bins_theta"] = 24
+        G.GuidingField(bad)
+    assert e.value.code == 3
+    f = G.GuidingField(G.hash_config(with_nrc=False))
+    feat = dev(torch, df_features(10, 1))
+    with pytest.raises(G.DfgError) as e:
+        f.query(feat, dev(torch, uniforms(10, 1)), None)
+    assert e.value.code == 1
+    with pytest.raises(G.DfgError) as e:
+        f.query(feat[:, :5].contiguous(), dev(torch, uniforms(10, 1)), dev(torch, wi_init(10)))
+    assert e.value.code == 2
+    with pytest.raises(G.DfgError) as e:
+        f.nrc_infer(feat, dev(torch, np.zeros((10, 3), np.float32)))
+    assert e.value.code == 1
+    assert lib.dfg_query(None, None, 1, None, 9, None, None, None, None) == 6
+    f.close()
+
+
+def test_save_load_roundtrip(torch_cuda):
+    f = G.GuidingField(G.reference_config())
+    g = G.GuidingField(G.reference_config(seed=1))
+    with tempfile.TemporaryDirectory() as d:
+        pre = os.path.join(d, "net_")
+        f.save(pre)
+        assert os.path.getsize(pre + "AR_network_theta.bin") == 4 * f.n_params(0)
+        g.load(pre)
+    for k in (0, 1, 2):
+        assert np.array_equal(f.get_params(k), g.get_params(k))
+        assert np.array_equal(f.get_params(k, 2), g.get_params(k, 2))
+    f.close()
+    g.close()
+
+
+def test_prepare_features(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(50)
+    n = 1000
+    pos = rng.uniform(-3, 5, (n, 3)).astype(np.float32)
+    aabb = np.array([-3, -2, -1, 5, 4, 6], np.float32)
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    rough = rng.random(n, dtype=np.float32)
+    d_feat = torch.zeros((n, 9), dtype=torch.float32, device="cuda")
+    G.prepare_features(dev(torch, pos), dev(torch, aabb), dev(torch, wo), dev(torch, nrm), dev(torch, rough), d_feat)
+    torch.cuda.synchronize()
+    g = host(d_feat).astype(np.float64)
+    lo, hi = aabb[:3].astype(np.float64), aabb[3:].astype(np.float64)
+    infl = np.linalg.norm(hi - lo) * 0.005
+    lo, hi = lo - infl, hi + infl
+    ref = np.zeros((n, 9))
+    ref[:, :3] = 0.5 + (pos - (lo + hi) / 2) / (hi - lo)
+    w = wo / np.linalg.norm(wo, axis=1, keepdims=True)
+    ref[:, 3:6] = (w + 1) / 2
+    nn = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    ref[:, 6] = np.arccos(np.clip(nn[:, 2], -1, 1)) / math.pi
+    ph = np.arctan2(nn[:, 1], nn[:, 0])
+    ref[:, 7] = np.where(ph < 0, ph + 2 * math.pi, ph) / (2 * math.pi)
+    ref[:, 8] = 1 - np.exp(-rough)
+    assert np.abs(g - ref).max() < 1e-5
+
+
+# ----------------------------------------------------------------------------
+# Full-size properties (config 2 shape: 1,048,576 queries)
+# ----------------------------------------------------------------------------
+def test_full_size_properties(torch_cuda):
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=True))
+    n = 1 << 20
+    feat = dev(torch, df_features(n, 60, nrc=True))
+    u = dev(torch, uniforms(n, 61))
+    w1, w2 = dev(torch, wi_init(n)), dev(torch, wi_init(n))
+    rgb = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    f.query(feat, u, w1, nrc_rgb=rgb)
+    f.query(feat, u, w2)
+    torch.cuda.synchronize()
+    a, b = host(w1), host(w2)
+    assert np.array_equal(a, b), "query not deterministic / NRC flag changed the DF result"
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(host(rgb)))
+    assert a[:, :2].min() >= 0.0 and a[:, :2].max() <= 1.0 - 1e-5
+    assert np.all(a[:, 2:] > 0.0)
+    # a slice queried alone is identical to the same rows inside the full batch
+    k = 5000
+    w3 = dev(torch, wi_init(k))
+    f.query(feat[-k:].contiguous(), u[-k:].contiguous(), w3)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(w3), a[-k:])
+    f.close()

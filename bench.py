@@ -22,8 +22,9 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(ROOT, "guiding-df_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "guiding-df_amd"))
 
 METRIC = "guided queries/s + train samples/s vs plain FP32 path; p99 batch latency"
 HBM_PEAK_GBS = 8000.0
@@ -94,6 +95,10 @@ def main():
     ms_step = wall * 1000.0 / args.steps
     qps = n / (wall / args.steps)
     kernel_ms = float(ms.mean())  # one kernel launch per step: its average duration
+
+    # ---- breakdown: DF nets only, radiance cache only (same batch)
+    _, ms_df = timed(lambda: field.query(feat, u, wi, stream=stream), max(3, args.steps // 5), 2)
+    _, ms_nrc = timed(lambda: field.nrc_infer(feat, rgb, stream=stream), max(3, args.steps // 5), 2)
 
     # ---- plain FP32 per-batch path on the same batch
     psteps = max(3, args.steps // 5)
@@ -183,6 +188,8 @@ def main():
         "train_samples_per_s": train_sps,
         "train_ms_per_step": float(twall * 1000.0 / tsteps),
         "batch_latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
+        "breakdown_ms": {"df_only": float(np.median(ms_df)), "nrc_only": float(np.median(ms_nrc)),
+                         "plain_fp32": float(np.median(pms)), "train_step": float(np.median(tms))},
         "pdf_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,

@@ -276,6 +276,29 @@ static NetSpec build_net(const Json& enc, const Json& net, int in_dims, int n_ou
         }
         g.offset[g.n_levels] = o;
         g.param_begin = ns.n_matrix_params;
+        for (int l = 0; l < g.n_levels; ++l) {
+            // closed form of grid_index's uint32 stride loop (grid.h:169-186)
+            const uint32_t hsize = g.offset[l + 1] - g.offset[l];
+            uint32_t stride = 1, s[3] = {0, 0, 0};
+            for (int d = 0; d < 3 && stride <= hsize; ++d) {
+                s[d] = stride;
+                stride *= g.res[l];
+            }
+            g.s1[l] = s[1];
+            g.s2[l] = s[2];
+            const bool hashed = g.type == GRID_HASH && hsize < stride;
+            const bool pow2 = (hsize & (hsize - 1)) == 0;
+            g.hashed[l] = (hashed ? 1u : 0u) | (pow2 ? 2u : 0u);
+            g.mod_mask[l] = hsize - 1;
+            // Granlund & Montgomery (1994), unsigned division by an invariant d >= 2:
+            // l = ceil(log2 d), m' = floor(2^32 (2^l - d) / d) + 1, q = (t + ((n - t) >> 1)) >> (l - 1)
+            uint32_t lg = 0;
+            while ((1ull << lg) < hsize) ++lg;
+            if (!pow2) {
+                g.mod_mul[l] = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << lg) - hsize)) / hsize + 1);
+                g.mod_shift[l] = lg - 1;
+            }
+        }
     }
     ns.n_params = ns.n_matrix_params + (ns.grid.present ? ns.grid.offset[ns.grid.n_levels] * ns.grid.n_feat : 0);
     // fragment packs: forward then backward
@@ -591,12 +614,8 @@ int dfg_load(dfg_ctx* ctx, const char* prefix) {
 
 // --- query family ---------------------------------------------------------------
 static void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
-    int xs = 0;
-    if (with_df) {
-        xs = std::max(c->net[0].spec.k_steps, c->net[1].spec.k_steps) * 32 + 8;
-    }
-    if (with_nrc) xs = std::max(xs, c->net[2].spec.k_steps * 32 + 8);
-    a.xs = xs;
+    a.xs = with_df ? std::max(c->net[0].spec.k_steps, c->net[1].spec.k_steps) * 32 + 4 : 0;
+    a.xs_nrc = with_nrc ? c->net[2].spec.k_steps * 32 + 8 : 0;
     a.ls = std::max(std::max(c->n_theta, c->n_phi), 16) + 1;
 }
 
@@ -771,7 +790,7 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     a.x = s.x.p;
     a.act = s.act.p;
     a.out = s.out.p;
-    a.xs = s.spec.k_steps * 32 + 8;
+    a.xs = s.spec.k_steps * 32 + 4;
     a.ls = s.spec.n_out_pad + 1;
     launch_train_fwd(a, s.spec, st);
 }

@@ -153,24 +153,46 @@ struct GridCorners {
     float w[8];
 };
 
+// n % hsize of one level: mask for powers of two, else Granlund & Montgomery's
+// multiply-shift for an invariant divisor (exact for every uint32 n).
+__device__ __forceinline__ uint32_t level_mod(const GridSpec& g, int level, uint32_t n, uint32_t hsize) {
+    if (g.hashed[level] & 2u) return n & g.mod_mask[level];
+    const uint32_t t = __umulhi(g.mod_mul[level], n);
+    const uint32_t q = (t + ((n - t) >> 1)) >> g.mod_shift[level];
+    return n - q * hsize;
+}
+
+// The 8 corner indices and weights of one level.  Same values as grid_index3
+// on every corner, from the per-level closed form (hash or dense strides).
 __device__ __forceinline__ void grid_corners(const GridSpec& g, int level, float px, float py, float pz, GridCorners& c) {
 #pragma clang fp contract(off)
     const float scale = g.scale[level];
-    const uint32_t res = g.res[level];
     const uint32_t hsize = g.offset[level + 1] - g.offset[level];
     float fx, fy, fz;
     uint32_t cx, cy, cz;
     pos_fract(px, scale, g.interp, fx, cx);
     pos_fract(py, scale, g.interp, fy, cy);
     pos_fract(pz, scale, g.interp, fz, cz);
+    uint32_t ty[2], tz[2];
+    const bool hashed = g.hashed[level] & 1u;
+    if (hashed) {
+        ty[0] = cy * 2654435761u; ty[1] = (cy + 1) * 2654435761u;
+        tz[0] = cz * 805459861u;  tz[1] = (cz + 1) * 805459861u;
+    } else {
+        ty[0] = cy * g.s1[level]; ty[1] = (cy + 1) * g.s1[level];
+        tz[0] = cz * g.s2[level]; tz[1] = (cz + 1) * g.s2[level];
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         float w = 1.0f;
-        uint32_t x = cx, y = cy, z = cz;
-        if (k & 1) { w *= fx; x += 1; } else { w *= 1.0f - fx; }
-        if (k & 2) { w *= fy; y += 1; } else { w *= 1.0f - fy; }
-        if (k & 4) { w *= fz; z += 1; } else { w *= 1.0f - fz; }
-        c.idx[k] = grid_index3(g.type, hsize, res, x, y, z);
+        if (k & 1) { w *= fx; } else { w *= 1.0f - fx; }
+        if (k & 2) { w *= fy; } else { w *= 1.0f - fy; }
+        if (k & 4) { w *= fz; } else { w *= 1.0f - fz; }
+        const uint32_t x = cx + (uint32_t)(k & 1);
+        const uint32_t yz_h = ty[(k >> 1) & 1] ^ tz[k >> 2];
+        const uint32_t yz_d = ty[(k >> 1) & 1] + tz[k >> 2];
+        const uint32_t n = hashed ? (x ^ yz_h) : (x + yz_d);
+        c.idx[k] = level_mod(g, level, n, hsize);
         c.w[k] = w;
     }
 }
@@ -182,14 +204,9 @@ __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __
 #pragma clang fp contract(off)
     const _Float16* lt = table + (size_t)g.offset[level] * F;
     if (g.interp == INTERP_NEAREST) {
-        const float scale = g.scale[level];
-        const uint32_t hsize = g.offset[level + 1] - g.offset[level];
-        float fx, fy, fz;
-        uint32_t cx, cy, cz;
-        pos_fract(px, scale, INTERP_LINEAR, fx, cx);
-        pos_fract(py, scale, INTERP_LINEAR, fy, cy);
-        pos_fract(pz, scale, INTERP_LINEAR, fz, cz);
-        load_feats<F>(lt, grid_index3(g.type, hsize, g.res[level], cx, cy, cz), res);
+        GridCorners c;
+        grid_corners(g, level, px, py, pz, c);  // corner 0 is the cell's own vertex
+        load_feats<F>(lt, c.idx[0], res);
         return;
     }
     GridCorners c;
@@ -284,15 +301,16 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
             }
             case ENC_FREQUENCY: {
                 // frequency_encoding, tcnn frequency.h:46-94:
-                // sin(2^f pi x + (j%2) pi/2) evaluated in half-turns as
-                // sinpi(2 frac(x 2^(f-1) + (j%2)/4)), an exact argument reduction.
+                // sin(2^f pi x + (j%2) pi/2) = sin(2 pi r), r = frac(x 2^(f-1) + (j%2)/4);
+                // the reduction to one turn is exact, then v_sin_f32 (whose
+                // argument is in turns) evaluates the reduced angle.
                 const int nf = P.param;
                 for (int d = 0; d < P.n_dims; ++d) {
                     const float xin = in(P.dim_begin + d);
                     for (int j = 0; j < 2 * nf; ++j) {
                         const float v = ldexpf(xin, (j >> 1) - 1) + (float)(j & 1) * 0.25f;
                         const float fr = v - floorf(v);
-                        out(ob + d * 2 * nf + j, sinpif(2.0f * fr));
+                        out(ob + d * 2 * nf + j, __builtin_amdgcn_sinf(fr));
                     }
                 }
                 break;
@@ -507,6 +525,14 @@ __device__ __forceinline__ h8 ld_frag(const _Float16* __restrict__ pack, int t, 
     return *(const h8*)(pack + ((size_t)((t * ks + s) * 64 + lane) << 3));
 }
 
+// fp16 conversion that saturates at +-65504 instead of overflowing to inf.
+// Loss-scaled gradients of small batches can exceed the half range; the
+// reference's plain conversion would turn them into inf and the backward into
+// NaN.  NaN inputs stay NaN.
+__device__ __forceinline__ _Float16 sat_h(float v) {
+    return (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f);
+}
+
 // relu(C) of 4 M-tiles (64 neurons) -> the two K-step operands of the next layer.
 __device__ __forceinline__ void relu_to_act(const f4 (&c)[4], h8 (&a)[2]) {
 #pragma unroll
@@ -519,7 +545,7 @@ __device__ __forceinline__ void c_to_act(const f4 (&c)[4], h8 (&a)[2]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[s][j] = (_Float16)c[2 * s + (j >> 2)][j & 3];
+        for (int j = 0; j < 8; ++j) a[s][j] = sat_h(c[2 * s + (j >> 2)][j & 3]);
 }
 
 // Optional per-layer activation store (training): feature-major [64][ld] fp16.
@@ -530,48 +556,101 @@ struct ActStore {
     int64_t row0;       // first batch row of this tile
 };
 
-// Forward of a whole MLP for a 64-row tile.  xt: encoded input in LDS,
-// row-major with stride xs halves.  Writes the fp32 output of the last layer to
-// `lo` (LDS, row-major stride ls floats) for columns < n_out_pad.
-template <int NT_OUT>
-__device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
-                                                 const _Float16* xt, int xs, float* lo, int ls,
-                                                 const ActStore* st) {
+// fp32 -> (hi, lo) fp16 pair with hi + lo == v to ~2^-22 relative: the MFMA
+// then sees the activation at fp32-level precision through two products.
+__device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
+
+// relu(C) of 4 M-tiles -> (hi, lo) operands of the next layer.
+__device__ __forceinline__ void relu_to_act2(const f4 (&c)[4], h8 (&hi)[2], h8 (&lo)[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            _Float16 h, l;
+            split_h(fmaxf(c[2 * s + (j >> 2)][j & 3], 0.0f), h, l);
+            hi[s][j] = h;
+            lo[s][j] = l;
+        }
+}
+
+// Forward of a whole MLP for a 64-row tile.  xt: encoded input in LDS as fp32,
+// row-major with stride xs floats (multiple of 4).  Writes the fp32 output of
+// the last layer to `lo` (LDS, row-major stride ls floats).
+// SPLIT: every activation (the encoded input included) enters the MFMA as an
+// fp16 (hi, lo) pair, so the only rounding left is the fp32 accumulation and
+// the fp16 weights themselves; otherwise activations are rounded to fp16
+// (the reference's storage precision, used by the training forward).
+template <int NT_OUT, bool SPLIT, int QN, typename TT>
+__device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
+                                                   const TT* xt, int xs, float* lo, int ls,
+                                                   const ActStore* st, int q0) {
+    // an fp16 input tile has no low part to add
+    constexpr bool SPLIT_IN = SPLIT && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, g = lane >> 4;
-    f4 acc[4][4];
-    h8 act[4][2];
+    f4 acc[QN][4];
+    h8 act[QN][2];
+    h8 actl[QN][2];
     // layer 0: B operand from the LDS tile, natural K order
     {
         const _Float16* pk = fwd_pack + ns.fwd_off[0];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < QN; ++q)
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
         for (int s = 0; s < ns.k_steps; ++s) {
-            h8 b[4];
+            h8 b[QN], bl[QN];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) b[q] = *(const h8*)(xt + (q * 16 + i16) * xs + s * 32 + g * 8);
+            for (int q = 0; q < QN; ++q) {
+                const TT* src = xt + ((q0 + q) * 16 + i16) * xs + s * 32 + g * 8;
+                if constexpr (sizeof(TT) == 2) {
+                    b[q] = *(const h8*)src;
+                } else {
+                    const float4 v0 = *(const float4*)src;
+                    const float4 v1 = *(const float4*)(src + 4);
+                    const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (SPLIT_IN) {
+                            _Float16 h, l;
+                            split_h(v[j], h, l);
+                            b[q][j] = h;
+                            bl[q][j] = l;
+                        } else {
+                            b[q][j] = (_Float16)v[j];
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const h8 a = ld_frag(pk, t, s, ns.k_steps, lane);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q][t] = mfma16(a, b[q], acc[q][t]);
+                for (int q = 0; q < QN; ++q) {
+                    acc[q][t] = mfma16(a, b[q], acc[q][t]);
+                    if (SPLIT_IN) acc[q][t] = mfma16(a, bl[q], acc[q][t]);
+                }
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) relu_to_act(acc[q], act[q]);
+        for (int q = 0; q < QN; ++q) {
+            if (SPLIT) relu_to_act2(acc[q], act[q], actl[q]);
+            else relu_to_act(acc[q], act[q]);
+        }
     }
     if (st) {
         _Float16* a1 = st->act;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < QN; ++q)
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
-                    a1[(size_t)n * st->ld + st->row0 + q * 16 + i16] = act[q][s][j];
+                    a1[(size_t)n * st->ld + st->row0 + (q0 + q) * 16 + i16] = act[q][s][j];
                 }
     }
     // hidden layers 1..H-1
@@ -582,23 +661,31 @@ __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float
             const h8 a0 = ld_frag(pk, t, 0, 2, lane);
             const h8 a1 = ld_frag(pk, t, 1, 2, lane);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < QN; ++q) {
                 f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
-                acc[q][t] = mfma16(a1, act[q][1], c);
+                c = mfma16(a1, act[q][1], c);
+                if (SPLIT) {
+                    c = mfma16(a0, actl[q][0], c);
+                    c = mfma16(a1, actl[q][1], c);
+                }
+                acc[q][t] = c;
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) relu_to_act(acc[q], act[q]);
+        for (int q = 0; q < QN; ++q) {
+            if (SPLIT) relu_to_act2(acc[q], act[q], actl[q]);
+            else relu_to_act(acc[q], act[q]);
+        }
         if (st) {
             _Float16* al = st->act + (size_t)l * 64 * st->ld;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < QN; ++q)
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
-                        al[(size_t)n * st->ld + st->row0 + q * 16 + i16] = act[q][s][j];
+                        al[(size_t)n * st->ld + st->row0 + (q0 + q) * 16 + i16] = act[q][s][j];
                     }
         }
     }
@@ -610,15 +697,29 @@ __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float
             const h8 a0 = ld_frag(pk, t, 0, 2, lane);
             const h8 a1 = ld_frag(pk, t, 1, 2, lane);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < QN; ++q) {
                 f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
                 c = mfma16(a1, act[q][1], c);
-                float* dst = lo + (q * 16 + i16) * ls + 16 * t + 4 * g;
+                if (SPLIT) {
+                    c = mfma16(a0, actl[q][0], c);
+                    c = mfma16(a1, actl[q][1], c);
+                }
+                float* dst = lo + ((q0 + q) * 16 + i16) * ls + 16 * t + 4 * g;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) dst[r] = c[r];
             }
         }
     }
+}
+
+// The 64-row tile runs through the MLP in passes of QN 16-row groups (fewer
+// live accumulators per pass keeps the occupancy up; the weight fragments are
+// re-read from L1 per pass).
+template <int NT_OUT, bool SPLIT, int QN = (SPLIT ? 2 : 4), typename TT>
+__device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
+                                                 const TT* xt, int xs, float* lo, int ls,
+                                                 const ActStore* st) {
+    for (int q0 = 0; q0 < 4; q0 += QN) mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0);
 }
 
 }  // namespace dfg

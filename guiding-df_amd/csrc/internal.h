@@ -32,8 +32,9 @@ struct QueryArgs {
     float* logits_th;    // debug export [n][n_out_th] or NULL
     float* logits_ph;
     const float* theta_in;  // debug: phi net theta input [n] (logits mode)
-    int xs;              // LDS strides (halves / floats)
-    int ls;
+    int xs;              // LDS row strides: DF encoded input (floats),
+    int xs_nrc;          // NRC encoded input (halves),
+    int ls;              // logits (floats)
 };
 
 // Arguments of the training forward kernel of one net.

@@ -46,15 +46,15 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
 __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    _Float16* xt = (_Float16*)smem;
+    float* xt = (float*)smem;
     float* lo = (float*)smem;
     const int lane = threadIdx.x;
     const int64_t row = (int64_t)blockIdx.x * 64 + lane;
     const bool valid = row < a.n;
     const int64_t r = valid ? row : a.n - 1;
     const float* frow = a.feat + r * a.stride;
-    _Float16* const xrow = xt + lane * a.xs;
-    auto wr = [xrow](int c, float v) { xrow[c] = (_Float16)v; };
+    float* const xrow = xt + lane * a.xs;
+    auto wr = [xrow](int c, float v) { xrow[c] = v; };
     float* lrow = lo + lane * a.ls;
 
   if constexpr (WITH_DF) {
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
 
     // ---- theta net
     encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr);
-    mlp_forward_tile<NTH / 16>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
+    mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_th && valid) {
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
 
     // ---- phi net (its own encoding, theta appended as the last raw input)
     encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr);
-    mlp_forward_tile<NPH / 16>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
+    mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_ph && valid) {
 #pragma unroll
         for (int i = 0; i < NPH; ++i) a.logits_ph[r * NPH + i] = lrow[i];
@@ -104,8 +104,11 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
 
     // ---- radiance cache
     if constexpr (WITH_NRC) {
-        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, wr);
-        mlp_forward_tile<1>(*a.nrc, a.pack_nrc, xt, a.xs, lo, a.ls, nullptr);
+        // the radiance cache keeps its (widest) encoded input in fp16
+        _Float16* const xh = (_Float16*)smem;
+        _Float16* const xhrow = xh + lane * a.xs_nrc;
+        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; });
+        mlp_forward_tile<1, true>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
         if (valid) {
             a.rgb[3 * r + 0] = lrow[0];
             a.rgb[3 * r + 1] = lrow[1];
@@ -117,9 +120,11 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
 template <int NTH, int NPH>
 static void launch_q(const QueryArgs& a, hipStream_t s) {
     const unsigned blocks = (unsigned)((a.n + 63) / 64);
-    size_t lds = (size_t)64 * a.xs * 2;
+    size_t lds = (size_t)64 * a.xs * 4;
     const size_t l2 = (size_t)64 * a.ls * 4;
+    const size_t l3 = (size_t)64 * a.xs_nrc * 2;
     if (l2 > lds) lds = l2;
+    if (l3 > lds) lds = l3;
     if (a.th == nullptr)
         hipLaunchKernelGGL((query_fused_kernel<16, 16, false, true>), dim3(blocks), dim3(64), lds, s, a);
     else if (a.rgb)

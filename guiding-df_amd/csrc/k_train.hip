@@ -24,7 +24,7 @@ struct RowInT {
 template <int NT_OUT, int NBINS>
 __global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    _Float16* xt = (_Float16*)smem;
+    float* xt = (float*)smem;
     float* lo = (float*)smem;
     const NetSpec& ns = *a.ns;
     const int lane = threadIdx.x;
@@ -39,12 +39,13 @@ __global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
         th_in = theta_input(a.target[r * 9], a.theta_bins, a.linear);
         base = ns.in_dims - 1;
     }
-    _Float16* const xrow = xt + lane * a.xs;
-    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (_Float16)v; });
+    float* const xrow = xt + lane * a.xs;
+    // the training forward feeds fp16 activations, like the reference's
+    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (float)(_Float16)v; });
     const int kc = ns.k_steps * 32;
-    for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = xrow[c];
+    for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = (_Float16)xrow[c];
     ActStore st{nullptr, a.act, a.ld, row0};
-    mlp_forward_tile<NT_OUT>(ns, a.pack, xt, a.xs, lo, a.ls, &st);
+    mlp_forward_tile<NT_OUT, false>(ns, a.pack, xt, a.xs, lo, a.ls, &st);
     const float* lrow = lo + lane * a.ls;
     _Float16* o = a.out + row * (NT_OUT * 16);
     if (a.mode == 0) {
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
 void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s) {
     if (a.n <= 0) return;
     const unsigned blocks = (unsigned)(a.ld / 64);
-    size_t lds = (size_t)64 * a.xs * 2;
+    size_t lds = (size_t)64 * a.xs * 4;
     const size_t l2 = (size_t)64 * a.ls * 4;
     if (l2 > lds) lds = l2;
     const int nt = hs.n_out_pad / 16;
@@ -116,7 +117,7 @@ __device__ __forceinline__ void df_grad(float x, float pdf_other, float prefix, 
     }
 #pragma unroll
     for (int i = 0; i < NPAD; ++i) {
-        const _Float16 h = i < N ? (_Float16)g[i < N ? i : 0] : (_Float16)0.0f;
+        const _Float16 h = i < N ? sat_h(g[i < N ? i : 0]) : (_Float16)0.0f;
         rm[row * NPAD + i] = h;
         fm[(size_t)i * ld + row] = h;
     }
@@ -185,7 +186,7 @@ __global__ void nrc_loss_kernel(const NrcLossArgs a) {
             const float diff = (float)p[c] - a.target[row * 3 + c];
             lsum += diff * diff / den / 1.0f / n_total;
             const float grad = 2.0f * diff / den / 1.0f;
-            gh = (_Float16)(a.loss_scale * grad / n_total);
+            gh = sat_h(a.loss_scale * grad / n_total);
         }
         a.gout_rm[row * 16 + c] = gh;
         a.gout_fm[(size_t)c * a.ld + row] = gh;
@@ -212,7 +213,7 @@ __device__ __forceinline__ void mask_store(f4 (&c)[4], const _Float16* __restric
             const float av = (float)act[(size_t)n * ld + row];
             const float v = av > 0.0f ? c[t][r] : 0.0f;
             c[t][r] = v;
-            gst[(size_t)n * ld + row] = (_Float16)v;
+            gst[(size_t)n * ld + row] = sat_h(v);
         }
 }
 
@@ -389,18 +390,12 @@ __global__ void grid_bwd_kernel(const GridBwdArgs a) {
     float* gg = a.grad + gs.param_begin + (size_t)gs.offset[level] * F;
     float d[8];
     for (int f = 0; f < F; ++f) d[f] = a.dx[(size_t)(level * F + f) * a.ld + row];
-    if (gs.interp == INTERP_NEAREST) {
-        float fx, fy, fz;
-        uint32_t cx, cy, cz;
-        pos_fract(fr[0], gs.scale[level], INTERP_LINEAR, fx, cx);
-        pos_fract(fr[1], gs.scale[level], INTERP_LINEAR, fy, cy);
-        pos_fract(fr[2], gs.scale[level], INTERP_LINEAR, fz, cz);
-        const uint32_t idx = grid_index3(gs.type, gs.offset[level + 1] - gs.offset[level], gs.res[level], cx, cy, cz);
-        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)idx * F + f, d[f]);
-        return;
-    }
     GridCorners c;
     grid_corners(gs, level, fr[0], fr[1], fr[2], c);
+    if (gs.interp == INTERP_NEAREST) {
+        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[0] * F + f, d[f]);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[k] * F + f, c.w[k] * d[f]);

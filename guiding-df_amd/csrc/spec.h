@@ -45,6 +45,17 @@ struct GridSpec {
     uint32_t offset[DFG_MAX_LEVELS + 1];  // level offsets in table entries
     uint32_t res[DFG_MAX_LEVELS];     // vertices per axis
     float scale[DFG_MAX_LEVELS];      // grid_scale(level)
+    // Per-level closed form of grid_index (precomputed on the host from the
+    // reference's uint32 stride loop): dense index x + y*s1 + z*s2 (uint32
+    // wrap, s = 0 for dims the loop never reached) or the CoherentPrime hash,
+    // then "% hsize" by mask (power of two) or by the Granlund-Montgomery
+    // multiply-shift for invariant divisors.
+    uint32_t s1[DFG_MAX_LEVELS];
+    uint32_t s2[DFG_MAX_LEVELS];
+    uint32_t hashed[DFG_MAX_LEVELS];
+    uint32_t mod_mask[DFG_MAX_LEVELS];   // hsize - 1 when hsize is a power of two, else 0
+    uint32_t mod_mul[DFG_MAX_LEVELS];    // Granlund-Montgomery m'
+    uint32_t mod_shift[DFG_MAX_LEVELS];  // l - 1
 };
 
 struct NetSpec {

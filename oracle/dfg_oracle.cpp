@@ -684,8 +684,11 @@ static void dpdf_dz(const double* p, int N, double x, int linear, double* out) {
     }
 }
 
+// half_bins: store the softmax in fp16 before the xN like the reference's
+// training forward (softmaxMultFactor with precision_t = __half, train.h:874-893).
 double orc_df_train_f64(const OrcNet* th, const OrcNet* ph, const double* pth, const double* pph, const float* feat,
-                        int stride, int D, const float* target, int64_t B, int linear, double* gth, double* gph) {
+                        int stride, int D, const float* target, int64_t B, int linear, int half_bins, double* gth,
+                        double* gph) {
     const double eps = (double)1e-5f;
     const double inv4pi = (double)0.07957747154594766788f;
     if (gth) memset(gth, 0, sizeof(double) * th->n_params);
@@ -704,6 +707,10 @@ double orc_df_train_f64(const OrcNet* th, const OrcNet* ph, const double* pth, c
         mlp_row<double>(*ph, pph, xp.data(), lp.data(), ap.data());
         softmax_times_n<double>(lt.data(), th->n_out, bt.data());
         softmax_times_n<double>(lp.data(), ph->n_out, bp.data());
+        if (half_bins) {
+            for (int j = 0; j < th->n_out; ++j) bt[j] = (double)half_round((float)(bt[j] / th->n_out)) * th->n_out;
+            for (int j = 0; j < ph->n_out; ++j) bp[j] = (double)half_round((float)(bp[j] / ph->n_out)) * ph->n_out;
+        }
         const double x0 = t[0], x1 = t[1];
         const double pt = linear ? pdf_linear<double>(bt.data(), th->n_out, x0) : pdf_nearest<double>(bt.data(), th->n_out, x0);
         const double pp = linear ? pdf_linear<double>(bp.data(), ph->n_out, x1) : pdf_nearest<double>(bp.data(), ph->n_out, x1);

@@ -228,7 +228,7 @@ def lib():
             "orc_query_f32_omp": ([P, P, P, P, P, I, I, I64, P, P, P, I, P], I),
             "orc_nrc_f64": ([P, P, P, I, I64, P], None),
             "orc_nrc_f32_omp": ([P, P, P, I, I64, P], None),
-            "orc_df_train_f64": ([P, P, P, P, P, I, I, P, I64, I, P, P], D),
+            "orc_df_train_f64": ([P, P, P, P, P, I, I, P, I64, I, I, P, P], D),
             "orc_nrc_train_f64": ([P, P, P, I, P, I64, P], D),
             "orc_adam_ema_f64": ([I64, I64, D, D, D, D, D, D, I, P, P, P, P, P, P, P], None),
         }
@@ -349,14 +349,14 @@ def nrc_f32_omp(L, params, feat):
     return out
 
 
-def df_train_f64(Lt, Lp, pt, pp, feat, D, target, linear, grads=True):
+def df_train_f64(Lt, Lp, pt, pp, feat, D, target, linear, grads=True, half_bins=False):
     st, sp = to_struct(Lt), to_struct(Lp)
     pt, pp = _c(pt, np.float64), _c(pp, np.float64)
     feat, target = _c(feat, np.float32), _c(target, np.float32)
     gt = np.zeros(Lt["n_params"], np.float64) if grads else None
     gp = np.zeros(Lp["n_params"], np.float64) if grads else None
     loss = lib().orc_df_train_f64(ctypes.byref(st), ctypes.byref(sp), _p(pt), _p(pp), _p(feat), feat.shape[1], D,
-                                  _p(target), feat.shape[0], int(linear), _p(gt), _p(gp))
+                                  _p(target), feat.shape[0], int(linear), int(half_bins), _p(gt), _p(gp))
     return loss, gt, gp
 
 

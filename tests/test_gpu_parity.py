@@ -112,8 +112,10 @@ def test_encode_matches_fp64(torch_cuda, cfgname):
         params = f.get_params(net, G.DFG_PARAMS_INFERENCE).astype(np.float64)
         o = O.encode_f64(L, params, x.astype(np.float64))
         err = np.abs(host(d_out) - o)
-        # sin/cos arguments up to 2^11 pi carry fp32 argument rounding (~6e-4)
-        tol = 2e-3 if net == G.DFG_NET_NRC else 2e-5
+        # fp32 argument rounding of the high frequencies, as in the reference:
+        # TriangleWave x 2^10 + f/4 rounds at ulp(1024) (x4 -> 5e-4); sin/cos
+        # arguments up to 2^11 pi (~6e-4)
+        tol = {G.DFG_NET_THETA: 2e-5, G.DFG_NET_PHI: 6e-4, G.DFG_NET_NRC: 2e-3}[net]
         assert err.max() < tol, "net %d encode max err %.3g" % (net, err.max())
     f.close()
 
@@ -277,7 +279,11 @@ def test_train_step_vs_fp64(torch_cuda, cfgname):
     torch.cuda.synchronize()
     loss_g = float(host(d_loss)[0])
     loss_o, gt_o, gp_o = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1)
-    assert abs(loss_g - loss_o) <= 2e-3 * abs(loss_o) + 1e-6, (loss_g, loss_o)
+    # the reference keeps training bins in fp16; with that storage emulated the
+    # loss agrees tightly (log q near q = 1 amplifies the fp16 rounding)
+    loss_h, _, _ = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1, grads=False, half_bins=True)
+    assert abs(loss_g - loss_h) <= 2e-3 * abs(loss_h) + 1e-6, (loss_g, loss_h, loss_o)
+    assert abs(loss_g - loss_o) <= 2e-2 * abs(loss_o), (loss_g, loss_o)
     for k, go, L in ((0, gt_o, th), (1, gp_o, ph)):
         gg = f.get_grad(k).astype(np.float64) / 1024.0
         nm = L["n_matrix"]
@@ -304,9 +310,10 @@ def test_nrc_train_step_vs_fp64(torch_cuda):
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
     f = G.GuidingField(cfg)
     _, _, L = O.config_layouts(cfg)
-    B = 1000
+    # targets and batch keep the loss-scaled fp16 gradients inside the half range
+    B = 4096
     feat = df_features(B, 30, nrc=True)
-    tgt = np.exp(np.random.default_rng(31).normal(0, 2, (B, 3))).astype(np.float32)
+    tgt = np.exp(np.random.default_rng(31).normal(0, 1, (B, 3))).astype(np.float32)
     w = f.get_params(2, G.DFG_PARAMS_TRAIN)
     d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     f.nrc_train_step(dev(torch, feat), dev(torch, tgt), d_loss)

@@ -354,12 +354,12 @@ struct NetState {
     DevBuf<uint32_t> steps;
     // training workspace
     DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
-    DevBuf<float> dx, partial;
+    DevBuf<float> dx, partial, gfeat;
     void release_all() {
         d_spec.release(); master.release(); m.release(); v.release(); grad.release();
         w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
         x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();
-        dx.release(); partial.release();
+        dx.release(); partial.release(); gfeat.release();
     }
     const _Float16* tab_infer() const { return spec.grid.present ? ema16.p + spec.grid.param_begin : nullptr; }
     const _Float16* tab_train() const { return spec.grid.present ? w16.p + spec.grid.param_begin : nullptr; }
@@ -373,6 +373,7 @@ struct dfg_ctx {
     NetState net[3];
     int n_theta = 16, n_phi = 32, linear = 1, D = 9;
     uint64_t seed = 7272;
+    int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
     std::string err;
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
@@ -464,6 +465,7 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         c->linear = interp == "linear";
         c->seed = (uint64_t)d.number_or("seed", 7272);
         c->D = (int)d.number_or("n_input_dims", 9);
+        c->prepass_min_rows = (int64_t)d.number_or("grid_prepass_min_rows", 32768);
         auto okb = [](int b) { return b == 16 || b == 32 || b == 64; };
         if (!okb(c->n_theta) || !okb(c->n_phi)) throw Error(DFG_ERR_CONFIG, "bins must be 16, 32 or 64");
         const bool pair_ok = (c->n_theta == 16 && (c->n_phi == 16 || c->n_phi == 32)) ||
@@ -637,6 +639,21 @@ static QueryArgs base_args(dfg_ctx* c) {
     return a;
 }
 
+// XCD-aware grid pre-pass of both DF nets for n rows (when worth a launch).
+static void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
+    if (n < c->prepass_min_rows) return;
+    const int64_t ld = round_up(n, 64);
+    for (int i = 0; i < 2; ++i) {
+        NetState& s = c->net[i];
+        if (!s.spec.grid.present) continue;
+        const size_t cols = (size_t)s.spec.grid.n_levels * s.spec.grid.n_feat;
+        s.gfeat.ensure(cols * ld);
+        launch_grid_encode(s.d_spec.p, s.spec, s.tab_infer(), feat, stride, n, ld, s.gfeat.p, st);
+        (i == 0 ? a.gfeat_th : a.gfeat_ph) = s.gfeat.p;
+    }
+    a.gld = ld;
+}
+
 static void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride) {
     if (n < 0 || n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "n out of range");
     if (n > 0 && !feat) throw Error(DFG_ERR_INVALID_ARG, "null features");
@@ -655,6 +672,7 @@ int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.u = u; a.guided = guided; a.wi = wi; a.rgb = nrc_rgb;
     fill_lds_strides(ctx, a, nrc_rgb != nullptr, true);
+    grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)
@@ -669,6 +687,7 @@ int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_s
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.wi2 = wi2; a.pdf_out = pdf;
     fill_lds_strides(ctx, a, false, true);
+    grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)
@@ -685,6 +704,7 @@ int dfg_logits(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int fea
     a.feat = feat; a.stride = feat_stride; a.n = n; a.theta_in = theta_in;
     a.logits_th = logits_theta; a.logits_ph = logits_phi;
     fill_lds_strides(ctx, a, false, true);
+    grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)
@@ -792,6 +812,11 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     a.out = s.out.p;
     a.xs = s.spec.k_steps * 32 + 4;
     a.ls = s.spec.n_out_pad + 1;
+    if (s.spec.grid.present) {
+        s.gfeat.ensure((size_t)s.spec.grid.n_levels * s.spec.grid.n_feat * ld);
+        launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), feat, stride, n, ld, s.gfeat.p, st);
+        a.gfeat = s.gfeat.p;
+    }
     launch_train_fwd(a, s.spec, st);
 }
 

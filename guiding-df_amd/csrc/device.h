@@ -255,7 +255,11 @@ __device__ __forceinline__ void sh_encode(int degree, float ix, float iy, float 
 // Out: callable out(col, float).  Writes enc_width columns and zero-fills up to
 // 32 * k_steps (the MFMA K padding, which has zero weights).
 template <typename In, typename Out>
-__device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out) {
+// gfeat (optional): this row's grid features already computed by the
+// XCD-aware grid kernel, feature-major with column stride gstride; when given
+// the grid part copies them instead of gathering.
+__device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out,
+                                           const float* __restrict__ gfeat = nullptr, int64_t gstride = 0) {
 #pragma clang fp contract(off)
     for (int pi = 0; pi < ns.n_parts; ++pi) {
         const EncPart& P = ns.parts[pi];
@@ -317,8 +321,12 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
             }
             case ENC_GRID: {
                 const GridSpec& g = ns.grid;
-                const float px = in(P.dim_begin), py = in(P.dim_begin + 1), pz = in(P.dim_begin + 2);
                 const int F = g.n_feat;
+                if (gfeat) {
+                    for (int c = 0; c < g.n_levels * F; ++c) out(ob + c, gfeat[(size_t)c * gstride]);
+                    break;
+                }
+                const float px = in(P.dim_begin), py = in(P.dim_begin + 1), pz = in(P.dim_begin + 2);
                 for (int l = 0; l < g.n_levels; ++l) {
                     if (F == 2) {
                         float r[2]; grid_level<2>(g, table, l, px, py, pz, r);
@@ -720,6 +728,32 @@ __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float
                                                  const TT* xt, int xs, float* lo, int ls,
                                                  const ActStore* st) {
     for (int q0 = 0; q0 < 4; q0 += QN) mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0);
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware grid encoding.  Workgroups are dispatched round-robin over the
+// XCDs, so block b runs on XCD b % n_groups; giving that block only the levels
+// l with l % n_groups == b % n_groups keeps every XCD's share of the table
+// (2 of 16 levels, <= 4 MB) resident in its own L2 instead of streaming all
+// 16 levels of a 28.5 MB table from the Infinity Cache.  One row per thread;
+// output feature-major fp32 [L*F][ld].
+// ---------------------------------------------------------------------------
+template <int F>
+__device__ __forceinline__ void grid_encode_block(const NetSpec& ns, const _Float16* __restrict__ table,
+                                                  const float* __restrict__ feat, int stride, int pos_dim, int64_t n,
+                                                  int64_t ld, float* __restrict__ out, int n_groups) {
+    const GridSpec& g = ns.grid;
+    const int grp = blockIdx.x % n_groups;
+    const int64_t row = (int64_t)(blockIdx.x / n_groups) * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const float* fr = feat + row * stride + pos_dim;
+    const float px = fr[0], py = fr[1], pz = fr[2];
+    for (int l = grp; l < g.n_levels; l += n_groups) {
+        float r[F];
+        grid_level<F>(g, table, l, px, py, pz, r);
+#pragma unroll
+        for (int f = 0; f < F; ++f) out[(size_t)(l * F + f) * ld + row] = r[f];
+    }
 }
 
 }  // namespace dfg

@@ -8,6 +8,10 @@
 
 namespace dfg {
 
+// Level groups of the XCD-aware grid kernels: MI355X dispatches workgroups
+// round-robin over its 8 XCDs.
+constexpr int kXcdGroups = 8;
+
 // Arguments of the fused query kernel (one launch per wavefront stage).
 struct QueryArgs {
     const NetSpec* th;         // device-resident specs (see DevSpecs)
@@ -32,6 +36,9 @@ struct QueryArgs {
     float* logits_th;    // debug export [n][n_out_th] or NULL
     float* logits_ph;
     const float* theta_in;  // debug: phi net theta input [n] (logits mode)
+    const float* gfeat_th;   // grid features from the XCD-aware pre-pass
+    const float* gfeat_ph;   // (feature-major [L*F][gld]) or NULL: gather in-kernel
+    int64_t gld;
     int xs;              // LDS row strides: DF encoded input (floats),
     int xs_nrc;          // NRC encoded input (halves),
     int ls;              // logits (floats)
@@ -53,10 +60,13 @@ struct TrainFwdArgs {
     _Float16* x;            // [k_steps*32][ld]
     _Float16* act;          // [H][64][ld]
     _Float16* out;          // row-major [ld][n_out_pad] (bins or prediction)
+    const float* gfeat;     // grid features from the pre-pass or NULL
     int xs, ls;
 };
 
 void launch_query(const QueryArgs& a, int n_theta, int n_phi, hipStream_t s);
+void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* feat, int stride,
+                        int64_t n, int64_t ld, float* out, hipStream_t s);
 // Host copies of the specs are passed beside the args where launch shapes depend on them.
 void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s);
 

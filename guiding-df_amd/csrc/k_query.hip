@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
     const int D = th.in_dims;
 
     // ---- theta net
-    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr);
+    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + r : nullptr, a.gld);
     mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_th && valid) {
 #pragma unroll
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
 
     // ---- phi net (its own encoding, theta appended as the last raw input)
-    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr);
+    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + r : nullptr, a.gld);
     mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_ph && valid) {
 #pragma unroll
@@ -141,6 +141,32 @@ void launch_query(const QueryArgs& a, int nt, int np, hipStream_t s) {
     if (nt == 64 && np == 64) return launch_q<64, 64>(a, s);
     if (nt == 16 && np == 16) return launch_q<16, 16>(a, s);
     if (nt == 32 && np == 64) return launch_q<32, 64>(a, s);
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware grid encoding pre-pass (see grid_encode_block).
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(256) void grid_encode_kernel(const NetSpec* __restrict__ dns, const _Float16* table,
+                                                          const float* feat, int stride, int pos_dim, int64_t n,
+                                                          int64_t ld, float* out, int n_groups) {
+    grid_encode_block<F>(*dns, table, feat, stride, pos_dim, n, ld, out, n_groups);
+}
+
+void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* feat, int stride,
+                        int64_t n, int64_t ld, float* out, hipStream_t s) {
+    if (n <= 0 || !hs.grid.present) return;
+    int pos_dim = 0;
+    for (int i = 0; i < hs.n_parts; ++i)
+        if (hs.parts[i].type == ENC_GRID) pos_dim = hs.parts[i].dim_begin;
+    const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
+    const unsigned blocks = (unsigned)(((n + 255) / 256) * groups);
+    switch (hs.grid.n_feat) {
+        case 1: hipLaunchKernelGGL((grid_encode_kernel<1>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
+        case 2: hipLaunchKernelGGL((grid_encode_kernel<2>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
+        case 4: hipLaunchKernelGGL((grid_encode_kernel<4>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
+        default: hipLaunchKernelGGL((grid_encode_kernel<8>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -41,7 +41,8 @@ __global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
     }
     float* const xrow = xt + lane * a.xs;
     // the training forward feeds fp16 activations, like the reference's
-    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (float)(_Float16)v; });
+    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (float)(_Float16)v; },
+               a.gfeat ? a.gfeat + r : nullptr, a.ld);
     const int kc = ns.k_steps * 32;
     for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = (_Float16)xrow[c];
     ActStore st{nullptr, a.act, a.ld, row0};
@@ -379,32 +380,36 @@ void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s) {
 // Grid backward: scatter w * dL/dy into the table gradient (kernel_grid_backward,
 // grid.h:394-520), fp32 atomics.
 // ---------------------------------------------------------------------------
-__global__ void grid_bwd_kernel(const GridBwdArgs a) {
+__global__ void grid_bwd_kernel(const GridBwdArgs a, int n_groups) {
     const NetSpec& ns = *a.ns;
     const GridSpec& gs = ns.grid;
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int level = blockIdx.y;
+    // XCD-aware: block b scatters only into levels l % n_groups == b % n_groups,
+    // so each XCD's atomics stay inside its own slice of the table
+    const int grp = blockIdx.x % n_groups;
+    const int64_t row = (int64_t)(blockIdx.x / n_groups) * blockDim.x + threadIdx.x;
     if (row >= a.n) return;
     const float* fr = a.feat + row * a.stride + a.pos_dim;
     const int F = gs.n_feat;
-    float* gg = a.grad + gs.param_begin + (size_t)gs.offset[level] * F;
-    float d[8];
-    for (int f = 0; f < F; ++f) d[f] = a.dx[(size_t)(level * F + f) * a.ld + row];
-    GridCorners c;
-    grid_corners(gs, level, fr[0], fr[1], fr[2], c);
-    if (gs.interp == INTERP_NEAREST) {
-        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[0] * F + f, d[f]);
-        return;
-    }
+    for (int level = grp; level < gs.n_levels; level += n_groups) {
+        float* gg = a.grad + gs.param_begin + (size_t)gs.offset[level] * F;
+        float d[8];
+        for (int f = 0; f < F; ++f) d[f] = a.dx[(size_t)(level * F + f) * a.ld + row];
+        GridCorners c;
+        grid_corners(gs, level, fr[0], fr[1], fr[2], c);
+        if (gs.interp == INTERP_NEAREST) {
+            for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[0] * F + f, d[f]);
+            continue;
+        }
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[k] * F + f, c.w[k] * d[f]);
+        for (int k = 0; k < 8; ++k)
+            for (int f = 0; f < F; ++f) unsafeAtomicAdd(gg + (size_t)c.idx[k] * F + f, c.w[k] * d[f]);
+    }
 }
 
 void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s) {
     if (a.n <= 0 || !hs.grid.present) return;
-    hipLaunchKernelGGL(grid_bwd_kernel, dim3((unsigned)((a.n + 255) / 256), (unsigned)hs.grid.n_levels), dim3(256), 0,
-                       s, a);
+    const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
+    hipLaunchKernelGGL(grid_bwd_kernel, dim3((unsigned)(((a.n + 255) / 256) * groups)), dim3(256), 0, s, a, groups);
 }
 
 // ---------------------------------------------------------------------------

@@ -229,6 +229,31 @@ def test_plain_path_matches(torch_cuda, cfgname):
     f.close()
 
 
+def test_grid_prepass_matches_in_kernel_gathers(torch_cuda):
+    """The XCD-aware grid pre-pass and the in-kernel gathers compute the same
+    level values; the query results are bit-identical."""
+    torch = torch_cuda
+    cfgs = []
+    for rows in (0, 1 << 30):
+        c = G.hash_config(with_nrc=True)
+        c["dfg"]["grid_prepass_min_rows"] = rows
+        cfgs.append(G.GuidingField(c))
+    n = 3000
+    feat, u = dev(torch, df_features(n, 70, nrc=True)), dev(torch, uniforms(n, 71))
+    outs = []
+    for f in cfgs:
+        w = dev(torch, wi_init(n))
+        rgb = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        f.query(feat, u, w, nrc_rgb=rgb)
+        d_pdf = torch.zeros(n, dtype=torch.float32, device="cuda")
+        f.pdf(feat, w[:, :2].contiguous(), d_pdf)
+        torch.cuda.synchronize()
+        outs.append((host(w), host(rgb), host(d_pdf)))
+        f.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_pdf_entry(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=False)

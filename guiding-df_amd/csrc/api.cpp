@@ -15,32 +15,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/dfg.h"
-#include "internal.h"
+#include "ctx.h"
 #include "json.h"
 
 namespace dfg {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPX(expr)                                                                               \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            throw Error(e_ == hipErrorOutOfMemory ? DFG_ERR_OOM : DFG_ERR_HIP,                    \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
-    } while (0)
-
-static void check_launch() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) throw Error(DFG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-}
-
-static inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
-static inline uint32_t next_multiple(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 
 static std::string lower(std::string s) {
     for (auto& c : s) c = (char)std::tolower((unsigned char)c);
@@ -71,10 +50,6 @@ struct Pcg32 {
 // tcnn encodings/composite.h:136-215 and encoding.h:85-87; grid level sizes
 // follow grid.h:959-1025; the MLP layout follows fully_fused_mlp.cu:636-688.
 // ---------------------------------------------------------------------------
-struct OptCfg {
-    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f, l2 = 1e-8f;
-    float ema_decay = 0.0f;  // 0 = no EMA (inference uses the training weights)
-};
 
 static OptCfg parse_optimizer(const Json& j) {
     OptCfg o;
@@ -324,81 +299,19 @@ static NetSpec build_net(const Json& enc, const Json& net, int in_dims, int n_ou
 // ---------------------------------------------------------------------------
 // Device state
 // ---------------------------------------------------------------------------
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    void ensure(size_t count) {
-        if (count <= n) return;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        HIPX(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
-        n = count;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-struct NetState {
-    bool present = false;
-    NetSpec spec;
-    OptCfg opt;
-    uint32_t step = 0;
-    DevBuf<NetSpec> d_spec;
-    DevBuf<float> master, m, v, grad;
-    DevBuf<_Float16> w16, ema16, pack_train, pack_infer;
-    DevBuf<uint32_t> steps;
-    // training workspace
-    DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
-    DevBuf<float> dx, partial, gfeat;
-    void release_all() {
-        d_spec.release(); master.release(); m.release(); v.release(); grad.release();
-        w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
-        x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();
-        dx.release(); partial.release(); gfeat.release();
-    }
-    const _Float16* tab_infer() const { return spec.grid.present ? ema16.p + spec.grid.param_begin : nullptr; }
-    const _Float16* tab_train() const { return spec.grid.present ? w16.p + spec.grid.param_begin : nullptr; }
-};
-
 }  // namespace dfg
 
 using namespace dfg;
 
-struct dfg_ctx {
-    NetState net[3];
-    int n_theta = 16, n_phi = 32, linear = 1, D = 9;
-    uint64_t seed = 7272;
-    int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
-    std::string err;
-    // plain-path and loss workspace
-    DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
-    DevBuf<float> tmp;
-    ~dfg_ctx() {
-        for (auto& n : net) n.release_all();
-        p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
-    }
-};
-
 static std::string g_create_error = "no error";
 
-static int fail(dfg_ctx* c, int code, const std::string& msg) {
+int dfg::fail(dfg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     else g_create_error = msg;
     return code;
 }
 
-#define API_BEGIN try {
-#define API_END(ctx)                                                        \
-    }                                                                       \
-    catch (const Error& e) { return fail(ctx, e.code, e.what()); }          \
-    catch (const std::bad_alloc&) { return fail(ctx, DFG_ERR_OOM, "host out of memory"); } \
-    catch (const std::exception& e) { return fail(ctx, DFG_ERR_CONFIG, e.what()); }         \
-    return DFG_OK;
+
 
 static void init_net_params(dfg_ctx* c, int id) {
     NetState& s = c->net[id];
@@ -614,14 +527,16 @@ int dfg_load(dfg_ctx* ctx, const char* prefix) {
     API_END(ctx)
 }
 
+}  // extern "C"
+
 // --- query family ---------------------------------------------------------------
-static void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
+void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
     a.xs = with_df ? std::max(c->net[0].spec.k_steps, c->net[1].spec.k_steps) * 32 + 4 : 0;
     a.xs_nrc = with_nrc ? c->net[2].spec.k_steps * 32 + 8 : 0;
     a.ls = std::max(std::max(c->n_theta, c->n_phi), 16) + 1;
 }
 
-static QueryArgs base_args(dfg_ctx* c) {
+QueryArgs dfg::base_args(dfg_ctx* c) {
     QueryArgs a;
     std::memset(&a, 0, sizeof(a));
     a.th = c->net[0].d_spec.p;
@@ -640,25 +555,41 @@ static QueryArgs base_args(dfg_ctx* c) {
 }
 
 // XCD-aware grid pre-pass of both DF nets for n rows (when worth a launch).
-static void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
+// Compact copy of the grid positions (dims pos_dim..pos_dim+2) for the pre-pass.
+static const float* compact_pos(dfg_ctx* c, const float* feat, int stride, int pos_dim, int64_t n, hipStream_t st) {
+    c->pos.ensure((size_t)n * 3);
+    launch_extract_pos(feat, stride, pos_dim, n, c->pos.p, st);
+    return c->pos.p;
+}
+
+void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
     if (n < c->prepass_min_rows) return;
     const int64_t ld = round_up(n, 64);
+    const float* pos = nullptr;
+    int pos_dim = -1;
     for (int i = 0; i < 2; ++i) {
         NetState& s = c->net[i];
         if (!s.spec.grid.present) continue;
+        const int pd = grid_pos_dim(s.spec);
+        if (pd != pos_dim) {
+            pos = compact_pos(c, feat, stride, pd, n, st);
+            pos_dim = pd;
+        }
         const size_t cols = (size_t)s.spec.grid.n_levels * s.spec.grid.n_feat;
         s.gfeat.ensure(cols * ld);
-        launch_grid_encode(s.d_spec.p, s.spec, s.tab_infer(), feat, stride, n, ld, s.gfeat.p, st);
+        launch_grid_encode(s.d_spec.p, s.spec, s.tab_infer(), pos, n, ld, s.gfeat.p, st);
         (i == 0 ? a.gfeat_th : a.gfeat_ph) = s.gfeat.p;
     }
     a.gld = ld;
 }
 
-static void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride) {
+void dfg::check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride) {
     if (n < 0 || n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "n out of range");
     if (n > 0 && !feat) throw Error(DFG_ERR_INVALID_ARG, "null features");
     if (stride < c->D) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below n_input_dims");
 }
+
+extern "C" {
 
 int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride, const float* u,
               const uint8_t* guided, float* wi, float* nrc_rgb) {
@@ -814,7 +745,8 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     a.ls = s.spec.n_out_pad + 1;
     if (s.spec.grid.present) {
         s.gfeat.ensure((size_t)s.spec.grid.n_levels * s.spec.grid.n_feat * ld);
-        launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), feat, stride, n, ld, s.gfeat.p, st);
+        const float* pos = compact_pos(c, feat, stride, grid_pos_dim(s.spec), n, st);
+        launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), pos, n, ld, s.gfeat.p, st);
         a.gfeat = s.gfeat.p;
     }
     launch_train_fwd(a, s.spec, st);

@@ -1,0 +1,118 @@
+// Host-side state of a guide context, shared by the C-ABI translation units
+// (api.cpp: networks, parameters, query/train entry points; serve.cpp:
+// wavefront frames and the request server).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/dfg.h"
+#include "internal.h"
+
+namespace dfg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                               \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw Error(e_ == hipErrorOutOfMemory ? DFG_ERR_OOM : DFG_ERR_HIP,                    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+inline void check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(DFG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+inline uint32_t next_multiple(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+struct OptCfg {
+    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f, l2 = 1e-8f;
+    float ema_decay = 0.0f;  // 0 = no EMA (inference uses the training weights)
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void ensure(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        HIPX(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct NetState {
+    bool present = false;
+    NetSpec spec;
+    OptCfg opt;
+    uint32_t step = 0;
+    DevBuf<NetSpec> d_spec;
+    DevBuf<float> master, m, v, grad;
+    DevBuf<_Float16> w16, ema16, pack_train, pack_infer;
+    DevBuf<uint32_t> steps;
+    // training workspace
+    DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
+    DevBuf<float> dx, partial, gfeat;
+    void release_all() {
+        d_spec.release(); master.release(); m.release(); v.release(); grad.release();
+        w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
+        x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();
+        dx.release(); partial.release(); gfeat.release();
+    }
+    const _Float16* tab_infer() const { return spec.grid.present ? ema16.p + spec.grid.param_begin : nullptr; }
+    const _Float16* tab_train() const { return spec.grid.present ? w16.p + spec.grid.param_begin : nullptr; }
+};
+
+
+}  // namespace dfg
+
+struct dfg_ctx {
+    using NetState = dfg::NetState;
+    template <typename T> using DevBuf = dfg::DevBuf<T>;
+    NetState net[3];
+    int n_theta = 16, n_phi = 32, linear = 1, D = 9;
+    uint64_t seed = 7272;
+    int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
+    std::string err;
+    // plain-path and loss workspace
+    DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
+    DevBuf<float> tmp, pos;
+    ~dfg_ctx() {
+        for (auto& n : net) n.release_all();
+        p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
+        pos.release();
+    }
+};
+
+namespace dfg {
+int fail(dfg_ctx* c, int code, const std::string& msg);
+QueryArgs base_args(dfg_ctx* c);
+void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df);
+void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st);
+void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride);
+}  // namespace dfg
+
+#define API_BEGIN try {
+#define API_END(ctx)                                                                        \
+    }                                                                                       \
+    catch (const dfg::Error& e) { return dfg::fail(ctx, e.code, e.what()); }                \
+    catch (const std::bad_alloc&) { return dfg::fail(ctx, DFG_ERR_OOM, "host out of memory"); } \
+    catch (const std::exception& e) { return dfg::fail(ctx, DFG_ERR_CONFIG, e.what()); }   \
+    return DFG_OK;

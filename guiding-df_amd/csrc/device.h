@@ -212,8 +212,35 @@ __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __
     GridCorners c;
     grid_corners(g, level, px, py, pz, c);
     float v[8][F];
+    if constexpr (F == 2) {
+        // Fewer distinct cache lines per lane: the x / x+1 corners usually sit
+        // in one aligned 8-byte pair (hash: (x^h, x^h^1) for even x; dense:
+        // idx, idx+1 for even idx), so load the pair holding corner x and only
+        // load corner x+1 separately when it lies elsewhere.  On levels whose
+        // dense stride for z wrapped to 0 the z+1 corners are the z corners.
+        const uint32_t* lt32 = (const uint32_t*)lt;
+        const bool zdup = !(g.hashed[level] & 1u) && g.s2[level] == 0u;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) load_feats<F>(lt, c.idx[k], v[k]);
+        for (int p = 0; p < 4; ++p) {
+            const int k0 = 2 * p, k1 = 2 * p + 1;
+            if (zdup && p >= 2) {
+                v[k0][0] = v[k0 - 4][0]; v[k0][1] = v[k0 - 4][1];
+                v[k1][0] = v[k1 - 4][0]; v[k1][1] = v[k1 - 4][1];
+                continue;
+            }
+            const uint32_t i0 = c.idx[k0], i1 = c.idx[k1];
+            const uint2 pr = *(const uint2*)(lt32 + (i0 & ~1u));
+            const uint32_t e0 = (i0 & 1u) ? pr.y : pr.x;
+            uint32_t e1 = (i0 & 1u) ? pr.x : pr.y;
+            if (i1 != (i0 ^ 1u)) e1 = lt32[i1];
+            const h2 h0 = __builtin_bit_cast(h2, e0), h1 = __builtin_bit_cast(h2, e1);
+            v[k0][0] = (float)h0[0]; v[k0][1] = (float)h0[1];
+            v[k1][0] = (float)h1[0]; v[k1][1] = (float)h1[1];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) load_feats<F>(lt, c.idx[k], v[k]);
+    }
 #pragma unroll
     for (int f = 0; f < F; ++f) res[f] = 0.0f;
 #pragma unroll
@@ -732,28 +759,29 @@ __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float
 
 // ---------------------------------------------------------------------------
 // XCD-aware grid encoding.  Workgroups are dispatched round-robin over the
-// XCDs, so block b runs on XCD b % n_groups; giving that block only the levels
-// l with l % n_groups == b % n_groups keeps every XCD's share of the table
-// (2 of 16 levels, <= 4 MB) resident in its own L2 instead of streaming all
-// 16 levels of a 28.5 MB table from the Infinity Cache.  One row per thread;
-// output feature-major fp32 [L*F][ld].
+// XCDs, so block b runs on XCD b % n_groups.  Blocks are ordered level-major in
+// phases: in phase p, block b encodes level p * n_groups + b % n_groups for a
+// chunk of rows, so each XCD works on one level (<= 2 MB of fp16 table) at a
+// time and keeps it resident in its own L2, instead of streaming all 16 levels
+// of a 28.5 MB table from the Infinity Cache.  One row per thread; positions
+// from a compact [n][3] array; output feature-major fp32 [L*F][ld].
 // ---------------------------------------------------------------------------
 template <int F>
 __device__ __forceinline__ void grid_encode_block(const NetSpec& ns, const _Float16* __restrict__ table,
-                                                  const float* __restrict__ feat, int stride, int pos_dim, int64_t n,
-                                                  int64_t ld, float* __restrict__ out, int n_groups) {
+                                                  const float* __restrict__ pos, int64_t n, int64_t ld,
+                                                  float* __restrict__ out, int n_groups, int chunks) {
     const GridSpec& g = ns.grid;
-    const int grp = blockIdx.x % n_groups;
-    const int64_t row = (int64_t)(blockIdx.x / n_groups) * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    const float* fr = feat + row * stride + pos_dim;
-    const float px = fr[0], py = fr[1], pz = fr[2];
-    for (int l = grp; l < g.n_levels; l += n_groups) {
-        float r[F];
-        grid_level<F>(g, table, l, px, py, pz, r);
+    const int b = blockIdx.x;
+    const int phase = b / (n_groups * chunks);
+    const int rem = b - phase * n_groups * chunks;
+    const int level = phase * n_groups + rem % n_groups;
+    const int64_t row = (int64_t)(rem / n_groups) * blockDim.x + threadIdx.x;
+    if (row >= n || level >= g.n_levels) return;
+    const float* p = pos + row * 3;
+    float r[F];
+    grid_level<F>(g, table, level, p[0], p[1], p[2], r);
 #pragma unroll
-        for (int f = 0; f < F; ++f) out[(size_t)(l * F + f) * ld + row] = r[f];
-    }
+    for (int f = 0; f < F; ++f) out[(size_t)(level * F + f) * ld + row] = r[f];
 }
 
 }  // namespace dfg

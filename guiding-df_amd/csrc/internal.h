@@ -65,8 +65,53 @@ struct TrainFwdArgs {
 };
 
 void launch_query(const QueryArgs& a, int n_theta, int n_phi, hipStream_t s);
-void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* feat, int stride,
-                        int64_t n, int64_t ld, float* out, hipStream_t s);
+
+// Rows of one query segment (caller-owned device buffers).
+struct RowsIO {
+    const float* feat;
+    int stride;
+    const float* u;
+    const uint8_t* guided;
+    float* wi;
+    float* rgb;
+    int64_t n;
+    int64_t gbase;   // row of this segment in the pre-pass feature buffers
+};
+
+// Work list of the persistent kernel: segments of rows, each starting at a
+// whole tile (64 rows) of the global tile sequence.
+#define DFG_MAX_SEGS 256
+struct WorkSeg {
+    const float* feat;
+    int stride;
+    const float* u;
+    const uint8_t* guided;
+    float* wi;
+    float* rgb;
+    int64_t n;
+    int64_t tile_begin;
+    int64_t gbase;
+};
+struct WorkList {
+    int n_seg;
+    int64_t total_tiles;
+    int64_t total_rows;
+    WorkSeg seg[DFG_MAX_SEGS];
+};
+void launch_query_worklist(const QueryArgs& a, int n_theta, int n_phi, const WorkList* wl, unsigned int* counter,
+                           int blocks, int tiles_per_fetch, hipStream_t s);
+// Frame set-up on the device: segment b = rows of bounce b packed back to back
+// starting at seg[0]'s buffers; resets the tile counter.
+void launch_worklist_setup(WorkList* wl, const int64_t* sizes, int n_bounces, int64_t capacity, unsigned int* counter,
+                           hipStream_t s);
+// Grid pre-pass on compact positions [n][3] (launch_extract_pos makes them).
+// n_dev (optional, device): the live row count; rows at or beyond it are
+// skipped, so a captured graph sized for the capacity n serves smaller frames.
+void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,
+                        int64_t ld, float* out, hipStream_t s, const int64_t* n_dev = nullptr);
+void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, float* pos, hipStream_t s,
+                        const int64_t* n_dev = nullptr);
+int grid_pos_dim(const NetSpec& hs);
 // Host copies of the specs are passed beside the args where launch shapes depend on them.
 void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s);
 
@@ -153,6 +198,7 @@ void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Fl
 void launch_f32_to_f16(const float* src, _Float16* dst, int64_t n, hipStream_t s);
 void launch_f16_to_f32(const _Float16* src, float* dst, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+void launch_fill_u32(unsigned int* p, unsigned int v, int64_t n, hipStream_t s);
 void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s);
 
 // Plain FP32 per-batch path.

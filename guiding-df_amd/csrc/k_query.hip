@@ -43,16 +43,16 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
     return x;
 }
 
+// One 64-row tile of a query: rows [row0, row0 + 64) of `io` (lane = row).
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
-__global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
     float* xt = (float*)smem;
     float* lo = (float*)smem;
-    const int lane = threadIdx.x;
-    const int64_t row = (int64_t)blockIdx.x * 64 + lane;
-    const bool valid = row < a.n;
-    const int64_t r = valid ? row : a.n - 1;
-    const float* frow = a.feat + r * a.stride;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = row0 + lane;
+    const bool valid = row < io.n;
+    const int64_t r = valid ? row : io.n - 1;
+    const float* frow = io.feat + r * io.stride;
     float* const xrow = xt + lane * a.xs;
     auto wr = [xrow](int c, float v) { xrow[c] = v; };
     float* lrow = lo + lane * a.ls;
@@ -63,28 +63,28 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
     const int D = th.in_dims;
 
     // ---- theta net
-    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + r : nullptr, a.gld);
+    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld);
     mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_th && valid) {
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
     }
-    const bool guided = a.wi2 == nullptr && a.u != nullptr && (a.guided == nullptr || a.guided[r] != 0);
+    const bool guided = a.wi2 == nullptr && io.u != nullptr && (io.guided == nullptr || io.guided[r] != 0);
     float4 w = make_float4(0.f, 0.f, 1.f, 1.f);
     if (a.wi2) {
         w.x = a.wi2[2 * r];
         w.y = a.wi2[2 * r + 1];
-    } else if (a.wi) {
-        w = *(const float4*)(a.wi + 4 * r);
+    } else if (io.wi) {
+        w = *(const float4*)(io.wi + 4 * r);
     }
     float4 uu = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (guided) uu = *(const float4*)(a.u + 4 * r);
+    if (guided) uu = *(const float4*)(io.u + 4 * r);
     float p_th;
     const float u0 = head_row<NTH>(lrow, guided, uu.x, uu.z, a.linear, w.x, p_th);
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
 
     // ---- phi net (its own encoding, theta appended as the last raw input)
-    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + r : nullptr, a.gld);
+    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld);
     mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
     if (a.logits_ph && valid) {
 #pragma unroll
@@ -96,8 +96,8 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
         if (a.wi2) {
             // AutoRegressive::pdf, AR.h:41-43
             a.pdf_out[r] = fabsf(p_th) * fabsf(p_ph) * DFG_INV_4PI;
-        } else if (a.wi) {
-            *(float4*)(a.wi + 4 * r) = make_float4(u0, u1, w.z * p_th, w.w * p_ph);
+        } else if (io.wi) {
+            *(float4*)(io.wi + 4 * r) = make_float4(u0, u1, w.z * p_th, w.w * p_ph);
         }
     }
   }
@@ -109,13 +109,79 @@ __global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
         _Float16* const xhrow = xh + lane * a.xs_nrc;
         encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; });
         mlp_forward_tile<1, true>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
-        if (valid) {
-            a.rgb[3 * r + 0] = lrow[0];
-            a.rgb[3 * r + 1] = lrow[1];
-            a.rgb[3 * r + 2] = lrow[2];
+        if (valid && io.rgb) {
+            io.rgb[3 * r + 0] = lrow[0];
+            io.rgb[3 * r + 1] = lrow[1];
+            io.rgb[3 * r + 2] = lrow[2];
         }
     }
 }
+
+template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
+__global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const RowsIO io{a.feat, a.stride, a.u, a.guided, a.wi, a.rgb, a.n, 0};
+    query_tile<NTH, NPH, WITH_DF, WITH_NRC>(a, io, (int64_t)blockIdx.x * 64, smem);
+}
+
+// Persistent work-list kernel: every wavefront pulls 64-row tiles from a device
+// counter over a table of segments (wavefront bounces or served requests) until
+// the table is exhausted.  Tiles never straddle segments, and the per-row math
+// is the one of query_fused_kernel, so results do not depend on the batching.
+template <int NTH, int NPH, bool WITH_NRC>
+__global__ __launch_bounds__(64) void query_worklist_kernel(const QueryArgs a, const WorkList* __restrict__ wl,
+                                                            unsigned int* __restrict__ counter, int tiles_per_fetch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int nseg = wl->n_seg;
+    const int64_t total = wl->total_tiles;
+    for (;;) {
+        unsigned int t0 = 0;
+        if (lane == 0) t0 = atomicAdd(counter, (unsigned int)tiles_per_fetch);
+        t0 = __shfl(t0, 0);
+        if ((int64_t)t0 >= total) break;
+        for (int k = 0; k < tiles_per_fetch; ++k) {
+            const int64_t tile = (int64_t)t0 + k;
+            if (tile >= total) break;
+            int sg = 0;
+            while (sg + 1 < nseg && wl->seg[sg + 1].tile_begin <= tile) ++sg;
+            const WorkSeg& ws = wl->seg[sg];
+            const RowsIO io{ws.feat, ws.stride, ws.u, ws.guided, ws.wi, ws.rgb, ws.n, ws.gbase};
+            query_tile<NTH, NPH, true, WITH_NRC>(a, io, (tile - ws.tile_begin) * 64, smem);
+        }
+    }
+}
+
+// Builds the work list of a wavefront frame on the device from the bounce
+// sizes (so a captured graph replays with new sizes) and resets the counter.
+__global__ void worklist_setup_kernel(WorkList* __restrict__ wl, const int64_t* __restrict__ sizes, int n_bounces,
+                                      int64_t capacity, unsigned int* __restrict__ counter) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t tile = 0, row = 0;
+    for (int b = 0; b < n_bounces; ++b) {
+        int64_t nb = sizes[b];
+        if (nb < 0) nb = 0;
+        if (row + nb > capacity) nb = capacity - row;
+        WorkSeg& s = wl->seg[b];
+        s.feat = wl->seg[0].feat + row * wl->seg[0].stride;
+        s.stride = wl->seg[0].stride;
+        s.u = wl->seg[0].u + row * 4;
+        s.guided = wl->seg[0].guided ? wl->seg[0].guided + row : nullptr;
+        s.wi = wl->seg[0].wi + row * 4;
+        s.rgb = wl->seg[0].rgb ? wl->seg[0].rgb + row * 3 : nullptr;
+        s.n = nb;
+        s.tile_begin = tile;
+        s.gbase = row;
+        tile += (nb + 63) / 64;
+        row += nb;
+    }
+    wl->n_seg = n_bounces;
+    wl->total_tiles = tile;
+    wl->total_rows = row;
+    *counter = 0u;
+}
+
+
 
 template <int NTH, int NPH>
 static void launch_q(const QueryArgs& a, hipStream_t s) {
@@ -133,6 +199,37 @@ static void launch_q(const QueryArgs& a, hipStream_t s) {
         hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, false>), dim3(blocks), dim3(64), lds, s, a);
 }
 
+static size_t query_lds(const QueryArgs& a) {
+    size_t lds = (size_t)64 * a.xs * 4;
+    const size_t l2 = (size_t)64 * a.ls * 4;
+    const size_t l3 = (size_t)64 * a.xs_nrc * 2;
+    if (l2 > lds) lds = l2;
+    if (l3 > lds) lds = l3;
+    return lds;
+}
+
+template <int NTH, int NPH>
+static void launch_wl(const QueryArgs& a, const WorkList* wl, unsigned int* counter, int blocks, int tpf,
+                      hipStream_t s) {
+    const size_t lds = query_lds(a);
+    if (a.rgb) hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, true>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
+    else hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, false>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
+}
+
+void launch_query_worklist(const QueryArgs& a, int nt, int np, const WorkList* wl, unsigned int* counter,
+                           int blocks, int tiles_per_fetch, hipStream_t s) {
+    if (nt == 16 && np == 32) return launch_wl<16, 32>(a, wl, counter, blocks, tiles_per_fetch, s);
+    if (nt == 32 && np == 32) return launch_wl<32, 32>(a, wl, counter, blocks, tiles_per_fetch, s);
+    if (nt == 64 && np == 64) return launch_wl<64, 64>(a, wl, counter, blocks, tiles_per_fetch, s);
+    if (nt == 16 && np == 16) return launch_wl<16, 16>(a, wl, counter, blocks, tiles_per_fetch, s);
+    if (nt == 32 && np == 64) return launch_wl<32, 64>(a, wl, counter, blocks, tiles_per_fetch, s);
+}
+
+void launch_worklist_setup(WorkList* wl, const int64_t* sizes, int n_bounces, int64_t capacity, unsigned int* counter,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(worklist_setup_kernel, dim3(1), dim3(64), 0, s, wl, sizes, n_bounces, capacity, counter);
+}
+
 void launch_query(const QueryArgs& a, int nt, int np, hipStream_t s) {
     if (a.n <= 0) return;
     if (a.th == nullptr) return launch_q<16, 16>(a, s);  // radiance cache only
@@ -148,24 +245,45 @@ void launch_query(const QueryArgs& a, int nt, int np, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <int F>
 __global__ __launch_bounds__(256) void grid_encode_kernel(const NetSpec* __restrict__ dns, const _Float16* table,
-                                                          const float* feat, int stride, int pos_dim, int64_t n,
-                                                          int64_t ld, float* out, int n_groups) {
-    grid_encode_block<F>(*dns, table, feat, stride, pos_dim, n, ld, out, n_groups);
+                                                          const float* pos, int64_t n, int64_t ld, float* out,
+                                                          int n_groups, int chunks, const int64_t* n_dev) {
+    grid_encode_block<F>(*dns, table, pos, n_dev ? min(n, *n_dev) : n, ld, out, n_groups, chunks);
 }
 
-void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* feat, int stride,
-                        int64_t n, int64_t ld, float* out, hipStream_t s) {
-    if (n <= 0 || !hs.grid.present) return;
-    int pos_dim = 0;
+__global__ void extract_pos_kernel(const float* __restrict__ feat, int stride, int pos_dim, int64_t n,
+                                   float* __restrict__ pos, const int64_t* n_dev) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev && *n_dev < n) n = *n_dev;
+    if (i >= n * 3) return;
+    const int64_t r = i / 3;
+    pos[i] = feat[r * stride + pos_dim + (i - r * 3)];
+}
+
+int grid_pos_dim(const NetSpec& hs) {
     for (int i = 0; i < hs.n_parts; ++i)
-        if (hs.parts[i].type == ENC_GRID) pos_dim = hs.parts[i].dim_begin;
+        if (hs.parts[i].type == ENC_GRID) return hs.parts[i].dim_begin;
+    return 0;
+}
+
+void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, float* pos, hipStream_t s,
+                        const int64_t* n_dev) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(extract_pos_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, s, feat, stride,
+                       pos_dim, n, pos, n_dev);
+}
+
+void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,
+                        int64_t ld, float* out, hipStream_t s, const int64_t* n_dev) {
+    if (n <= 0 || !hs.grid.present) return;
     const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
-    const unsigned blocks = (unsigned)(((n + 255) / 256) * groups);
+    const int phases = (hs.grid.n_levels + groups - 1) / groups;
+    const int chunks = (int)((n + 255) / 256);
+    const unsigned blocks = (unsigned)((int64_t)chunks * groups * phases);
     switch (hs.grid.n_feat) {
-        case 1: hipLaunchKernelGGL((grid_encode_kernel<1>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
-        case 2: hipLaunchKernelGGL((grid_encode_kernel<2>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
-        case 4: hipLaunchKernelGGL((grid_encode_kernel<4>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
-        default: hipLaunchKernelGGL((grid_encode_kernel<8>), dim3(blocks), dim3(256), 0, s, dns, table, feat, stride, pos_dim, n, ld, out, groups); break;
+        case 1: hipLaunchKernelGGL((grid_encode_kernel<1>), dim3(blocks), dim3(256), 0, s, dns, table, pos, n, ld, out, groups, chunks, n_dev); break;
+        case 2: hipLaunchKernelGGL((grid_encode_kernel<2>), dim3(blocks), dim3(256), 0, s, dns, table, pos, n, ld, out, groups, chunks, n_dev); break;
+        case 4: hipLaunchKernelGGL((grid_encode_kernel<4>), dim3(blocks), dim3(256), 0, s, dns, table, pos, n, ld, out, groups, chunks, n_dev); break;
+        default: hipLaunchKernelGGL((grid_encode_kernel<8>), dim3(blocks), dim3(256), 0, s, dns, table, pos, n, ld, out, groups, chunks, n_dev); break;
     }
 }
 

@@ -524,6 +524,10 @@ __global__ void f16_to_f32_kernel(const _Float16* __restrict__ a, float* __restr
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
+__global__ void fill_u32_kernel(unsigned int* __restrict__ p, unsigned int v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 __global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -551,6 +555,9 @@ void launch_f16_to_f32(const _Float16* a, float* b, int64_t n, hipStream_t s) {
 }
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_u32(unsigned int* p, unsigned int v, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s) {
     hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, s, v, n, scale, out);

@@ -36,6 +36,10 @@ EXPORTS = [
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
+    "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
+    "dfg_wavefront_run", "dfg_wavefront_autotune",
+    "dfg_server_create", "dfg_server_destroy", "dfg_server_submit", "dfg_server_submit_train",
+    "dfg_server_progress", "dfg_server_get_stats", "dfg_server_drain",
 ]
 
 
@@ -54,6 +58,29 @@ class DfgInfo(ctypes.Structure):
         ("n_params", ctypes.c_size_t * 3), ("n_matrix_params", ctypes.c_size_t * 3),
         ("max_batch", ctypes.c_int64),
     ]
+
+
+class DfgWavefrontTuning(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int), ("tiles_per_fetch", ctypes.c_int),
+                ("grid_prepass", ctypes.c_int), ("ms", ctypes.c_float)]
+
+
+class DfgServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("max_batch_requests", ctypes.c_int),
+                ("n_streams", ctypes.c_int), ("train_every", ctypes.c_int)]
+
+
+class DfgRequest(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("n", ctypes.c_int64), ("feat", ctypes.c_void_p),
+                ("feat_stride", ctypes.c_int), ("u", ctypes.c_void_p), ("guided", ctypes.c_void_p),
+                ("wi", ctypes.c_void_p), ("nrc_rgb", ctypes.c_void_p)]
+
+
+class DfgServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("train_steps", ctypes.c_int64), ("rows", ctypes.c_int64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("busy_s", ctypes.c_double)]
 
 
 _lib = None
@@ -90,6 +117,19 @@ def load_library(path=LIB_PATH):
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
         "dfg_encode": ([P, P, I, I64, P, I, P], I),
         "dfg_grid_index": ([P, P, I, I64, P, I, P], I),
+        "dfg_wavefront_create": ([P, I, I64, ctypes.POINTER(P)], I),
+        "dfg_wavefront_destroy": ([P], I),
+        "dfg_wavefront_set_tuning": ([P, ctypes.POINTER(DfgWavefrontTuning)], I),
+        "dfg_wavefront_get_tuning": ([P, ctypes.POINTER(DfgWavefrontTuning)], I),
+        "dfg_wavefront_run": ([P, P, I, P, P, I, P, P, P], I),
+        "dfg_wavefront_autotune": ([P, P, I, P, P, I, P, P, P, ctypes.POINTER(DfgWavefrontTuning)], I),
+        "dfg_server_create": ([P, ctypes.POINTER(DfgServerConfig), ctypes.POINTER(P)], I),
+        "dfg_server_destroy": ([P], I),
+        "dfg_server_submit": ([P, ctypes.POINTER(DfgRequest)], I),
+        "dfg_server_submit_train": ([P, I64, P, I, P, P, P], I),
+        "dfg_server_progress": ([P, I], I),
+        "dfg_server_get_stats": ([P, ctypes.POINTER(DfgServerStats)], I),
+        "dfg_server_drain": ([P, P, P, P, I, ctypes.POINTER(I)], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -306,6 +346,102 @@ class GuidingField:
     def grid_index(self, net, inp, idx, stream=None):
         self._c(self.lib.dfg_grid_index(self.ctx, _stream(stream), net, inp.shape[0], _ptr(inp), inp.shape[1],
                                         _ptr(idx)))
+
+
+class Wavefront:
+    """All bounces of a wavefront frame in one persistent, graph-captured launch.
+    Bounce batches are packed back to back in feat/u/guided/wi; `sizes` is a
+    device int64 tensor with one entry per bounce."""
+
+    def __init__(self, field, max_bounces, capacity):
+        self.field = field
+        self.lib = field.lib
+        h = ctypes.c_void_p()
+        field._c(self.lib.dfg_wavefront_create(field.ctx, max_bounces, capacity, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dfg_wavefront_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, sizes, feat, u, wi, guided=None, stream=None):
+        self.field._c(self.lib.dfg_wavefront_run(self.h, _stream(stream), sizes.numel(), _ptr(sizes), _ptr(feat),
+                                                 feat.shape[1], _ptr(u), _ptr(guided), _ptr(wi)))
+
+    def autotune(self, sizes, feat, u, wi, guided=None, stream=None):
+        t = DfgWavefrontTuning()
+        self.field._c(self.lib.dfg_wavefront_autotune(self.h, _stream(stream), sizes.numel(), _ptr(sizes), _ptr(feat),
+                                                      feat.shape[1], _ptr(u), _ptr(guided), _ptr(wi),
+                                                      ctypes.byref(t)))
+        return {"blocks_per_cu": t.blocks_per_cu, "tiles_per_fetch": t.tiles_per_fetch,
+                "grid_prepass": t.grid_prepass, "ms": t.ms}
+
+    def set_tuning(self, blocks_per_cu, tiles_per_fetch, grid_prepass):
+        t = DfgWavefrontTuning(blocks_per_cu, tiles_per_fetch, grid_prepass, 0.0)
+        self.field._c(self.lib.dfg_wavefront_set_tuning(self.h, ctypes.byref(t)))
+
+
+class Server:
+    """Continuous-batching request server over one GuidingField."""
+
+    def __init__(self, field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8):
+        self.field = field
+        self.lib = field.lib
+        cfg = DfgServerConfig(budget_bytes, max_batch_requests, n_streams, train_every)
+        h = ctypes.c_void_p()
+        field._c(self.lib.dfg_server_create(field.ctx, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dfg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, rid, feat, u, wi, guided=None, nrc_rgb=None):
+        r = DfgRequest(rid, feat.shape[0], feat.data_ptr(), feat.shape[1], u.data_ptr(),
+                       guided.data_ptr() if guided is not None else None, wi.data_ptr(),
+                       nrc_rgb.data_ptr() if nrc_rgb is not None else None)
+        self._keep[rid] = (feat, u, wi, guided, nrc_rgb)
+        self.field._c(self.lib.dfg_server_submit(self.h, ctypes.byref(r)))
+
+    def submit_train(self, feat, target, nrc_feat=None, nrc_target=None):
+        self._keep[("train", len(self._keep))] = (feat, target, nrc_feat, nrc_target)
+        self.field._c(self.lib.dfg_server_submit_train(self.h, feat.shape[0], _ptr(feat), feat.shape[1], _ptr(target),
+                                                       _ptr(nrc_feat), _ptr(nrc_target)))
+
+    def progress(self, wait=False):
+        self.field._c(self.lib.dfg_server_progress(self.h, 1 if wait else 0))
+
+    def stats(self):
+        st = DfgServerStats()
+        self.field._c(self.lib.dfg_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in DfgServerStats._fields_}
+
+    def drain(self, max_records=1 << 16):
+        import numpy as np
+        ids = np.zeros(max_records, np.int64)
+        lat = np.zeros(max_records, np.float64)
+        ver = np.zeros(max_records, np.int64)
+        cnt = ctypes.c_int(0)
+        self.field._c(self.lib.dfg_server_drain(self.h, ids.ctypes.data_as(ctypes.c_void_p),
+                                                lat.ctypes.data_as(ctypes.c_void_p),
+                                                ver.ctypes.data_as(ctypes.c_void_p), max_records, ctypes.byref(cnt)))
+        k = cnt.value
+        return ids[:k], lat[:k], ver[:k]
 
 
 def df_head(logits_theta, logits_phi, u, wi, guided=None, interpolation=1, bins_theta=None,

@@ -167,6 +167,89 @@ int dfg_df_head(void* stream, int64_t n, int n_theta, int n_phi, int interpolati
                 const uint8_t* guided, float* wi, float* bins_theta, float* bins_phi,
                 int32_t* seg);
 
+/* ---- Batched wavefront (all bounces of a frame in one persistent launch) ----
+ * Replaces the per-bounce LearnedDistribution::inferenceStep calls of the
+ * bounce loop (integrator.cpp:588-633, one call + device sync per bounce) for
+ * a frame whose bounce batches are packed back to back: bounce b occupies rows
+ * [sum_{k<b} sizes[k], +sizes[b]) of feat/u/guided/wi.  The sizes are read on
+ * the device, so the captured hipGraph replays for any sizes whose sum stays
+ * within the plan's capacity. */
+typedef struct dfg_wavefront dfg_wavefront;
+
+typedef struct dfg_wavefront_tuning {
+    int blocks_per_cu;     /* resident wavefronts launched per CU (persistent grid) */
+    int tiles_per_fetch;   /* 64-row tiles taken per work-counter fetch            */
+    int grid_prepass;      /* 1: XCD-aware grid pre-pass nodes before the kernel   */
+    float ms;              /* autotune: measured frame time of this choice         */
+} dfg_wavefront_tuning;
+
+int dfg_wavefront_create(dfg_ctx* ctx, int max_bounces, int64_t capacity, dfg_wavefront** out);
+int dfg_wavefront_destroy(dfg_wavefront* w);
+int dfg_wavefront_set_tuning(dfg_wavefront* w, const dfg_wavefront_tuning* t);
+int dfg_wavefront_get_tuning(const dfg_wavefront* w, dfg_wavefront_tuning* t);
+/* sizes: DEVICE int64 [n_bounces].  Captures the frame graph on first use (and
+ * again when a buffer pointer, the bounce count or the tuning changes), then
+ * launches it on `stream`. */
+int dfg_wavefront_run(dfg_wavefront* w, void* stream, int n_bounces, const int64_t* sizes, const float* feat,
+                      int feat_stride, const float* u, const uint8_t* guided, float* wi);
+/* Times every candidate (persistent grid size x tiles per fetch x pre-pass) on
+ * the given frame and keeps the fastest; writes it to *best (may be NULL). */
+int dfg_wavefront_autotune(dfg_wavefront* w, void* stream, int n_bounces, const int64_t* sizes, const float* feat,
+                           int feat_stride, const float* u, const uint8_t* guided, float* wi,
+                           dfg_wavefront_tuning* best);
+
+/* ---- Served loop: continuous batching of guidance queries ----------------
+ * Requests (device-resident rows) are queued by dfg_server_submit; every
+ * dfg_server_progress admits queued requests while the rows in flight stay
+ * within the memory budget, packs all admitted ones into one persistent
+ * work-list launch on the next of `n_streams` streams (so a batch that
+ * arrives while another runs overlaps it on the device), interleaves a queued
+ * training step every `train_every` launches (ordered after the launches
+ * before it and before the ones after it), and retires finished launches,
+ * timing each request from submission to the completion event of its launch.
+ * Each row's result is identical whether its request ran alone or in a mix.
+ * No counterpart in the reference (its queries are one batch per bounce). */
+typedef struct dfg_server dfg_server;
+
+typedef struct dfg_server_config {
+    int64_t budget_bytes;   /* device bytes of request rows admitted at once        */
+    int max_batch_requests; /* requests packed into one launch (<= 256)             */
+    int n_streams;          /* launch streams (1..4)                                */
+    int train_every;        /* launches between queued training steps (0: never)   */
+} dfg_server_config;
+
+typedef struct dfg_request {
+    int64_t id;
+    int64_t n;
+    const float* feat;      /* [n][feat_stride] */
+    int feat_stride;
+    const float* u;         /* [n][4] */
+    const uint8_t* guided;  /* [n] or NULL */
+    float* wi;              /* [n][4] in/out */
+    float* nrc_rgb;         /* [n][3] or NULL (all requests of a server agree) */
+} dfg_request;
+
+typedef struct dfg_server_stats {
+    int64_t submitted, completed, launches, train_steps, rows;
+    double p50_ms, p99_ms, mean_ms, max_ms;  /* request latency, submit -> done */
+    double busy_s;                            /* first submit -> last completion */
+} dfg_server_stats;
+
+int dfg_server_create(dfg_ctx* ctx, const dfg_server_config* cfg, dfg_server** out);
+int dfg_server_destroy(dfg_server* s);
+int dfg_server_submit(dfg_server* s, const dfg_request* r);
+/* Training step queued for the next interleave point: DF records (and, if
+ * nrc_feat is non-NULL, an NRC step on nrc_feat/nrc_target). */
+int dfg_server_submit_train(dfg_server* s, int64_t B, const float* feat, int feat_stride, const float* target,
+                            const float* nrc_feat, const float* nrc_target);
+/* wait = 0: admit/launch/retire what is ready and return; 1: until idle. */
+int dfg_server_progress(dfg_server* s, int wait);
+int dfg_server_get_stats(dfg_server* s, dfg_server_stats* out);
+/* Per-request records of completed requests (drained): id, latency (ms) and
+ * the number of training steps applied before the request's launch. */
+int dfg_server_drain(dfg_server* s, int64_t* ids, double* latency_ms, int64_t* weights_version, int max,
+                     int* count);
+
 /* Debug exports.  dfg_logits: raw logits of both nets (inference weights) for
  * rows feat[n] with the phi net fed theta_in[n] (square u0 of theta).
  * dfg_encode: encoded (padded) input of net `net` [n][enc_width] for raw inputs

@@ -1,0 +1,172 @@
+"""GPU tests of the batched paths: wavefront frames (all bounces in one
+persistent, graph-captured launch) and the continuous-batching server.
+Every row must equal the same row queried alone through dfg_query."""
+import numpy as np
+import pytest
+
+import dfguide as G
+from gpu_util import df_features, uniforms, wi_init, train_targets, dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def _bounces(first, k, survival=0.7):
+    sizes = [first]
+    for _ in range(k - 1):
+        sizes.append(int(sizes[-1] * survival))
+    return sizes
+
+
+def _alone(torch, f, feat, u, wi0, sizes, guided=None):
+    out, row = [], 0
+    for n in sizes:
+        w = dev(torch, wi0[row:row + n])
+        g = dev(torch, guided[row:row + n]) if guided is not None else None
+        if n:
+            f.query(dev(torch, feat[row:row + n]), dev(torch, u[row:row + n]), w, guided=g)
+        out.append(w)
+        row += n
+    torch.cuda.synchronize()
+    return np.concatenate([host(w) for w in out], 0) if out else np.zeros((0, 4), np.float32)
+
+
+@pytest.mark.parametrize("tuning", [(4, 1, 0), (8, 2, 1), (2, 4, 1)])
+def test_wavefront_matches_per_bounce_queries(torch_cuda, tuning):
+    torch = torch_cuda
+    cfg = G.hash_config(n_bins_theta=64, n_bins_phi=64, with_nrc=False)
+    cfg["dfg"]["grid_prepass_min_rows"] = 1 << 30  # per-bounce reference: in-kernel gathers
+    f = G.GuidingField(cfg)
+    sizes = _bounces(20000, 6)
+    cap = sum(sizes)
+    feat, u = df_features(cap, 80), uniforms(cap, 81)
+    guided = (np.random.default_rng(82).random(cap) < 0.8).astype(np.uint8)
+    wi0 = wi_init(cap)
+    wi0[:, :2] = np.random.default_rng(83).random((cap, 2), dtype=np.float32) * np.float32(0.999)
+    ref = _alone(torch, f, feat, u, wi0, sizes, guided)
+    wf = G.Wavefront(f, 8, cap)
+    wf.set_tuning(*tuning)
+    d_sizes = torch.tensor(sizes, dtype=torch.int64, device="cuda")
+    d_feat, d_u, d_g = dev(torch, feat), dev(torch, u), dev(torch, guided)
+    d_wi = dev(torch, wi0)
+    wf.run(d_sizes, d_feat, d_u, d_wi, guided=d_g)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(d_wi), ref)
+    # replay the captured graph with new (smaller, uneven) sizes read on the device
+    sizes2 = [7000, 0, 3333, 64, 1, 5000]
+    n2 = sum(sizes2)
+    ref2 = _alone(torch, f, feat[:n2], u[:n2], wi0[:n2], sizes2, guided[:n2])
+    d_sizes.copy_(torch.tensor(sizes2, dtype=torch.int64, device="cuda"))
+    d_wi.copy_(dev(torch, wi0))
+    wf.run(d_sizes, d_feat, d_u, d_wi, guided=d_g)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(d_wi)[:n2], ref2)
+    assert np.array_equal(host(d_wi)[n2:], wi0[n2:])   # rows beyond the frame untouched
+    wf.close()
+    f.close()
+
+
+def test_wavefront_autotune(torch_cuda):
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False))
+    sizes = _bounces(30000, 6)
+    cap = sum(sizes)
+    feat, u = dev(torch, df_features(cap, 84)), dev(torch, uniforms(cap, 85))
+    d_sizes = torch.tensor(sizes, dtype=torch.int64, device="cuda")
+    wf = G.Wavefront(f, 6, cap)
+    wi = dev(torch, wi_init(cap))
+    best = wf.autotune(d_sizes, feat, u, wi)
+    assert best["ms"] > 0 and best["blocks_per_cu"] in (2, 4, 6, 8, 12) and best["tiles_per_fetch"] in (1, 2, 4)
+    wi2 = dev(torch, wi_init(cap))
+    wf.run(d_sizes, feat, u, wi2)
+    w3 = dev(torch, wi_init(cap))
+    f.query(feat, u, w3)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(wi2), host(w3))
+    wf.close()
+    f.close()
+
+
+def _requests(torch, n_req, seed, nrc=False, lo=1024, hi=40000):
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_req)).astype(np.int64)
+    reqs = []
+    for i, n in enumerate(sizes):
+        feat = df_features(int(n), seed * 1000 + i, nrc=nrc)
+        u = uniforms(int(n), seed * 1000 + 500 + i)
+        reqs.append((i, feat, u))
+    return reqs
+
+
+@pytest.mark.parametrize("n_streams,budget_rows", [(2, 1 << 22), (1, 50000), (3, 120000)])
+def test_server_alone_equals_mix(torch_cuda, n_streams, budget_rows):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True)
+    cfg["dfg"]["grid_prepass_min_rows"] = 1 << 30
+    f = G.GuidingField(cfg)
+    reqs = _requests(torch, 24, 90 + n_streams, nrc=True)
+    alone = {}
+    for i, feat, u in reqs:
+        w = dev(torch, wi_init(len(feat)))
+        rgb = torch.zeros((len(feat), 3), dtype=torch.float32, device="cuda")
+        f.query(dev(torch, feat), dev(torch, u), w, nrc_rgb=rgb)
+        alone[i] = (w, rgb)
+    torch.cuda.synchronize()
+    srv = G.Server(f, budget_bytes=budget_rows * 80, max_batch_requests=8, n_streams=n_streams, train_every=0)
+    mixed = {}
+    for i, feat, u in reqs:
+        w = dev(torch, wi_init(len(feat)))
+        rgb = torch.zeros((len(feat), 3), dtype=torch.float32, device="cuda")
+        mixed[i] = (w, rgb)
+        srv.submit(i, dev(torch, feat), dev(torch, u), w, nrc_rgb=rgb)
+        srv.progress()
+    srv.progress(wait=True)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["submitted"] == len(reqs)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    ids, lat, ver = srv.drain()
+    assert sorted(ids.tolist()) == list(range(len(reqs))) and np.all(lat > 0)
+    for i in alone:
+        assert np.array_equal(host(alone[i][0]), host(mixed[i][0])), "request %d differs alone vs in the mix" % i
+        assert np.array_equal(host(alone[i][1]), host(mixed[i][1]))
+    srv.close()
+    f.close()
+
+
+def test_server_training_interleave_is_ordered(torch_cuda):
+    """Requests launched before the first training step see the initial weights;
+    requests launched after the last one see the final weights: no torn reads."""
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False))
+    reqs = _requests(torch, 30, 7)
+    before = {}
+    for i, feat, u in reqs:
+        w = dev(torch, wi_init(len(feat)))
+        f.query(dev(torch, feat), dev(torch, u), w)
+        before[i] = w
+    torch.cuda.synchronize()
+    B = 65536
+    tf, tt = dev(torch, df_features(B, 8)), dev(torch, train_targets(B, 9))
+    srv = G.Server(f, budget_bytes=1 << 30, max_batch_requests=2, n_streams=2, train_every=3)
+    for _ in range(4):
+        srv.submit_train(tf, tt)
+    res = {}
+    for i, feat, u in reqs:
+        w = dev(torch, wi_init(len(feat)))
+        res[i] = (w, dev(torch, feat), dev(torch, u))
+        srv.submit(i, res[i][1], res[i][2], w)
+    srv.progress(wait=True)
+    ids, lat, ver = srv.drain()
+    st = srv.stats()
+    assert st["train_steps"] >= 1
+    final = int(ver.max())
+    assert final == st["train_steps"]
+    for i, v in zip(ids.tolist(), ver.tolist()):
+        if v == 0:
+            assert np.array_equal(host(res[i][0]), host(before[i]))
+        elif v == final:
+            w = dev(torch, wi_init(res[i][0].shape[0]))
+            f.query(res[i][1], res[i][2], w)
+            torch.cuda.synchronize()
+            assert np.array_equal(host(res[i][0]), host(w))
+    srv.close()
+    f.close()

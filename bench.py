@@ -36,6 +36,33 @@ def net_flops(enc_width, n_hidden, n_out_pad):
     return 2 * (enc_width * 64 + (n_hidden - 1) * 64 * 64 + 64 * n_out_pad)
 
 
+def pdf_error(field, cfg, feat, u, feat_h, u_h, k=4096):
+    """Relative error of p_theta * p_phi against the FP64 oracle on the first
+    k queries, fused path and plain FP32 path, with the field's current
+    inference weights."""
+    import numpy as np
+    import torch
+    from gpu_util import wi_init
+    try:
+        import oracle as O
+        th, ph, _ = O.config_layouts(cfg)
+        w0 = torch.from_numpy(wi_init(k)).cuda()
+        w1 = torch.from_numpy(wi_init(k)).cuda()
+        field.query(feat[:k].contiguous(), u[:k].contiguous(), w0)
+        field.query_plain(feat[:k].contiguous(), u[:k].contiguous(), w1)
+        torch.cuda.synchronize()
+        o = O.query_f64(th, ph, field.get_params(0, 2), field.get_params(1, 2), feat_h[:k], 9, u_h[:k], wi_init(k), 1)
+        ref = o[:, 2] * o[:, 3]
+        out = {"sample": k}
+        for name, w in (("fused", w0), ("plain_fp32", w1)):
+            g = w.cpu().numpy()
+            rel = np.abs(g[:, 2] * g[:, 3] - ref) / np.abs(ref)
+            out[name] = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99))}
+        return out
+    except Exception as e:  # accuracy is reported, never required for the timing
+        return {"error": str(e)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +72,8 @@ def main():
     ap.add_argument("--train-batch", type=int, default=262144)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the wavefront-frame and served-loop legs")
+    ap.add_argument("--requests", type=int, default=160, help="served-loop requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -96,6 +125,9 @@ def main():
     qps = n / (wall / args.steps)
     kernel_ms = float(ms.mean())  # one kernel launch per step: its average duration
 
+    # ---- accuracy on the seeded weights of the timed configuration
+    err = pdf_error(field, cfg, feat, u, feat_h, u_h)
+
     # ---- breakdown: DF nets only, radiance cache only (same batch)
     _, ms_df = timed(lambda: field.query(feat, u, wi, stream=stream), max(3, args.steps // 5), 2)
     _, ms_nrc = timed(lambda: field.nrc_infer(feat, rgb, stream=stream), max(3, args.steps // 5), 2)
@@ -120,22 +152,95 @@ def main():
     twall, tms = timed(train, tsteps, 2)
     train_sps = B / (twall / tsteps)
 
-    # ---- accuracy of this run's weights against the FP64 oracle (sample)
-    err = None
-    try:
-        import oracle as O
-        th, ph, _ = O.config_layouts(cfg)
-        k = 4096
-        field.query(feat[:k].contiguous(), u[:k].contiguous(), wi[:k].contiguous(), stream=stream)
-        w0 = torch.from_numpy(wi_init(k)).cuda()
-        field.query(feat[:k].contiguous(), u[:k].contiguous(), w0, stream=stream)
+    # ---- config 4: batched wavefront frame (6 bounces x0.7, 64/64 bins), one
+    #      persistent graph launch, autotuned; vs the plain path per bounce
+    wave = None
+    if not args.no_batched:
+        cfg4 = G.hash_config(n_bins_theta=64, n_bins_phi=64, with_nrc=False)
+        f4 = G.GuidingField(cfg4)
+        sizes = [2097152]
+        for _ in range(5):
+            sizes.append(int(sizes[-1] * 0.7))
+        cap = sum(sizes)
+        feat4 = torch.from_numpy(df_features(cap, 11)).cuda()
+        u4 = torch.from_numpy(uniforms(cap, 12)).cuda()
+        wi4 = torch.from_numpy(wi_init(cap)).cuda()
+        d_sizes = torch.tensor(sizes, dtype=torch.int64, device="cuda")
+        wf = G.Wavefront(f4, 6, cap)
+        tuning = wf.autotune(d_sizes, feat4, u4, wi4, stream=stream)
+        fwall, fms = timed(lambda: wf.run(d_sizes, feat4, u4, wi4, stream=stream), max(3, args.steps // 5), 2)
+
+        def plain_frame():
+            row = 0
+            for nb in sizes:
+                f4.query_plain(feat4[row:row + nb], u4[row:row + nb], wi4[row:row + nb], stream=stream)
+                row += nb
+
+        pwall4, _ = timed(plain_frame, 3, 1)
+        fsteps = max(3, args.steps // 5)
+        wave = {"queries_per_frame": cap, "bounce_sizes": sizes, "frame_ms": float(fwall * 1000 / fsteps),
+                "queries_per_s": cap / (fwall / fsteps), "plain_fp32_frame_ms": float(pwall4 * 1000 / 3),
+                "plain_fp32_queries_per_s": cap / (pwall4 / 3),
+                "speedup_vs_plain_fp32": (pwall4 / 3) / (fwall / fsteps),
+                "frame_latency_ms": {"p50": float(np.percentile(fms, 50)), "p99": float(np.percentile(fms, 99))},
+                "autotuned": tuning}
+        wf.close()
+        f4.close()
+        del feat4, u4, wi4
+
+    # ---- config 5: served loop, Poisson arrivals of query batches of
+    #      log-uniform size in [1024, 1048576], a 262,144-sample train step
+    #      after every 8 launches, memory-budget admission
+    served = None
+    if not args.no_batched:
+        rng = np.random.default_rng(4)
+        n_req = args.requests
+        rsizes = np.exp(rng.uniform(np.log(1024), np.log(1 << 20), n_req)).astype(np.int64)
+        t_train = float(np.median(tms))
+        mean_service = float(rsizes.mean()) / qps + t_train * 1e-3 / 8.0
+        rate = 0.8 / mean_service
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        feat9 = feat[:, :9].contiguous()
+        outs = [torch.empty((int(k), 4), dtype=torch.float32, device="cuda") for k in rsizes]
+        for o in outs:
+            o[:, :2] = 0.0
+            o[:, 2:] = 1.0
+        srv = G.Server(field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8)
+        for _ in range(n_req // 8 + 2):
+            srv.submit_train(tf, tt)
         torch.cuda.synchronize()
-        g = w0.cpu().numpy()
-        o = O.query_f64(th, ph, field.get_params(0, 2), field.get_params(1, 2), feat_h[:k], 9, u_h[:k], wi_init(k), 1)
-        rel = np.abs(g[:, 2] * g[:, 3] - o[:, 2] * o[:, 3]) / np.abs(o[:, 2] * o[:, 3])
-        err = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)), "sample": k}
-    except Exception as e:  # accuracy is reported, never required for the timing
-        err = {"error": str(e)}
+        t0 = time.perf_counter()
+        k = 0
+        while k < n_req:
+            now = time.perf_counter() - t0
+            while k < n_req and arrivals[k] <= now:
+                nk = int(rsizes[k])
+                srv.submit(k, feat9[:nk], u[:nk], outs[k])
+                k += 1
+            srv.progress()
+            if k < n_req:
+                time.sleep(max(0.0, min(2e-4, arrivals[k] - (time.perf_counter() - t0))))
+        srv.progress(wait=True)
+        st = srv.stats()
+        srv.close()
+        # plain comparator: the same requests one per call, host-synchronised
+        pk = min(n_req, 24)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for j in range(pk):
+            nk = int(rsizes[j])
+            field.query_plain(feat9[:nk], u[:nk], outs[j], stream=stream)
+        torch.cuda.synchronize()
+        plain_q = float(rsizes[:pk].sum()) / (time.perf_counter() - tp)
+        served_q = st["rows"] / st["busy_s"] if st["busy_s"] > 0 else 0.0
+        served = {"requests": n_req, "offered_load": 0.8, "arrival_rate_per_s": rate,
+                  "latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "mean": st["mean_ms"], "max": st["max_ms"]},
+                  "queries_per_s": served_q, "launches": st["launches"], "train_steps": st["train_steps"],
+                  "plain_fp32_queries_per_s": plain_q, "speedup_vs_plain_fp32": served_q / plain_q if plain_q else None}
+        del outs
+
+    # ---- accuracy after the training steps above (weights moved by training)
+    err_trained = pdf_error(field, cfg, feat, u, feat_h, u_h)
 
     # ---- CPU baseline: FP32 reference-order restatement on the host cores
     cpu = None
@@ -191,12 +296,15 @@ def main():
         "breakdown_ms": {"df_only": float(np.median(ms_df)), "nrc_only": float(np.median(ms_nrc)),
                          "plain_fp32": float(np.median(pms)), "train_step": float(np.median(tms))},
         "pdf_rel_err_vs_fp64": err,
+        "pdf_rel_err_vs_fp64_after_training": err_trained,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "query_fused_kernel", "kernel_ms": kernel_ms,
                      "bytes_per_query": bytes_q, "flops_per_query": fl_q,
                      "mfma_tflops": achieved_tf, "mfma_frac": achieved_tf / MFMA_F16_PEAK_TFLOPS},
         "cpu_baseline": cpu,
+        "wavefront_frame": wave,
+        "served_loop": served,
     }
     print(json.dumps(out), flush=True)
     field.close()

@@ -37,9 +37,15 @@ def net_flops(enc_width, n_hidden, n_out_pad):
 
 
 def pdf_error(field, cfg, feat, u, feat_h, u_h, k=4096):
-    """Relative error of p_theta * p_phi against the FP64 oracle on the first
-    k queries, fused path and plain FP32 path, with the field's current
-    inference weights."""
+    """Accuracy of the fused and the plain FP32 query against the FP64 oracle
+    on the first k queries, with the field's current inference weights.
+
+    pdf_at_sample: p_theta * p_phi reported by the GPU vs the FP64 density at
+    the very direction the GPU sampled (the value a renderer's MIS weight
+    uses).  sample_abs: |x - x64| of the sampled unit coordinates.
+    end_to_end: GPU pdf vs the FP64 pdf at the FP64 sample; for sharp learned
+    distributions this composes the sampler's rounding with the pdf's slope
+    and grows with the logit range, in FP32 as much as in FP16/FP32."""
     import numpy as np
     import torch
     from gpu_util import wi_init
@@ -51,13 +57,24 @@ def pdf_error(field, cfg, feat, u, feat_h, u_h, k=4096):
         field.query(feat[:k].contiguous(), u[:k].contiguous(), w0)
         field.query_plain(feat[:k].contiguous(), u[:k].contiguous(), w1)
         torch.cuda.synchronize()
-        o = O.query_f64(th, ph, field.get_params(0, 2), field.get_params(1, 2), feat_h[:k], 9, u_h[:k], wi_init(k), 1)
-        ref = o[:, 2] * o[:, 3]
+        pt, pp = field.get_params(0, 2), field.get_params(1, 2)
+        o = O.query_f64(th, ph, pt, pp, feat_h[:k], 9, u_h[:k], wi_init(k), 1)
+        ref = o[:, 2].astype(np.float64) * o[:, 3]
+        unguided = np.zeros(k, np.uint8)
+
+        def stat(r):
+            return {"median": float(np.median(r)), "p99": float(np.percentile(r, 99))}
+
         out = {"sample": k}
         for name, w in (("fused", w0), ("plain_fp32", w1)):
-            g = w.cpu().numpy()
-            rel = np.abs(g[:, 2] * g[:, 3] - ref) / np.abs(ref)
-            out[name] = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99))}
+            g = w.cpu().numpy().astype(np.float64)
+            at = wi_init(k)
+            at[:, :2] = g[:, :2]
+            o2 = O.query_f64(th, ph, pt, pp, feat_h[:k], 9, u_h[:k], at, 1, guided=unguided)
+            ref2 = o2[:, 2].astype(np.float64) * o2[:, 3]
+            out[name] = {"pdf_at_sample": stat(np.abs(g[:, 2] * g[:, 3] - ref2) / np.abs(ref2)),
+                         "sample_abs": stat(np.abs(g[:, :2] - o[:, :2]).max(1)),
+                         "end_to_end": stat(np.abs(g[:, 2] * g[:, 3] - ref) / np.abs(ref))}
         return out
     except Exception as e:  # accuracy is reported, never required for the timing
         return {"error": str(e)}
@@ -152,6 +169,16 @@ def main():
     twall, tms = timed(train, tsteps, 2)
     train_sps = B / (twall / tsteps)
 
+    # ---- plain FP32 per-batch training step on the same batch (comparator)
+    def plain_train():
+        field.train_step_plain(tf, tt, loss[0:1], stream=stream)
+        field.nrc_train_step_plain(tf, trgb, loss[1:2], stream=stream)
+
+    ptsteps = max(3, args.steps // 10)
+    ptwall, _ = timed(plain_train, ptsteps, 1)
+    plain_train_sps = B / (ptwall / ptsteps)
+    plain_train_kind = "plain FP32 per-batch step (DF + NRC)"
+
     # ---- config 4: batched wavefront frame (6 bounces x0.7, 64/64 bins), one
     #      persistent graph launch, autotuned; vs the plain path per bounce
     wave = None
@@ -188,55 +215,78 @@ def main():
         f4.close()
         del feat4, u4, wi4
 
-    # ---- config 5: served loop, Poisson arrivals of query batches of
-    #      log-uniform size in [1024, 1048576], a 262,144-sample train step
-    #      after every 8 launches, memory-budget admission
+    # ---- config 5: served loop, query batches of log-uniform size in
+    #      [1024, 1048576], a 262,144-sample train step (DF + NRC) after every
+    #      8 query batches, memory-budget admission.  First saturated (every
+    #      request queued at once: capacity), then Poisson arrivals at 0.8x that
+    #      capacity (latency).  The plain comparator serves the same requests one
+    #      batch per launch with the same training cadence.
     served = None
     if not args.no_batched:
         rng = np.random.default_rng(4)
         n_req = args.requests
         rsizes = np.exp(rng.uniform(np.log(1024), np.log(1 << 20), n_req)).astype(np.int64)
-        t_train = float(np.median(tms))
-        mean_service = float(rsizes.mean()) / qps + t_train * 1e-3 / 8.0
-        rate = 0.8 / mean_service
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
         feat9 = feat[:, :9].contiguous()
         outs = [torch.empty((int(k), 4), dtype=torch.float32, device="cuda") for k in rsizes]
-        for o in outs:
-            o[:, :2] = 0.0
-            o[:, 2:] = 1.0
-        srv = G.Server(field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8)
-        for _ in range(n_req // 8 + 2):
-            srv.submit_train(tf, tt)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        k = 0
-        while k < n_req:
-            now = time.perf_counter() - t0
-            while k < n_req and arrivals[k] <= now:
-                nk = int(rsizes[k])
-                srv.submit(k, feat9[:nk], u[:nk], outs[k])
-                k += 1
-            srv.progress()
-            if k < n_req:
-                time.sleep(max(0.0, min(2e-4, arrivals[k] - (time.perf_counter() - t0))))
-        srv.progress(wait=True)
-        st = srv.stats()
-        srv.close()
-        # plain comparator: the same requests one per call, host-synchronised
-        pk = min(n_req, 24)
+
+        def reset_outs():
+            for o in outs:
+                o[:, :2] = 0.0
+                o[:, 2:] = 1.0
+
+        def serve(arrivals):
+            reset_outs()
+            srv = G.Server(field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8)
+            for _ in range(n_req // 8 + 1):
+                srv.submit_train(tf, tt, nrc_feat=tf, nrc_target=trgb)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            k = 0
+            while k < n_req:
+                now = time.perf_counter() - t0
+                while k < n_req and (arrivals is None or arrivals[k] <= now):
+                    nk = int(rsizes[k])
+                    srv.submit(k, feat9[:nk], u[:nk], outs[k])
+                    k += 1
+                srv.progress()
+                if k < n_req:
+                    time.sleep(max(0.0, min(2e-4, arrivals[k] - (time.perf_counter() - t0))))
+            srv.progress(wait=True)
+            st = srv.stats()
+            srv.close()
+            return st
+
+        sat = serve(None)
+        sat_qps = sat["rows"] / sat["busy_s"]
+        rate = 0.8 * n_req / sat["busy_s"]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        poi = serve(arrivals)
+        # plain comparator, saturated: one launch per batch, same train cadence
+        reset_outs()
         torch.cuda.synchronize()
         tp = time.perf_counter()
-        for j in range(pk):
+        p_steps = 0
+        for j in range(n_req):
             nk = int(rsizes[j])
             field.query_plain(feat9[:nk], u[:nk], outs[j], stream=stream)
+            if (j + 1) % 8 == 0:
+                plain_train()
+                p_steps += 1
         torch.cuda.synchronize()
-        plain_q = float(rsizes[:pk].sum()) / (time.perf_counter() - tp)
-        served_q = st["rows"] / st["busy_s"] if st["busy_s"] > 0 else 0.0
-        served = {"requests": n_req, "offered_load": 0.8, "arrival_rate_per_s": rate,
-                  "latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "mean": st["mean_ms"], "max": st["max_ms"]},
-                  "queries_per_s": served_q, "launches": st["launches"], "train_steps": st["train_steps"],
-                  "plain_fp32_queries_per_s": plain_q, "speedup_vs_plain_fp32": served_q / plain_q if plain_q else None}
+        p_wall = time.perf_counter() - tp
+        plain_q = float(rsizes.sum()) / p_wall
+        served = {"requests": n_req, "mean_batch": float(rsizes.mean()),
+                  "saturated": {"queries_per_s": sat_qps, "train_samples_per_s": sat["train_steps"] * B / sat["busy_s"],
+                                "busy_s": sat["busy_s"], "launches": sat["launches"], "train_steps": sat["train_steps"],
+                                "latency_ms": {"p50": sat["p50_ms"], "p99": sat["p99_ms"]}},
+                  "poisson": {"offered_load": 0.8, "arrival_rate_per_s": rate,
+                              "queries_per_s": poi["rows"] / poi["busy_s"],
+                              "latency_ms": {"p50": poi["p50_ms"], "p99": poi["p99_ms"], "mean": poi["mean_ms"],
+                                             "max": poi["max_ms"]},
+                              "launches": poi["launches"], "train_steps": poi["train_steps"]},
+                  "plain_fp32": {"queries_per_s": plain_q, "train_samples_per_s": p_steps * B / p_wall,
+                                 "train": plain_train_kind},
+                  "speedup_vs_plain_fp32": sat_qps / plain_q}
         del outs
 
     # ---- accuracy after the training steps above (weights moved by training)
@@ -292,6 +342,8 @@ def main():
         "speedup_vs_plain_fp32": qps / plain_qps,
         "train_samples_per_s": train_sps,
         "train_ms_per_step": float(twall * 1000.0 / tsteps),
+        "plain_fp32_train_samples_per_s": plain_train_sps,
+        "train_speedup_vs_plain_fp32": train_sps / plain_train_sps,
         "batch_latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
         "breakdown_ms": {"df_only": float(np.median(ms_df)), "nrc_only": float(np.median(ms_nrc)),
                          "plain_fp32": float(np.median(pms)), "train_step": float(np.median(tms))},

@@ -752,6 +752,8 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     launch_train_fwd(a, s.spec, st);
 }
 
+static void net_update(NetState& s, float loss_scale, hipStream_t st);
+
 static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
                                 int n_chunks, int chunk, float loss_scale, hipStream_t st) {
     (void)c;
@@ -801,7 +803,13 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         gb.ld = ld;
         launch_grid_bwd(gb, ns, st);
     }
-    // Adam + EMA (EmaOptimizer::step, ema.h:102-136)
+    net_update(s, loss_scale, st);
+}
+
+// Adam + EMA (EmaOptimizer::step, ema.h:102-136) on the gradient in s.grad,
+// then the fragment packs of the new weights.
+static void net_update(NetState& s, float loss_scale, hipStream_t st) {
+    const NetSpec& ns = s.spec;
     s.step += 1;
     AdamArgs o;
     std::memset(&o, 0, sizeof(o));
@@ -899,6 +907,131 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE_NRC, parameters.h:59
     launch_nrc_loss(l, st);
     net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    check_launch();
+    API_END(ctx)
+}
+
+// --- plain FP32 training step (comparator) ---------------------------------------------
+static void ensure_plain_train_ws(NetState& s, int64_t ld, int n_chunks) {
+    const NetSpec& ns = s.spec;
+    s.px.ensure((size_t)ns.enc_width * ld);
+    s.pact.ensure((size_t)std::max(ns.n_hidden, 1) * 64 * ld);
+    s.pout.ensure((size_t)ns.n_out_pad * ld);
+    s.pgout.ensure((size_t)ns.n_out_pad * ld);
+    s.pg.ensure((size_t)2 * 64 * ld);
+    if (ns.grid.present) s.dx.ensure((size_t)ns.grid.n_levels * ns.grid.n_feat * ld);
+    s.partial.ensure((size_t)n_chunks * ns.n_matrix_params);
+}
+
+// encode, then one fp32 kernel per layer; hidden activations kept for the backward
+static void plain_net_forward(NetState& s, const float* feat, int stride, const float* extra, int64_t n, int64_t ld,
+                              hipStream_t st) {
+    const NetSpec& ns = s.spec;
+    launch_plain_encode(s.d_spec.p, s.tab_train(), feat, stride, extra, n, ld, s.px.p, st);
+    const float* X = s.px.p;
+    for (int l = 0; l <= ns.n_hidden; ++l) {
+        const int M = l == ns.n_hidden ? ns.n_out_pad : 64;
+        const int K = l == 0 ? ns.enc_width : 64;
+        float* Y = l == ns.n_hidden ? s.pout.p : s.pact.p + (size_t)l * 64 * ld;
+        launch_plain_layer(s.w16.p + ns.layer_off[l], M, K, K, X, Y, l < ns.n_hidden, n, ld, st);
+        X = Y;
+    }
+}
+
+// from the loss-scaled output gradient in s.pgout: per layer the weight
+// gradient and the ReLU-masked data gradient, the grid features' gradient,
+// the grid scatter, then Adam + EMA
+static void plain_net_backward_update(NetState& s, const float* feat, int stride, int64_t n, int64_t ld, int n_chunks,
+                                      int chunk, float loss_scale, hipStream_t st) {
+    const NetSpec& ns = s.spec;
+    const int H = ns.n_hidden;
+    const int gp = grid_part_index(ns);
+    const float* dY = s.pgout.p;
+    for (int l = H; l >= 0; --l) {
+        const int M = l == H ? ns.n_out_pad : 64;  // outputs of layer l
+        const int K = l == 0 ? ns.enc_width : 64;  // inputs of layer l
+        const float* in = l == 0 ? s.px.p : s.pact.p + (size_t)(l - 1) * 64 * ld;
+        launch_plain_wgrad(dY, M, in, K, s.partial.p + ns.layer_off[l], ns.n_matrix_params, n, ld, chunk, n_chunks,
+                           st);
+        if (l > 0) {
+            float* dX = s.pg.p + (size_t)(l & 1) * 64 * ld;
+            launch_plain_layer_bwd(s.w16.p + ns.layer_off[l], K, 0, 64, M, dY, in, dX, n, ld, st);
+            dY = dX;
+        } else if (gp >= 0) {
+            const int nf = ns.grid.n_levels * ns.grid.n_feat;
+            if (nf > 64) throw Error(DFG_ERR_CONFIG, "plain path: more than 64 grid features");
+            launch_plain_layer_bwd(s.w16.p + ns.layer_off[0], K, ns.parts[gp].out_begin, nf, M, dY, nullptr, s.dx.p,
+                                   n, ld, st);
+        }
+    }
+    launch_wgrad_reduce(s.partial.p, n_chunks, ns.n_matrix_params, s.grad.p, st);
+    if (gp >= 0) {
+        launch_fill_f32(s.grad.p + ns.grid.param_begin, 0.0f, (int64_t)ns.n_params - ns.grid.param_begin, st);
+        GridBwdArgs gb;
+        std::memset(&gb, 0, sizeof(gb));
+        gb.ns = s.d_spec.p;
+        gb.feat = feat;
+        gb.stride = stride;
+        gb.pos_dim = ns.parts[gp].dim_begin;
+        gb.dx = s.dx.p;
+        gb.grad = s.grad.p;
+        gb.n = n;
+        gb.ld = ld;
+        launch_grid_bwd(gb, ns, st);
+    }
+    net_update(s, loss_scale, st);
+}
+
+int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                         const float* target, float* loss_out) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_rows(ctx, B, feat, feat_stride);
+    if (B > 0 && !target) throw Error(DFG_ERR_INVALID_ARG, "null target");
+    if (B == 0) return DFG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    // the reference synchronises the device at the start of every training batch
+    HIPX(hipStreamSynchronize(st));
+    const int64_t ld = round_up(B, 64);
+    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    NetState& th = ctx->net[0];
+    NetState& ph = ctx->net[1];
+    ensure_plain_train_ws(th, ld, n_chunks);
+    ensure_plain_train_ws(ph, ld, n_chunks);
+    ctx->loss.ensure((size_t)ld);
+    ctx->p_theta.ensure((size_t)ld);
+    plain_net_forward(th, feat, feat_stride, nullptr, B, ld, st);
+    launch_plain_theta_target(target, B, ctx->n_theta, ctx->linear, ctx->p_theta.p, st);
+    plain_net_forward(ph, feat, feat_stride, ctx->p_theta.p, B, ld, st);
+    launch_plain_df_loss(target, th.pout.p, ph.pout.p, ctx->n_theta, ctx->n_phi, ctx->linear, th.pgout.p, ph.pgout.p,
+                         ctx->loss.p, 1024.0f, B, ld, st);
+    plain_net_backward_update(th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    plain_net_backward_update(ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    check_launch();
+    API_END(ctx)
+}
+
+int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                             const float* target_rgb, float* loss_out) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    if (B < 0 || B > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "B out of range");
+    if (feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below 15");
+    if (B > 0 && (!feat || !target_rgb)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    if (B == 0) return DFG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    HIPX(hipStreamSynchronize(st));
+    const int64_t ld = round_up(B, 64);
+    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    NetState& s = ctx->net[2];
+    ensure_plain_train_ws(s, ld, n_chunks);
+    ctx->loss.ensure((size_t)ld);
+    plain_net_forward(s, feat, feat_stride, nullptr, B, ld, st);
+    launch_plain_nrc_loss(s.pout.p, target_rgb, s.pgout.p, ctx->loss.p, 1024.0f, B, ld, st);
+    plain_net_backward_update(s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
     check_launch();
     API_END(ctx)

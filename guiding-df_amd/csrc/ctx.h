@@ -70,7 +70,10 @@ struct NetState {
     // training workspace
     DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
     DevBuf<float> dx, partial, gfeat;
+    // plain FP32 training workspace (feature-major fp32)
+    DevBuf<float> px, pact, pout, pgout, pg;
     void release_all() {
+        px.release(); pact.release(); pout.release(); pgout.release(); pg.release();
         d_spec.release(); master.release(); m.release(); v.release(); grad.release();
         w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
         x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();

@@ -544,6 +544,41 @@ __device__ __forceinline__ float theta_input(float theta, int n_theta, int linea
     return (float)fi / (float)n_theta;
 }
 
+// dL/dz_j for one net's logits (compute_gradients_specialized_linear_interpolation
+// / _nearest, AR.h:50-104): multiplier * d pdf(x) / d z_j through the softmax
+// Jacobian, where P(k) is the softmax probability of bin k (bins[k] / N).
+template <int N, typename Prob>
+__device__ __forceinline__ void df_logit_grad(float x, float multiplier, int linear, Prob P, float (&g)[N]) {
+#pragma clang fp contract(off)
+    if (linear) {
+        const float idx = x * (float)N - 0.5f;
+        int i0 = min((int)floorf(idx), N - 1);
+        int i1 = i0 + 1;
+        float w = idx - (float)i0;
+        if (i0 < 0) { i0 = N - 1; w = 1.0f; }
+        if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+        const float p0 = P(i0);
+        const float p1 = P(i1);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float pi = P(i);
+            const float j0 = i0 == i ? p0 * (1.0f - p0) : -p0 * pi;
+            const float j1 = i1 == i ? p1 * (1.0f - p1) : -p1 * pi;
+            const float j = (1.0f - w) * j0 + w * j1;
+            g[i] = multiplier * j;
+        }
+    } else {
+        int i0 = min((int)(x * (float)N), N - 1);
+        i0 = max(i0, 0);
+        const float p0 = P(i0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float pi = P(i);
+            g[i] = multiplier * (i0 == i ? p0 * (1.0f - p0) : -p0 * pi);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // MFMA helpers.  v_mfma_f32_16x16x32_f16: A 16x32 (lane l: row l%16, K slots
 // 8 per lane), B 32x16 (lane l: column l%16, the same K slots), D 16x16 (lane

@@ -210,6 +210,18 @@ void launch_plain_head(int phase, int n_bins, int n_theta, int linear, const flo
                        int64_t ld, int64_t n, const float* u, const uint8_t* guided, float* wi,
                        float* theta_in, hipStream_t s);
 void launch_plain_rgb(const float* out, int64_t ld, int64_t n, float* rgb, hipStream_t s);
+// Plain FP32 training step (the comparator of the training path).
+void launch_wgrad_reduce(const float* partial, int n_chunks, int64_t n_matrix, float* grad, hipStream_t s);
+void launch_plain_theta_target(const float* target, int64_t n, int n_theta, int linear, float* out, hipStream_t s);
+void launch_plain_layer_bwd(const _Float16* W, int K_row, int m_begin, int M, int K, const float* dY,
+                            const float* mask, float* dX, int64_t n, int64_t ld, hipStream_t s);
+void launch_plain_df_loss(const float* target, const float* lt, const float* lp, int n_theta, int n_phi,
+                          int linear, float* g_th, float* g_ph, float* loss, float loss_scale, int64_t n,
+                          int64_t ld, hipStream_t s);
+void launch_plain_nrc_loss(const float* pred, const float* target, float* g, float* loss, float loss_scale,
+                           int64_t n, int64_t ld, hipStream_t s);
+void launch_plain_wgrad(const float* dY, int M, const float* X, int K, float* partial, int64_t pstride,
+                        int64_t n, int64_t ld, int chunk, int n_chunks, hipStream_t s);
 
 // Stateless DF head.
 void launch_df_head(int64_t n, int n_theta, int n_phi, int linear, const float* lt,

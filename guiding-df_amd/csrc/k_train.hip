@@ -89,33 +89,7 @@ __device__ __forceinline__ void df_grad(float x, float pdf_other, float prefix, 
     const float inv_res = 1.0f / (float)N;
     const float multiplier = prefix * pdf_other * (float)N;
     float g[N];
-    if (linear) {
-        const float idx = x * (float)N - 0.5f;
-        int i0 = min((int)floorf(idx), N - 1);
-        int i1 = i0 + 1;
-        float w = idx - (float)i0;
-        if (i0 < 0) { i0 = N - 1; w = 1.0f; }
-        if (i1 > N - 1) { i1 = 0; w = 0.0f; }
-        const float p0 = (float)bins[i0] * inv_res;
-        const float p1 = (float)bins[i1] * inv_res;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float pi = (float)bins[i] * inv_res;
-            const float j0 = i0 == i ? p0 * (1.0f - p0) : -p0 * pi;
-            const float j1 = i1 == i ? p1 * (1.0f - p1) : -p1 * pi;
-            const float j = (1.0f - w) * j0 + w * j1;
-            g[i] = multiplier * j;
-        }
-    } else {
-        int i0 = min((int)(x * (float)N), N - 1);
-        i0 = max(i0, 0);
-        const float p0 = (float)bins[i0] * inv_res;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float pi = (float)bins[i] * inv_res;
-            g[i] = multiplier * (i0 == i ? p0 * (1.0f - p0) : -p0 * pi);
-        }
-    }
+    df_logit_grad<N>(x, multiplier, linear, [&](int k) { return (float)bins[k] * inv_res; }, g);
 #pragma unroll
     for (int i = 0; i < NPAD; ++i) {
         const _Float16 h = i < N ? sat_h(g[i < N ? i : 0]) : (_Float16)0.0f;
@@ -367,6 +341,11 @@ __global__ void wgrad_reduce_kernel(const float* __restrict__ partial, int n_chu
     float s = 0.0f;
     for (int c = 0; c < n_chunks; ++c) s += partial[(size_t)c * n_matrix + i];
     grad[i] = s;
+}
+
+void launch_wgrad_reduce(const float* partial, int n_chunks, int64_t n_matrix, float* grad, hipStream_t s) {
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n_matrix + 255) / 256)), dim3(256), 0, s, partial,
+                       n_chunks, n_matrix, grad);
 }
 
 void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s) {

@@ -392,10 +392,7 @@ bool server_launch(dfg_server* s) {
     if (l.reqs.empty()) return false;
     wl.total_tiles = tiles;
     wl.total_rows = l.rows;
-    if (s->cfg.train_every > 0 && s->since_train >= s->cfg.train_every && !s->train_queue.empty()) {
-        server_train(s);
-        s->since_train = 0;
-    }
+    const int64_t n_reqs = (int64_t)l.reqs.size();
     hipStream_t st = s->streams[s->next_stream];
     s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
     if (s->train_steps > 0) HIPX(hipStreamWaitEvent(st, s->last_train, 0));
@@ -421,7 +418,13 @@ bool server_launch(dfg_server* s) {
     s->slot_busy[slot] = true;
     s->inflight.push_back(std::move(l));
     ++s->launches;
-    ++s->since_train;
+    // a training step is due after every `train_every` query batches; it is
+    // enqueued right behind the launch that completes the count
+    s->since_train += n_reqs;
+    while (s->cfg.train_every > 0 && s->since_train >= s->cfg.train_every && !s->train_queue.empty()) {
+        server_train(s);
+        s->since_train -= s->cfg.train_every;
+    }
     return true;
 }
 
@@ -509,7 +512,15 @@ int dfg_server_progress(dfg_server* s, int wait) {
         while (server_launch(s)) {
         }
         if (!wait) break;
-        if (s->queue.empty() && s->inflight.empty()) break;
+        if (s->queue.empty() && s->inflight.empty()) {
+            if (s->train_steps > 0) {  // the busy span ends with the last training step too
+                HIPX(hipEventSynchronize(s->last_train));
+                float ms = 0.0f;
+                HIPX(hipEventElapsedTime(&ms, s->origin, s->last_train));
+                s->last_done = std::max(s->last_done, ms * 1e-3);
+            }
+            break;
+        }
         if (!s->inflight.empty()) server_retire(s, true);
     }
     API_END(ctx)

@@ -34,7 +34,8 @@ STATUS = {
 EXPORTS = [
     "dfg_create", "dfg_destroy", "dfg_last_error", "dfg_get_info", "dfg_get_params", "dfg_get_grad",
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
-    "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_prepare_features",
+    "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
+    "dfg_nrc_train_step_plain", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -110,8 +111,10 @@ def load_library(path=LIB_PATH):
         "dfg_query_plain": ([P, P, I64, P, I, P, P, P, P], I),
         "dfg_pdf": ([P, P, I64, P, I, P, P], I),
         "dfg_train_step": ([P, P, I64, P, I, P, P], I),
+"dfg_train_step_plain": ([P, P, I64, P, I, P, P], I),
         "dfg_nrc_infer": ([P, P, I64, P, I, P], I),
         "dfg_nrc_train_step": ([P, P, I64, P, I, P, P], I),
+"dfg_nrc_train_step_plain": ([P, P, I64, P, I, P, P], I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
         "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
@@ -326,6 +329,16 @@ class GuidingField:
     def train_step(self, feat, target, loss_out=None, stream=None):
         self._c(self.lib.dfg_train_step(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
                                         _ptr(target), _ptr(loss_out)))
+
+    def train_step_plain(self, feat, target, loss_out=None, stream=None):
+        """dfg_train_step on the plain FP32 per-batch path (the comparator)."""
+        self._c(self.lib.dfg_train_step_plain(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],
+                                              _ptr(target), _ptr(loss_out)))
+
+    def nrc_train_step_plain(self, feat, target_rgb, loss_out=None, stream=None):
+        """dfg_nrc_train_step on the plain FP32 per-batch path (the comparator)."""
+        self._c(self.lib.dfg_nrc_train_step_plain(self.ctx, _stream(stream), feat.shape[0], _ptr(feat),
+                                                  feat.shape[1], _ptr(target_rgb), _ptr(loss_out)))
 
     def nrc_infer(self, feat, rgb, stream=None):
         self._c(self.lib.dfg_nrc_infer(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],

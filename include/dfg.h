@@ -148,6 +148,17 @@ int dfg_nrc_infer(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int 
 int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
                        const float* target_rgb, float* loss_out);
 
+/* dfg_train_step and dfg_nrc_train_step computed by the plain FP32 per-batch
+ * path: one kernel per encoding and per layer forward and backward, fp32
+ * activations, bins and gradients, no MFMA, a host synchronisation per call.
+ * They update the same weights, Adam and EMA state.  No counterpart in the
+ * reference: they are the comparator the training throughput is quoted
+ * against. */
+int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                         const float* target, float* loss_out);
+int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
+                             const float* target_rgb, float* loss_out);
+
 /* Feature preparation (generate_inference_data_AR, train.cu:334-362):
  * shading points -> DF feature rows [n][9].
  *   pos [n][3] world space, aabb[6] = (min xyz, max xyz), wo [n][3] (any length),
@@ -204,7 +215,8 @@ int dfg_wavefront_autotune(dfg_wavefront* w, void* stream, int n_bounces, const 
  * within the memory budget, packs all admitted ones into one persistent
  * work-list launch on the next of `n_streams` streams (so a batch that
  * arrives while another runs overlaps it on the device), interleaves a queued
- * training step every `train_every` launches (ordered after the launches
+ * training step after every `train_every` query batches (requests; enqueued
+ * behind the launch that completes the count, ordered after the launches
  * before it and before the ones after it), and retires finished launches,
  * timing each request from submission to the completion event of its launch.
  * Each row's result is identical whether its request ran alone or in a mix.
@@ -215,7 +227,7 @@ typedef struct dfg_server_config {
     int64_t budget_bytes;   /* device bytes of request rows admitted at once        */
     int max_batch_requests; /* requests packed into one launch (<= 256)             */
     int n_streams;          /* launch streams (1..4)                                */
-    int train_every;        /* launches between queued training steps (0: never)   */
+    int train_every;        /* query batches between training steps (0: never)     */
 } dfg_server_config;
 
 typedef struct dfg_request {

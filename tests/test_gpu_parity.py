@@ -351,6 +351,69 @@ def test_nrc_train_step_vs_fp64(torch_cuda):
     f.close()
 
 
+@pytest.mark.parametrize("cfgname", ["plain", "hash", "reference"])
+def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
+    """The plain FP32 training comparator: fp32 bins, activations and
+    gradients, so loss and gradients sit at fp32 accuracy of the FP64 oracle
+    (the mixed path's fp16 activations are what its wider tolerance covers)."""
+    torch = torch_cuda
+    if cfgname == "plain":
+        cfg = G.plain_head_config()
+        D, feat = 8, plain_features(1000, 22)
+    elif cfgname == "reference":
+        cfg = G.reference_config(with_nrc=False)
+        D, feat = 9, df_features(1000, 22)
+    else:
+        cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+        D, feat = 9, df_features(1000, 22)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    B = feat.shape[0]
+    tgt = train_targets(B, 23)
+    before = {k: f.get_params(k, G.DFG_PARAMS_MASTER) for k in (0, 1)}
+    wtrain = {k: f.get_params(k, G.DFG_PARAMS_TRAIN) for k in (0, 1)}
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    f.train_step_plain(dev(torch, feat), dev(torch, tgt), d_loss)
+    torch.cuda.synchronize()
+    loss_g = float(host(d_loss)[0])
+    loss_o, gt_o, gp_o = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1)
+    assert abs(loss_g - loss_o) <= 1e-4 * abs(loss_o) + 1e-7, (loss_g, loss_o)
+    for k, go, L in ((0, gt_o, th), (1, gp_o, ph)):
+        gg = f.get_grad(k).astype(np.float64) / 1024.0
+        nm = L["n_matrix"]
+        assert _grad_rel(gg[:nm], go[:nm]) <= 1e-3, "net %d matrix grad rel err %.3g" % (k, _grad_rel(gg[:nm], go[:nm]))
+        if L["grid"]:
+            assert _grad_rel(gg[nm:], go[nm:]) <= 1e-3, "net %d grid grad rel err %.3g" % (k, _grad_rel(gg[nm:], go[nm:]))
+        opt = cfg["nn"]["optimizer"]
+        ad = opt["nested"]
+        master, _, _, _, _ = O.adam_ema_f64(nm, ad["learning_rate"], ad["beta1"], ad["beta2"], ad["epsilon"],
+                                            ad["l2_reg"], opt["decay"], 1, gg, before[k], np.zeros(gg.size),
+                                            np.zeros(gg.size), np.zeros(gg.size, np.uint32))
+        after = f.get_params(k, G.DFG_PARAMS_MASTER).astype(np.float64)
+        assert np.allclose(after, master, rtol=1e-5, atol=1e-7), "net %d Adam update differs" % k
+    f.close()
+
+
+def test_plain_nrc_train_step_vs_fp64(torch_cuda):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
+    f = G.GuidingField(cfg)
+    _, _, L = O.config_layouts(cfg)
+    B = 4096
+    feat = df_features(B, 32, nrc=True)
+    tgt = np.exp(np.random.default_rng(33).normal(0, 2, (B, 3))).astype(np.float32)
+    w = f.get_params(2, G.DFG_PARAMS_TRAIN)
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    f.nrc_train_step_plain(dev(torch, feat), dev(torch, tgt), d_loss)
+    torch.cuda.synchronize()
+    loss_o, g_o = O.nrc_train_f64(L, w, feat, tgt)
+    loss_g = float(host(d_loss)[0])
+    assert abs(loss_g - loss_o) <= 1e-4 * abs(loss_o), (loss_g, loss_o)
+    gg = f.get_grad(2).astype(np.float64) / 1024.0
+    assert _grad_rel(gg, g_o) <= 1e-3, "NRC grad rel err %.3g" % _grad_rel(gg, g_o)
+    f.close()
+
+
 # ----------------------------------------------------------------------------
 # Sizes, errors, formats
 # ----------------------------------------------------------------------------

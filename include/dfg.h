@@ -61,9 +61,12 @@ typedef enum dfg_status {
  *   target       [B][9]  : dir(2) square coords, L(3), wiPdf, bsdfVal(3) (GuidedOutput)
  */
 
-/* Create a context from a JSON configuration (the keys of the reference's nn JSON files,
- * see the "dfg" and "nn"/"nrc" sections documented in INTEGRATION.md), build
- * the networks and initialise parameters from the configured seed.
+/* Create a context from a JSON configuration, build the networks and
+ * initialise parameters from the configured seed.  Sections: "nn" holds the
+ * reference's nn JSON keys (optimizer, encoding, network; "phi_encoding" for
+ * the phi net's encoding), "nrc" the NRC net's keys, and "dfg" the guide's own
+ * keys: n_bins_theta, n_bins_phi, interpolation ("linear"|"nearest"), seed,
+ * n_input_dims, grid_prepass_min_rows.
  * Replaces LearnedDistribution::resetNetwork (learnedDistribution.cpp:64-165)
  * and NRC::resetNetwork (nrc.cpp:35-60). */
 int dfg_create(const char* json_config, dfg_ctx** out_ctx);
@@ -115,7 +118,8 @@ int dfg_load(dfg_ctx* ctx, const char* path_prefix);
  * (the reference keeps the sign of its initial +-1 there, train.h:741-750).
  * feat has `feat_stride` floats per row (>= n_input_dims).  When nrc_rgb is
  * non-NULL and an NRC net exists, also writes the NRC radiance [n][3]
- * (feat_stride must then be >= 15).  Fused MFMA path: one launch. */
+ * (feat_stride must then be >= 15).  Fused MFMA path: one query kernel, led
+ * for n >= grid_prepass_min_rows by a grid-encoding pre-pass (three kernels). */
 int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
               const float* u, const uint8_t* guided, float* wi, float* nrc_rgb);
 

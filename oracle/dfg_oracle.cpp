@@ -13,8 +13,18 @@
 //           as the CPU baseline (OpenMP over rows).
 //
 // Parity pins (see tests/test_oracle.py): closed forms, invariants, finite
-// differences, brute force on tiny grids; the reference holds no golden
-// vectors for this path.
+// differences, brute force on tiny grids, numpy matmul chains; the reference
+// holds no golden vectors for this path.
+//
+// Parity unpinned (reference behaviour that only running the reference could
+// fix, and the reference cannot run here): tiny-cuda-nn's initial weights for
+// a seed (host pcg32 through std::seed_seq, trainer.h:53-58, gpu_matrix.h:291-306,
+// and the device generate_random stream, random.h:67-97) — parameters are fed
+// to oracle and library alike instead; the rounding of its fp16-accumulating
+// WMMA/CUTLASS products (fully_fused_mlp.cu:68,333,436); the CUTLASS split-k
+// order; the order of its half2 gradient atomics (grid.h:433-453); the
+// outputs of CUDA's expf/logf/__sinf, which the restatement replaces by the
+// deterministic exp/log below and libm.
 #include <stdint.h>
 #include <string.h>
 

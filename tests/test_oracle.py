@@ -424,3 +424,55 @@ def test_ema_debias_identity():
         _, _, _, _, ema = O.adam_ema_f64(0, 0.0, 0.9, 0.99, 1e-8, 0.0, 0.95, step, np.zeros(n), w, np.zeros(n),
                                          np.zeros(n), np.zeros(n, np.uint32), w_lowp=w, ema=ema)
         assert np.allclose(ema, w, atol=1e-12)   # constant weights stay constant after debiasing
+
+
+# ----------------------------------------------------------------------------
+# MLP and whole-query composition against plain numpy
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("cfgname", ["hash", "reference"])
+def test_mlp_is_a_relu_chain_of_row_major_matrices(cfgname):
+    """The oracle's MLP equals numpy matmuls over the reference parameter
+    layout (first, hidden..., last layer; each row-major [out][in]; ReLU on
+    hidden layers, linear output) — a transposed or misplaced matrix fails."""
+    cfg = G.hash_config(log2_hashmap_size=12) if cfgname == "hash" else G.reference_config()
+    for L in O.config_layouts(cfg):
+        if L is None:
+            continue
+        rng = np.random.default_rng(41)
+        p = rng.normal(0, 0.2, L["n_params"])
+        x = rng.normal(0, 1, (64, L["enc_width"]))
+        h = x
+        off = L["layer_off"]
+        for l in range(L["n_hidden"] + 1):
+            rows = L["n_out_pad"] if l == L["n_hidden"] else 64
+            W = p[off[l]:off[l + 1]].reshape(rows, h.shape[1])
+            h = h @ W.T
+            if l < L["n_hidden"]:
+                h = np.maximum(h, 0.0)
+        assert np.allclose(O.mlp_f64(L, p, x), h, rtol=1e-12, atol=1e-12)
+
+
+def test_query_composes_encode_mlp_head():
+    """query_f64 = theta head of mlp(encode(feat)), then the phi head of
+    mlp(encode(feat, theta_input(x_theta))), pdfs multiplied into wi."""
+    from gpu_util import df_features
+    cfg = G.hash_config(log2_hashmap_size=12, with_nrc=False)
+    th, ph, _ = O.config_layouts(cfg)
+    rng = np.random.default_rng(43)
+    # query_f64 takes fp32 parameters (as the library holds them)
+    pt = rng.normal(0, 0.3, th["n_params"]).astype(np.float32).astype(np.float64)
+    pp = rng.normal(0, 0.3, ph["n_params"]).astype(np.float32).astype(np.float64)
+    n = 256
+    feat = df_features(n, 44)
+    u = rng.random((n, 4))
+    wi = np.concatenate([np.zeros((n, 2)), np.ones((n, 2))], 1).astype(np.float32)
+    q = O.query_f64(th, ph, pt, pp, feat, 9, u.astype(np.float32), wi, 1)
+    f64 = feat.astype(np.float64)
+    u32 = u.astype(np.float32).astype(np.float64)
+    lt = O.mlp_f64(th, pt, O.encode_f64(th, pt, f64))[:, :32]
+    x0, p0, _, _ = O.head(lt, 1, u32[:, 0], u32[:, 2])
+    t_in = np.array([O.lib().orc_theta_input_f64(float(v), 32, 1) for v in x0])
+    lp = O.mlp_f64(ph, pp, O.encode_f64(ph, pp, np.concatenate([f64, t_in[:, None]], 1)))[:, :32]
+    x1, p1, _, _ = O.head(lp, 1, u32[:, 1], u32[:, 3])
+    assert np.allclose(q[:, 0], x0, rtol=0, atol=1e-6) and np.allclose(q[:, 1], x1, rtol=0, atol=1e-6)
+    assert np.allclose(q[:, 2], p0, rtol=1e-6) and np.allclose(q[:, 3], p1, rtol=1e-6)

@@ -379,6 +379,9 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         c->seed = (uint64_t)d.number_or("seed", 7272);
         c->D = (int)d.number_or("n_input_dims", 9);
         c->prepass_min_rows = (int64_t)d.number_or("grid_prepass_min_rows", 32768);
+        const std::string gbw = lower(d.string_or("grid_backward", "bucketed"));
+        if (gbw != "bucketed" && gbw != "atomic") throw Error(DFG_ERR_CONFIG, "grid_backward must be bucketed or atomic");
+        c->grid_bwd_bucketed = gbw == "bucketed";
         auto okb = [](int b) { return b == 16 || b == 32 || b == 64; };
         if (!okb(c->n_theta) || !okb(c->n_phi)) throw Error(DFG_ERR_CONFIG, "bins must be 16, 32 or 64");
         const bool pair_ok = (c->n_theta == 16 && (c->n_phi == 16 || c->n_phi == 32)) ||
@@ -756,7 +759,6 @@ static void net_update(NetState& s, float loss_scale, hipStream_t st);
 
 static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
                                 int n_chunks, int chunk, float loss_scale, hipStream_t st) {
-    (void)c;
     const NetSpec& ns = s.spec;
     const int gp = grid_part_index(ns);
     TrainBwdArgs b;
@@ -788,7 +790,37 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
     w.chunk = chunk;
     w.n_chunks = n_chunks;
     launch_wgrad(w, ns, st);
-    if (gp >= 0) {
+    const GridBucketPlan plan = gp >= 0 ? grid_bucket_plan(ns, n) : GridBucketPlan{};
+    if (gp >= 0 && c->grid_bwd_bucketed && plan.ok) {
+        const int F = ns.grid.n_feat;
+        c->gb_offs.ensure((size_t)plan.n_slices + 1);
+        c->gb_cursor.ensure((size_t)plan.n_slices);
+        c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
+        if (F != 2) c->gb_idx.ensure((size_t)plan.records);
+        c->gb_val.ensure((size_t)plan.records * (F == 2 ? 4 : F));
+        c->gb_items.ensure((size_t)plan.max_items);
+        c->gb_n_items.ensure(1);
+        GridBucketArgs gb;
+        std::memset(&gb, 0, sizeof(gb));
+        gb.ns = s.d_spec.p;
+        gb.feat = feat;
+        gb.stride = stride;
+        gb.pos_dim = ns.parts[gp].dim_begin;
+        gb.dx = s.dx.p;
+        gb.grad = s.grad.p;
+        gb.n = n;
+        gb.ld = ld;
+        gb.slice_shift = plan.slice_shift;
+        gb.n_chunks = plan.n_chunks;
+        gb.offs = c->gb_offs.p;
+        gb.cursor = c->gb_cursor.p;
+        gb.cnt = c->gb_cnt.p;
+        gb.rec_idx = c->gb_idx.p;
+        gb.rec_val = c->gb_val.p;
+        gb.items = c->gb_items.p;
+        gb.n_items = c->gb_n_items.p;
+        launch_grid_bwd_bucketed(gb, ns, plan, st);
+    } else if (gp >= 0) {
         const size_t ngrid = (size_t)ns.n_params - ns.grid.param_begin;
         launch_fill_f32(s.grad.p + ns.grid.param_begin, 0.0f, (int64_t)ngrid, st);
         GridBwdArgs gb;
@@ -874,7 +906,10 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
     net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    if (loss_out) {
+        ctx->mean_ws.ensure(256);
+        launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+    }
     check_launch();
     API_END(ctx)
 }
@@ -907,7 +942,10 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE_NRC, parameters.h:59
     launch_nrc_loss(l, st);
     net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    if (loss_out) {
+        ctx->mean_ws.ensure(256);
+        launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+    }
     check_launch();
     API_END(ctx)
 }
@@ -1008,7 +1046,10 @@ int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* fea
                          ctx->loss.p, 1024.0f, B, ld, st);
     plain_net_backward_update(th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     plain_net_backward_update(ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    if (loss_out) {
+        ctx->mean_ws.ensure(256);
+        launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+    }
     check_launch();
     API_END(ctx)
 }
@@ -1032,7 +1073,10 @@ int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float*
     plain_net_forward(s, feat, feat_stride, nullptr, B, ld, st);
     launch_plain_nrc_loss(s.pout.p, target_rgb, s.pgout.p, ctx->loss.p, 1024.0f, B, ld, st);
     plain_net_backward_update(s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    if (loss_out) launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, st);
+    if (loss_out) {
+        ctx->mean_ws.ensure(256);
+        launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+    }
     check_launch();
     API_END(ctx)
 }

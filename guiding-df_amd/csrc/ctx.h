@@ -93,14 +93,23 @@ struct dfg_ctx {
     int n_theta = 16, n_phi = 32, linear = 1, D = 9;
     uint64_t seed = 7272;
     int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
+    bool grid_bwd_bucketed = true;     // bucketed (LDS) grid backward; false: fp32 atomic scatter
     std::string err;
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
-    DevBuf<float> tmp, pos;
+    DevBuf<float> tmp, pos, mean_ws;
+    // bucketed grid-backward workspace (shared by the nets, used in stream order)
+    DevBuf<int64_t> gb_offs, gb_cursor;
+    DevBuf<uint32_t> gb_idx, gb_cnt;
+    DevBuf<float> gb_val;
+    DevBuf<int4> gb_items;
+    DevBuf<int> gb_n_items;
     ~dfg_ctx() {
+        gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
+        gb_items.release(); gb_n_items.release();
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
-        pos.release();
+        pos.release(); mean_ws.release();
     }
 };
 

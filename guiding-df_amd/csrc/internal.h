@@ -179,6 +179,42 @@ struct GridBwdArgs {
 };
 void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s);
 
+// Bucketed grid backward: the table gradient is built slice by slice in LDS
+// (one slice = kGridSliceFloats floats of one level) from compact update
+// records, then written once; no global atomics, no zero fill.
+constexpr int kGridSliceFloats = 16384;       // 64 KB of LDS per slice
+constexpr int kGridMaxSlicesPerLevel = 2048;
+constexpr int kGridChunkRows = 4096;
+constexpr int kGridPartRecords = 65536;       // records reduced by one block
+struct GridBucketPlan {
+    bool ok;                 // the configuration fits the bucketed kernels
+    int slice_shift;         // log2 entries per slice
+    int n_slices;            // all levels
+    int n_chunks;            // row chunks of kGridChunkRows
+    int64_t records;         // n * levels * corners
+    int64_t max_items;       // bound on (slice, part) work items
+};
+GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n);
+struct GridBucketArgs {
+    const NetSpec* ns;
+    const float* feat;
+    int stride;
+    int pos_dim;
+    const float* dx;         // [L*F][ld]
+    float* grad;             // parameter gradient base
+    int64_t n, ld;
+    int slice_shift;
+    int n_chunks;
+    uint32_t* cnt;           // [n_slices][n_chunks] updates per slice and row chunk
+    int64_t* offs;           // [n_slices + 1]: slice totals, then exclusive offsets
+    int64_t* cursor;         // [n_slices] next free record of each slice
+    uint32_t* rec_idx;       // [records] entry within the slice (F != 2)
+    float* rec_val;          // F == 2: [records] 16-byte {entry, g0, g1, -}; else [records][F]
+    int4* items;             // [max_items] (slice, part, parts of the slice, -)
+    int* n_items;            // [1]
+};
+void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s);
+
 struct AdamArgs {
     int64_t n_params, n_matrix;
     float lr, beta1, beta2, eps, l2_reg, loss_scale;
@@ -199,7 +235,8 @@ void launch_f32_to_f16(const float* src, _Float16* dst, int64_t n, hipStream_t s
 void launch_f16_to_f32(const _Float16* src, float* dst, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
 void launch_fill_u32(unsigned int* p, unsigned int v, int64_t n, hipStream_t s);
-void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s);
+// Mean of v[0..n) times scale into out[0]; ws holds 256 floats of partial sums.
+void launch_mean(const float* v, int64_t n, float scale, float* out, float* ws, hipStream_t s);
 
 // Plain FP32 per-batch path.
 void launch_plain_encode(const NetSpec* dns, const _Float16* tab, const float* feat, int stride,

@@ -334,25 +334,35 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const WgradArgs a) {
     }
 }
 
-__global__ void wgrad_reduce_kernel(const float* __restrict__ partial, int n_chunks, int64_t n_matrix,
-                                    float* __restrict__ grad) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_matrix) return;
+// grad[i] = sum over chunks of partial[c][i], in a fixed order: block of 64
+// params x 16 chunk groups, group g summing chunks g, g+16, ..., then the 16
+// group sums added in order.
+__global__ __launch_bounds__(1024) void wgrad_reduce_kernel(const float* __restrict__ partial, int n_chunks,
+                                                            int64_t n_matrix, float* __restrict__ grad) {
+    __shared__ float red[16][64];
+    const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + j;
     float s = 0.0f;
-    for (int c = 0; c < n_chunks; ++c) s += partial[(size_t)c * n_matrix + i];
-    grad[i] = s;
+    if (i < n_matrix)
+        for (int c = g; c < n_chunks; c += 16) s += partial[(size_t)c * n_matrix + i];
+    red[g][j] = s;
+    __syncthreads();
+    if (g == 0 && i < n_matrix) {
+        float t = red[0][j];
+        for (int k = 1; k < 16; ++k) t += red[k][j];
+        grad[i] = t;
+    }
 }
 
 void launch_wgrad_reduce(const float* partial, int n_chunks, int64_t n_matrix, float* grad, hipStream_t s) {
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n_matrix + 255) / 256)), dim3(256), 0, s, partial,
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n_matrix + 63) / 64)), dim3(1024), 0, s, partial,
                        n_chunks, n_matrix, grad);
 }
 
 void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s) {
     if (a.n <= 0) return;
     hipLaunchKernelGGL(wgrad_kernel, dim3((unsigned)a.n_chunks, (unsigned)(hs.n_hidden + 1)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((hs.n_matrix_params + 255) / 256)), dim3(256), 0, s,
-                       a.partial, a.n_chunks, (int64_t)hs.n_matrix_params, a.grad);
+    launch_wgrad_reduce(a.partial, a.n_chunks, (int64_t)hs.n_matrix_params, a.grad, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,6 +399,258 @@ void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s) {
     if (a.n <= 0 || !hs.grid.present) return;
     const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
     hipLaunchKernelGGL(grid_bwd_kernel, dim3((unsigned)(((a.n + 255) / 256) * groups)), dim3(256), 0, s, a, groups);
+}
+
+// ---------------------------------------------------------------------------
+// Bucketed grid backward.  Scattered fp32 atomics run at a fixed device-wide
+// rate (~21 G/s on MI355X whatever their scope or XCD placement), so the
+// scatter of kernel_grid_backward (grid.h:394-520) is restructured as a
+// counting sort by table slice:
+//   count   : per (row chunk, level) block, updates per slice of the level
+//   scan    : exclusive offsets in (slice, chunk) order
+//   scatter : records {entry in slice, w * dL/dy[f]} to their slice's range
+//   reduce  : one block per slice sums its records in LDS and writes the
+//             slice of the gradient once (every table entry is written, so
+//             the gradient needs no zero fill)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int level_slices(const GridSpec& g, int l, int shift) {
+    const uint32_t e = g.offset[l + 1] - g.offset[l];
+    return (int)((e + (1u << shift) - 1) >> shift);
+}
+
+__device__ __forceinline__ int level_slice_base(const GridSpec& g, int l, int shift) {
+    int b = 0;
+    for (int k = 0; k < l; ++k) b += level_slices(g, k, shift);
+    return b;
+}
+
+// Records: F == 2 as one 16-byte {entry, g0, g1, -} store; other F as an
+// entry array plus an [F] value array.
+__device__ __forceinline__ void put_record(const GridBucketArgs& a, int F, int64_t pos, uint32_t e, float w,
+                                           const float (&d)[8]) {
+    if (F == 2) {
+        ((uint4*)a.rec_val)[pos] = make_uint4(e, __float_as_uint(w * d[0]), __float_as_uint(w * d[1]), 0u);
+    } else {
+        a.rec_idx[pos] = e;
+        for (int f = 0; f < F; ++f) a.rec_val[pos * F + f] = w * d[f];
+    }
+}
+
+// Per (row chunk, level) block: updates per slice of the level, kept per
+// (slice, chunk) and added to the slice totals.
+__global__ __launch_bounds__(256) void grid_bwd_count_kernel(const GridBucketArgs a) {
+    __shared__ uint32_t hist[kGridMaxSlicesPerLevel];
+    const GridSpec& gs = a.ns->grid;
+    const int level = blockIdx.y;
+    const int c = blockIdx.x;
+    const int nsl = level_slices(gs, level, a.slice_shift);
+    for (int i = threadIdx.x; i < nsl; i += blockDim.x) hist[i] = 0u;
+    __syncthreads();
+    const int64_t r0 = (int64_t)c * kGridChunkRows;
+    const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
+    const int nc = gs.interp == INTERP_NEAREST ? 1 : 8;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+        const float* fr = a.feat + row * a.stride + a.pos_dim;
+        GridCorners cr;
+        grid_corners(gs, level, fr[0], fr[1], fr[2], cr);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nc) atomicAdd(&hist[cr.idx[k] >> a.slice_shift], 1u);
+    }
+    __syncthreads();
+    const int sb = level_slice_base(gs, level, a.slice_shift);
+    for (int i = threadIdx.x; i < nsl; i += blockDim.x) {
+        const uint32_t h = hist[i];
+        a.cnt[(size_t)(sb + i) * a.n_chunks + c] = h;
+        if (h) atomicAdd((unsigned long long*)&a.offs[sb + i], (unsigned long long)h);
+    }
+}
+
+// Inclusive scan of one int64 per thread over a 1024-thread block.
+__device__ __forceinline__ int64_t block_scan_1024(int64_t* part, int64_t x) {
+    const int t = threadIdx.x;
+    part[t] = x;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int64_t y = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    return part[t];
+}
+
+// One block: slice totals -> exclusive offsets (total to offs[n_slices]) and
+// write cursors, then the reduce work list: each slice cut into parts of at
+// most kGridPartRecords records.
+__global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArgs a, int n_slices) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n_slices + 1023) / 1024;
+    int b = t * per;
+    if (b > n_slices) b = n_slices;
+    int e = b + per;
+    if (e > n_slices) e = n_slices;
+    int64_t sum = 0, np = 0;
+    for (int i = b; i < e; ++i) {
+        const int64_t c = a.offs[i];
+        sum += c;
+        np += c > 0 ? (c + kGridPartRecords - 1) / kGridPartRecords : 1;
+    }
+    const int64_t inc = block_scan_1024(part, sum);
+    __syncthreads();
+    const int64_t inc_p = block_scan_1024(part, np);
+    int64_t run = inc - sum, it = inc_p - np;
+    for (int i = b; i < e; ++i) {
+        const int64_t c = a.offs[i];
+        a.offs[i] = run;
+        a.cursor[i] = run;
+        run += c;
+        const int parts = c > 0 ? (int)((c + kGridPartRecords - 1) / kGridPartRecords) : 1;
+        for (int p = 0; p < parts; ++p) a.items[it++] = make_int4(i, p, parts, 0);
+    }
+    if (t == 1023) {
+        a.offs[n_slices] = inc;
+        *a.n_items = (int)inc_p;
+    }
+}
+
+// Per (row chunk, level) block: reserve this block's range of every slice,
+// then write one record per (row, corner).
+__global__ __launch_bounds__(256) void grid_bwd_scatter_kernel(const GridBucketArgs a) {
+    __shared__ uint32_t fill[kGridMaxSlicesPerLevel];
+    __shared__ int64_t base[kGridMaxSlicesPerLevel];
+    const GridSpec& gs = a.ns->grid;
+    const int F = gs.n_feat;
+    const int level = blockIdx.y;
+    const int c = blockIdx.x;
+    const int nsl = level_slices(gs, level, a.slice_shift);
+    const int sb = level_slice_base(gs, level, a.slice_shift);
+    for (int i = threadIdx.x; i < nsl; i += blockDim.x) {
+        fill[i] = 0u;
+        const uint32_t h = a.cnt[(size_t)(sb + i) * a.n_chunks + c];
+        base[i] = h ? (int64_t)atomicAdd((unsigned long long*)&a.cursor[sb + i], (unsigned long long)h) : 0;
+    }
+    __syncthreads();
+    const int64_t r0 = (int64_t)c * kGridChunkRows;
+    const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
+    const uint32_t emask = (1u << a.slice_shift) - 1u;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+        float d[8];
+        for (int f = 0; f < F; ++f) d[f] = a.dx[(size_t)(level * F + f) * a.ld + row];
+        const float* fr = a.feat + row * a.stride + a.pos_dim;
+        GridCorners cr;
+        grid_corners(gs, level, fr[0], fr[1], fr[2], cr);
+        if (gs.interp == INTERP_NEAREST) {
+            const uint32_t sl = cr.idx[0] >> a.slice_shift;
+            const int64_t pos = base[sl] + atomicAdd(&fill[sl], 1u);
+            put_record(a, F, pos, cr.idx[0] & emask, 1.0f, d);
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t sl = cr.idx[k] >> a.slice_shift;
+            const int64_t pos = base[sl] + atomicAdd(&fill[sl], 1u);
+            put_record(a, F, pos, cr.idx[k] & emask, cr.w[k], d);
+        }
+    }
+}
+
+// One block per (slice, part): sum the part's records into the slice in LDS
+// (8 records in flight per thread), then write the slice (single part) or add
+// it into the zero-filled gradient (several parts).
+__global__ __launch_bounds__(512) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
+    extern __shared__ float acc[];  // [entries of the slice][F]
+    if ((int)blockIdx.x >= *a.n_items) return;
+    const int4 item = a.items[blockIdx.x];
+    const GridSpec& gs = a.ns->grid;
+    const int F = gs.n_feat;
+    const int s = item.x;
+    int level = 0, sb = 0;
+    for (; level < gs.n_levels; ++level) {
+        const int k = level_slices(gs, level, a.slice_shift);
+        if (s < sb + k) break;
+        sb += k;
+    }
+    const uint32_t lvl_entries = gs.offset[level + 1] - gs.offset[level];
+    const uint32_t e0 = (uint32_t)(s - sb) << a.slice_shift;
+    const uint32_t cap = 1u << a.slice_shift;
+    const uint32_t ne = lvl_entries - e0 < cap ? lvl_entries - e0 : cap;
+    const int nf = (int)ne * F;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) acc[i] = 0.0f;
+    __syncthreads();
+    const int64_t se = a.offs[s + 1];
+    const int64_t pb = a.offs[s] + (int64_t)item.y * kGridPartRecords;
+    const int64_t pe = pb + kGridPartRecords < se ? pb + kGridPartRecords : se;
+    constexpr int U = 8;
+    if (F == 2) {
+        const uint4* r4 = (const uint4*)a.rec_val;
+        for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
+            uint4 r[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int64_t p = p0 + (int64_t)j * blockDim.x;
+                r[j] = p < pe ? r4[p] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (p0 + (int64_t)j * blockDim.x < pe) {
+                    atomicAdd(&acc[2 * r[j].x], __uint_as_float(r[j].y));
+                    atomicAdd(&acc[2 * r[j].x + 1], __uint_as_float(r[j].z));
+                }
+            }
+        }
+    } else {
+        for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+            const uint32_t e = a.rec_idx[p];
+            for (int f = 0; f < F; ++f) atomicAdd(&acc[e * F + f], a.rec_val[p * F + f]);
+        }
+    }
+    __syncthreads();
+    float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
+    if (item.z == 1) {
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) g[i] = acc[i];
+    } else {  // a slice cut into parts: add into the zero-filled gradient
+        for (int i = threadIdx.x; i < nf; i += blockDim.x)
+            if (acc[i] != 0.0f) unsafeAtomicAdd(g + i, acc[i]);
+    }
+}
+
+GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n) {
+    GridBucketPlan p{};
+    const GridSpec& g = hs.grid;
+    const int F = g.n_feat;
+    p.ok = hs.grid.present && (F == 1 || F == 2 || F == 4 || F == 8);
+    if (!p.ok) return p;
+    int shift = 0;
+    while ((1 << (shift + 1)) * F <= kGridSliceFloats) ++shift;
+    p.slice_shift = shift;
+    int maxl = 0;
+    for (int l = 0; l < g.n_levels; ++l) {
+        const uint32_t e = g.offset[l + 1] - g.offset[l];
+        const int k = (int)((e + (1u << shift) - 1) >> shift);
+        p.n_slices += k;
+        if (k > maxl) maxl = k;
+    }
+    p.n_chunks = (int)((n + kGridChunkRows - 1) / kGridChunkRows);
+    p.records = n * (int64_t)g.n_levels * (g.interp == INTERP_NEAREST ? 1 : 8);
+    p.max_items = p.n_slices + (p.records + kGridPartRecords - 1) / kGridPartRecords;
+    p.ok = maxl <= kGridMaxSlicesPerLevel && p.records <= ((int64_t)1 << 31) && p.n_chunks <= 65535 &&
+           p.max_items < ((int64_t)1 << 30);
+    return p;
+}
+
+void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s) {
+    if (a.n <= 0 || !p.ok) return;
+    const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
+    launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1), s);  // int64 slice totals
+    hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
+    hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(256), 0, s, a);
+    const int64_t ngrid = (int64_t)(hs.n_params - hs.grid.param_begin);
+    launch_fill_f32(a.grad + hs.grid.param_begin, 0.0f, ngrid, s);
+    const size_t lds = (size_t)kGridSliceFloats * sizeof(float);
+    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(512), lds, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -512,14 +774,32 @@ __global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
     if (i < n) p[i] = v;
 }
 // Deterministic mean: one block, fixed-order strided partial sums then a tree.
-__global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ v, int64_t n, float scale,
-                                                    float* __restrict__ out) {
-    __shared__ float red[1024];
+// Mean of v[0..n) in two deterministic stages: block b sums its contiguous
+// range into ws[b], then one block sums ws in order.
+constexpr int kMeanBlocks = 256;
+
+__global__ __launch_bounds__(256) void sum_part_kernel(const float* __restrict__ v, int64_t n,
+                                                       float* __restrict__ ws) {
+    __shared__ float red[256];
+    const int64_t per = (n + kMeanBlocks - 1) / kMeanBlocks;
+    const int64_t b = (int64_t)blockIdx.x * per;
+    const int64_t e = b + per < n ? b + per : n;
     float s = 0.0f;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+    for (int64_t i = b + threadIdx.x; i < e; i += 256) s += v[i];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ws[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void mean_kernel(const float* __restrict__ ws, float scale, float* __restrict__ out) {
+    __shared__ float red[256];
+    red[threadIdx.x] = ws[threadIdx.x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -538,8 +818,9 @@ void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
 void launch_fill_u32(unsigned int* p, unsigned int v, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
-void launch_mean(const float* v, int64_t n, float scale, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, s, v, n, scale, out);
+void launch_mean(const float* v, int64_t n, float scale, float* out, float* ws, hipStream_t s) {
+    hipLaunchKernelGGL(sum_part_kernel, dim3(kMeanBlocks), dim3(256), 0, s, v, n, ws);
+    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(256), 0, s, ws, scale, out);
 }
 
 }  // namespace dfg

@@ -405,6 +405,41 @@ def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
     f.close()
 
 
+@pytest.mark.parametrize("cfgname,B", [("hash", 70000), ("hash_nearest", 5000), ("reference", 9000)])
+def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
+    """The bucketed (count / scan / scatter / LDS-reduce) grid backward gives
+    the atomic scatter's table gradient up to fp32 summation order, covers
+    every table entry (untouched entries exactly 0) and leaves the MLP
+    gradients untouched."""
+    torch = torch_cuda
+    if cfgname == "reference":
+        cfg = G.reference_config(with_nrc=False)
+    else:
+        cfg = G.hash_config(with_nrc=False, log2_hashmap_size=15)
+        if cfgname == "hash_nearest":
+            for e in cfg["nn"]["encoding"]["nested"]:
+                if e["otype"] == "HashGrid":
+                    e["interpolation"] = "Nearest"
+    import copy
+    cfg_a = copy.deepcopy(cfg)
+    cfg_a["dfg"]["grid_backward"] = "atomic"
+    fb, fa = G.GuidingField(cfg), G.GuidingField(cfg_a)
+    feat, tgt = dev(torch, df_features(B, 60)), dev(torch, train_targets(B, 61))
+    fb.train_step(feat, tgt)
+    fa.train_step(feat, tgt)
+    torch.cuda.synchronize()
+    th, ph, _ = O.config_layouts(cfg)
+    for k, L in ((0, th), (1, ph)):
+        gb, ga = fb.get_grad(k).astype(np.float64), fa.get_grad(k).astype(np.float64)
+        nm = L["n_matrix"]
+        assert np.array_equal(gb[:nm], ga[:nm]), "net %d MLP gradients differ" % k
+        assert np.array_equal(gb[nm:] == 0, ga[nm:] == 0), "net %d touched entries differ" % k
+        r = _grad_rel(gb[nm:], ga[nm:])
+        assert r <= 1e-5, "net %d grid grad bucketed vs atomic rel err %.3g" % (k, r)
+    fb.close()
+    fa.close()
+
+
 def test_plain_nrc_train_step_vs_fp64(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)

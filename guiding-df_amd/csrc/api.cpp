@@ -191,6 +191,11 @@ static NetSpec build_net(const Json& enc, const Json& net, int in_dims, int n_ou
     }
     ns.enc_width = so_far;
     ns.k_steps = (so_far + 31) / 32;
+    // a grid part that opens the encoding and fills whole K-steps can feed the
+    // first MFMA layer straight from the pre-pass buffer
+    ns.grid_ksteps = 0;
+    if (ns.n_parts > 0 && ns.parts[0].type == ENC_GRID && ns.parts[0].n_pad == 0 && ns.parts[0].n_out % 32 == 0)
+        ns.grid_ksteps = ns.parts[0].n_out / 32;
     if (ns.k_steps * 32 > DFG_MAX_ENC) throw Error(DFG_ERR_CONFIG, "encoded width above 128");
     // MLP
     const int width_n = (int)net.number_or("n_neurons", 64);
@@ -534,7 +539,14 @@ int dfg_load(dfg_ctx* ctx, const char* prefix) {
 
 // --- query family ---------------------------------------------------------------
 void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
-    a.xs = with_df ? std::max(c->net[0].spec.k_steps, c->net[1].spec.k_steps) * 32 + 4 : 0;
+    // DF rows in LDS: the K-steps not read straight from a grid pre-pass buffer
+    int ks = 0;
+    for (int i = 0; i < 2; ++i) {
+        const NetSpec& ns = c->net[i].spec;
+        const bool direct = (i == 0 ? a.gfeat_th : a.gfeat_ph) != nullptr;
+        ks = std::max(ks, ns.k_steps - (direct ? ns.grid_ksteps : 0));
+    }
+    a.xs = with_df ? ks * 32 + 4 : 0;
     a.xs_nrc = with_nrc ? c->net[2].spec.k_steps * 32 + 8 : 0;
     a.ls = std::max(std::max(c->n_theta, c->n_phi), 16) + 1;
 }
@@ -605,8 +617,8 @@ int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat
     if (n == 0) return DFG_OK;
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.u = u; a.guided = guided; a.wi = wi; a.rgb = nrc_rgb;
-    fill_lds_strides(ctx, a, nrc_rgb != nullptr, true);
     grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
+    fill_lds_strides(ctx, a, nrc_rgb != nullptr, true);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)
@@ -620,8 +632,8 @@ int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_s
     if (n == 0) return DFG_OK;
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.wi2 = wi2; a.pdf_out = pdf;
-    fill_lds_strides(ctx, a, false, true);
     grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
+    fill_lds_strides(ctx, a, false, true);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)
@@ -637,8 +649,8 @@ int dfg_logits(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int fea
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.theta_in = theta_in;
     a.logits_th = logits_theta; a.logits_ph = logits_phi;
-    fill_lds_strides(ctx, a, false, true);
     grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
+    fill_lds_strides(ctx, a, false, true);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
     check_launch();
     API_END(ctx)

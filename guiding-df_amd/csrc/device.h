@@ -285,8 +285,11 @@ template <typename In, typename Out>
 // gfeat (optional): this row's grid features already computed by the
 // XCD-aware grid kernel, feature-major with column stride gstride; when given
 // the grid part copies them instead of gathering.
+// skip_grid: the grid part is neither gathered nor written (its features
+// reach the MLP straight from the pre-pass buffer).
 __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out,
-                                           const float* __restrict__ gfeat = nullptr, int64_t gstride = 0) {
+                                           const float* __restrict__ gfeat = nullptr, int64_t gstride = 0,
+                                           bool skip_grid = false) {
 #pragma clang fp contract(off)
     for (int pi = 0; pi < ns.n_parts; ++pi) {
         const EncPart& P = ns.parts[pi];
@@ -349,6 +352,7 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
             case ENC_GRID: {
                 const GridSpec& g = ns.grid;
                 const int F = g.n_feat;
+                if (skip_grid) break;
                 if (gfeat) {
                     for (int c = 0; c < g.n_levels * F; ++c) out(ob + c, gfeat[(size_t)c * gstride]);
                     break;
@@ -619,6 +623,17 @@ __device__ __forceinline__ void c_to_act(const f4 (&c)[4], h8 (&a)[2]) {
 }
 
 // Optional per-layer activation store (training): feature-major [64][ld] fp16.
+// The first `steps` K-steps of layer 0 read straight from the grid pre-pass
+// buffer (fp32, feature-major, column stride `stride`) instead of the LDS
+// tile; rows past `last` are clamped to it (their results are discarded).
+struct GridDirect {
+    const float* base;
+    int64_t stride;
+    int steps;
+    int64_t row0;
+    int64_t last;
+};
+
 struct ActStore {
     _Float16* x;        // encoded input, [k_steps*32][ld] (NULL: not stored)
     _Float16* act;      // hidden activations a_1..a_H, [H][64][ld]
@@ -656,7 +671,7 @@ __device__ __forceinline__ void relu_to_act2(const f4 (&c)[4], h8 (&hi)[2], h8 (
 template <int NT_OUT, bool SPLIT, int QN, typename TT>
 __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
                                                    const TT* xt, int xs, float* lo, int ls,
-                                                   const ActStore* st, int q0) {
+                                                   const ActStore* st, int q0, const GridDirect& gd) {
     // an fp16 input tile has no low part to add
     constexpr bool SPLIT_IN = SPLIT && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
@@ -675,13 +690,23 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
             h8 b[QN], bl[QN];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
-                const TT* src = xt + ((q0 + q) * 16 + i16) * xs + s * 32 + g * 8;
+                const TT* src = xt + ((q0 + q) * 16 + i16) * xs + (s - gd.steps) * 32 + g * 8;
                 if constexpr (sizeof(TT) == 2) {
                     b[q] = *(const h8*)src;
                 } else {
-                    const float4 v0 = *(const float4*)src;
-                    const float4 v1 = *(const float4*)(src + 4);
-                    const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                    float v[8];
+                    if (s < gd.steps) {
+                        int64_t rr = gd.row0 + (q0 + q) * 16 + i16;
+                        if (rr > gd.last) rr = gd.last;
+                        const float* gsrc = gd.base + (size_t)(s * 32 + g * 8) * gd.stride + rr;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[j] = gsrc[(size_t)j * gd.stride];
+                    } else {
+                        const float4 v0 = *(const float4*)src;
+                        const float4 v1 = *(const float4*)(src + 4);
+                        v[0] = v0.x; v[1] = v0.y; v[2] = v0.z; v[3] = v0.w;
+                        v[4] = v1.x; v[5] = v1.y; v[6] = v1.z; v[7] = v1.w;
+                    }
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         if (SPLIT_IN) {
@@ -788,8 +813,8 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
 template <int NT_OUT, bool SPLIT, int QN = (SPLIT ? 2 : 4), typename TT>
 __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
                                                  const TT* xt, int xs, float* lo, int ls,
-                                                 const ActStore* st) {
-    for (int q0 = 0; q0 < 4; q0 += QN) mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0);
+                                                 const ActStore* st, const GridDirect& gd = GridDirect{nullptr, 0, 0, 0, 0}) {
+    for (int q0 = 0; q0 < 4; q0 += QN) mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0, gd);
 }
 
 // ---------------------------------------------------------------------------

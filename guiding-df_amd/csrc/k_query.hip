@@ -62,9 +62,17 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     const NetSpec& ph = *a.ph;
     const int D = th.in_dims;
 
-    // ---- theta net
-    encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld);
-    mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr);
+    // ---- theta net (with the pre-pass, its leading grid K-steps skip the LDS tile)
+    const int gk_th = a.gfeat_th ? th.grid_ksteps : 0;
+    if (gk_th) {
+        const int o = 32 * gk_th;
+        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
+                   nullptr, 0, true);
+    } else {
+        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld);
+    }
+    mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
+                                     GridDirect{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1});
     if (a.logits_th && valid) {
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
@@ -84,8 +92,16 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
 
     // ---- phi net (its own encoding, theta appended as the last raw input)
-    encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld);
-    mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr);
+    const int gk_ph = a.gfeat_ph ? ph.grid_ksteps : 0;
+    if (gk_ph) {
+        const int o = 32 * gk_ph;
+        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
+                   nullptr, 0, true);
+    } else {
+        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld);
+    }
+    mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
+                                     GridDirect{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1});
     if (a.logits_ph && valid) {
 #pragma unroll
         for (int i = 0; i < NPH; ++i) a.logits_ph[r * NPH + i] = lrow[i];

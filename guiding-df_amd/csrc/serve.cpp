@@ -72,7 +72,6 @@ static void wavefront_enqueue(dfg_wavefront* w, hipStream_t st, int n_bounces, c
     a.feat = w->c_feat;
     a.stride = w->c_stride;
     a.n = w->capacity;
-    fill_lds_strides(c, a, false, true);
     const int64_t* n_dev = &w->wl.p->total_rows;
     if (w->tune.grid_prepass) {
         int pd = -1;
@@ -90,6 +89,7 @@ static void wavefront_enqueue(dfg_wavefront* w, hipStream_t st, int n_bounces, c
         }
         a.gld = ld;
     }
+    fill_lds_strides(c, a, false, true);
     const int blocks = std::max(1, cu_count() * std::max(1, w->tune.blocks_per_cu));
     launch_query_worklist(a, c->n_theta, c->n_phi, w->wl.p, w->counter.p, blocks, std::max(1, w->tune.tiles_per_fetch),
                           st);

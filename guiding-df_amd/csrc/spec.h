@@ -65,6 +65,7 @@ struct NetSpec {
     GridSpec grid;
     int enc_width;     // encoded width padded to 16 (the reference MLP input width)
     int k_steps;       // ceil(enc_width / 32): K-steps of the first MFMA layer
+    int grid_ksteps;   // leading K-steps made of grid features alone (0: none)
     int n_hidden;      // hidden layers (n_hidden_layers)
     int n_out;         // real outputs (bins, or 3 for RGB)
     int n_out_pad;     // outputs padded to 16

@@ -340,6 +340,7 @@ static void sync_copies(dfg_ctx* c, int id, hipStream_t st) {
     HIPX(hipMemcpyAsync(s.ema16.p, s.w16.p, s.spec.n_params * sizeof(_Float16), hipMemcpyDeviceToDevice, st));
     launch_pack(s.d_spec.p, s.spec, s.w16.p, s.pack_train.p, 3, st);
     launch_pack(s.d_spec.p, s.spec, s.ema16.p, s.pack_infer.p, 1, st);
+    if (id < 2) refresh_pair_table(c, st);
     check_launch();
 }
 
@@ -402,6 +403,21 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         const NetSpec ph = build_net(pick("phi_encoding", "encoding"), pick("phi_network", "network"), c->D + 1, c->n_phi);
         alloc_net(c.get(), DFG_NET_THETA, th, parse_optimizer(pick("theta_optimizer", "optimizer")));
         alloc_net(c.get(), DFG_NET_PHI, ph, parse_optimizer(pick("phi_optimizer", "optimizer")));
+        {
+            const GridSpec& a = c->net[0].spec.grid;
+            const GridSpec& b = c->net[1].spec.grid;
+            bool same = a.present && b.present && a.n_feat == 2 && b.n_feat == 2 && a.n_levels == b.n_levels &&
+                        a.interp == b.interp && a.type == b.type &&
+                        grid_pos_dim(c->net[0].spec) == grid_pos_dim(c->net[1].spec);
+            for (int l = 0; same && l < a.n_levels; ++l)
+                same = a.offset[l + 1] == b.offset[l + 1] && a.res[l] == b.res[l] && a.scale[l] == b.scale[l] &&
+                       a.s1[l] == b.s1[l] && a.s2[l] == b.s2[l] && a.hashed[l] == b.hashed[l] &&
+                       a.mod_mask[l] == b.mod_mask[l] && a.mod_mul[l] == b.mod_mul[l] &&
+                       a.mod_shift[l] == b.mod_shift[l];
+            c->pair_ok = same;
+            refresh_pair_table(c.get(), nullptr);
+            HIPX(hipDeviceSynchronize());
+        }
         if (root.contains("nrc")) {
             const Json& nr = root.at("nrc");
             const NetSpec n = build_net(nr.at("encoding"), nr.at("network"), 15, 3);
@@ -544,11 +560,17 @@ void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool w
     for (int i = 0; i < 2; ++i) {
         const NetSpec& ns = c->net[i].spec;
         const bool direct = (i == 0 ? a.gfeat_th : a.gfeat_ph) != nullptr;
-        ks = std::max(ks, ns.k_steps - (direct ? ns.grid_ksteps : 0));
+        ks = std::max(ks, ns.enc_width - (direct ? 32 * ns.grid_ksteps : 0));
     }
-    a.xs = with_df ? ks * 32 + 4 : 0;
-    a.xs_nrc = with_nrc ? c->net[2].spec.k_steps * 32 + 8 : 0;
+    // the tiles hold the encoded width only (columns up to the K-step boundary
+    // are zero and never stored); +4 / +8 keeps 16-byte rows off one bank.
+    // The output tile (stride ls floats) aliases the input tile and is written
+    // 32 rows at a time while the next 32 input rows are still to be read, so
+    // an input row must be at least as long as an output row.
     a.ls = std::max(std::max(c->n_theta, c->n_phi), 16) + 1;
+    const int ls4 = (a.ls + 3) / 4 * 4;
+    a.xs = with_df ? std::max((ks + 3) / 4 * 4 + 4, ls4) : 0;
+    a.xs_nrc = with_nrc ? std::max((c->net[2].spec.enc_width + 7) / 8 * 8 + 8, 2 * ls4) : 0;
 }
 
 QueryArgs dfg::base_args(dfg_ctx* c) {
@@ -577,9 +599,29 @@ static const float* compact_pos(dfg_ctx* c, const float* feat, int stride, int p
     return c->pos.p;
 }
 
+void dfg::refresh_pair_table(dfg_ctx* c, hipStream_t st) {
+    if (!c->pair_ok) return;
+    const NetSpec& ns = c->net[0].spec;
+    const int64_t entries = ns.grid.offset[ns.grid.n_levels];
+    c->tab_pair.ensure((size_t)entries);
+    launch_interleave_tables(c->net[0].tab_infer(), c->net[1].tab_infer(), c->tab_pair.p, entries, st);
+}
+
 void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
     if (n < c->prepass_min_rows) return;
     const int64_t ld = round_up(n, 64);
+    if (c->pair_ok) {
+        const size_t cols = (size_t)c->net[0].spec.grid.n_levels * 2;
+        c->net[0].gfeat.ensure(cols * ld);
+        c->net[1].gfeat.ensure(cols * ld);
+        const float* pos = compact_pos(c, feat, stride, grid_pos_dim(c->net[0].spec), n, st);
+        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, pos, n, ld, c->net[0].gfeat.p,
+                                c->net[1].gfeat.p, st);
+        a.gfeat_th = c->net[0].gfeat.p;
+        a.gfeat_ph = c->net[1].gfeat.p;
+        a.gld = ld;
+        return;
+    }
     const float* pos = nullptr;
     int pos_dim = -1;
     for (int i = 0; i < 2; ++i) {
@@ -918,6 +960,7 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
     net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    refresh_pair_table(ctx, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
@@ -1058,6 +1101,7 @@ int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* fea
                          ctx->loss.p, 1024.0f, B, ld, st);
     plain_net_backward_update(th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     plain_net_backward_update(ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    refresh_pair_table(ctx, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);

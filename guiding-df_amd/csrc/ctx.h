@@ -94,6 +94,10 @@ struct dfg_ctx {
     uint64_t seed = 7272;
     int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
     bool grid_bwd_bucketed = true;     // bucketed (LDS) grid backward; false: fp32 atomic scatter
+    // theta and phi grids of identical structure (F = 2): their inference
+    // tables are also kept interleaved, one 8-byte entry per vertex for both
+    bool pair_ok = false;
+    DevBuf<uint2> tab_pair;
     std::string err;
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
@@ -105,6 +109,7 @@ struct dfg_ctx {
     DevBuf<int4> gb_items;
     DevBuf<int> gb_n_items;
     ~dfg_ctx() {
+        tab_pair.release();
         gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
         gb_items.release(); gb_n_items.release();
         for (auto& n : net) n.release_all();
@@ -119,6 +124,7 @@ QueryArgs base_args(dfg_ctx* c);
 void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df);
 void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st);
 void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride);
+void refresh_pair_table(dfg_ctx* c, hipStream_t st);
 }  // namespace dfg
 
 #define API_BEGIN try {

@@ -249,6 +249,58 @@ __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __
         for (int f = 0; f < F; ++f) res[f] += c.w[k] * v[k][f];
 }
 
+// One level of two grids of identical structure (the theta and phi nets) from
+// their interleaved table: entry e is {theta f0 f1, phi f0 f1} in 8 bytes, so
+// one line fetch serves both nets.  Same corner order and sums as grid_level<2>.
+__device__ __forceinline__ void grid_level_pair(const GridSpec& g, const uint2* __restrict__ tp, int level,
+                                                float px, float py, float pz, float (&rt)[2], float (&rp)[2]) {
+#pragma clang fp contract(off)
+    const uint2* lt = tp + g.offset[level];
+    GridCorners c;
+    grid_corners(g, level, px, py, pz, c);
+    if (g.interp == INTERP_NEAREST) {
+        const uint2 e = lt[c.idx[0]];
+        const h2 a = __builtin_bit_cast(h2, e.x), b = __builtin_bit_cast(h2, e.y);
+        rt[0] = (float)a[0]; rt[1] = (float)a[1];
+        rp[0] = (float)b[0]; rp[1] = (float)b[1];
+        return;
+    }
+    float vt[8][2], vp[8][2];
+    const bool zdup = !(g.hashed[level] & 1u) && g.s2[level] == 0u;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int k0 = 2 * p, k1 = 2 * p + 1;
+        if (zdup && p >= 2) {
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                vt[k0][f] = vt[k0 - 4][f]; vt[k1][f] = vt[k1 - 4][f];
+                vp[k0][f] = vp[k0 - 4][f]; vp[k1][f] = vp[k1 - 4][f];
+            }
+            continue;
+        }
+        const uint32_t i0 = c.idx[k0], i1 = c.idx[k1];
+        const uint4 pr = *(const uint4*)(lt + (i0 & ~1u));
+        const uint2 lo = make_uint2(pr.x, pr.y), hi = make_uint2(pr.z, pr.w);
+        const uint2 e0 = (i0 & 1u) ? hi : lo;
+        uint2 e1 = (i0 & 1u) ? lo : hi;
+        if (i1 != (i0 ^ 1u)) e1 = lt[i1];
+        const h2 t0 = __builtin_bit_cast(h2, e0.x), p0 = __builtin_bit_cast(h2, e0.y);
+        const h2 t1 = __builtin_bit_cast(h2, e1.x), p1 = __builtin_bit_cast(h2, e1.y);
+        vt[k0][0] = (float)t0[0]; vt[k0][1] = (float)t0[1];
+        vt[k1][0] = (float)t1[0]; vt[k1][1] = (float)t1[1];
+        vp[k0][0] = (float)p0[0]; vp[k0][1] = (float)p0[1];
+        vp[k1][0] = (float)p1[0]; vp[k1][1] = (float)p1[1];
+    }
+    rt[0] = rt[1] = rp[0] = rp[1] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            rt[f] += c.w[k] * vt[k][f];
+            rp[f] += c.w[k] * vp[k][f];
+        }
+}
+
 // Spherical harmonics up to degree 4 of (2x-1, 2y-1, 2z-1), tcnn
 // spherical_harmonics.h:46-152 (the coefficients are the closed forms in its
 // comments, e.g. 1/(2 sqrt(pi))).
@@ -287,9 +339,11 @@ template <typename In, typename Out>
 // the grid part copies them instead of gathering.
 // skip_grid: the grid part is neither gathered nor written (its features
 // reach the MLP straight from the pre-pass buffer).
+// pad_k: also write zeros from enc_width up to the K-step boundary (the MLP
+// loader treats those columns as zero either way).
 __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out,
                                            const float* __restrict__ gfeat = nullptr, int64_t gstride = 0,
-                                           bool skip_grid = false) {
+                                           bool skip_grid = false, bool pad_k = true) {
 #pragma clang fp contract(off)
     for (int pi = 0; pi < ns.n_parts; ++pi) {
         const EncPart& P = ns.parts[pi];
@@ -379,7 +433,8 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
             }
         }
     }
-    for (int c = ns.enc_width; c < ns.k_steps * 32; ++c) out(c, 0.0f);
+    if (pad_k)
+        for (int c = ns.enc_width; c < ns.k_steps * 32; ++c) out(c, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -691,8 +746,10 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 const TT* src = xt + ((q0 + q) * 16 + i16) * xs + (s - gd.steps) * 32 + g * 8;
+                // columns from enc_width to the K-step boundary are zero and not in the tile
+                const bool beyond = s * 32 + g * 8 >= ns.enc_width;
                 if constexpr (sizeof(TT) == 2) {
-                    b[q] = *(const h8*)src;
+                    b[q] = beyond ? h8{} : *(const h8*)src;
                 } else {
                     float v[8];
                     if (s < gd.steps) {
@@ -701,6 +758,9 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                         const float* gsrc = gd.base + (size_t)(s * 32 + g * 8) * gd.stride + rr;
 #pragma unroll
                         for (int j = 0; j < 8; ++j) v[j] = gsrc[(size_t)j * gd.stride];
+                    } else if (beyond) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[j] = 0.0f;
                     } else {
                         const float4 v0 = *(const float4*)src;
                         const float4 v1 = *(const float4*)(src + 4);
@@ -842,6 +902,29 @@ __device__ __forceinline__ void grid_encode_block(const NetSpec& ns, const _Floa
     grid_level<F>(g, table, level, p[0], p[1], p[2], r);
 #pragma unroll
     for (int f = 0; f < F; ++f) out[(size_t)(level * F + f) * ld + row] = r[f];
+}
+
+// The theta and phi pre-pass in one launch over their interleaved table
+// (same XCD-aware level mapping as grid_encode_block).
+__device__ __forceinline__ void grid_encode_pair_block(const NetSpec& ns, const uint2* __restrict__ tp,
+                                                       const float* __restrict__ pos, int64_t n, int64_t ld,
+                                                       float* __restrict__ out_th, float* __restrict__ out_ph,
+                                                       int n_groups, int chunks) {
+    const GridSpec& g = ns.grid;
+    const int b = blockIdx.x;
+    const int phase = b / (n_groups * chunks);
+    const int rem = b - phase * n_groups * chunks;
+    const int level = phase * n_groups + rem % n_groups;
+    const int64_t row = (int64_t)(rem / n_groups) * blockDim.x + threadIdx.x;
+    if (row >= n || level >= g.n_levels) return;
+    const float* p = pos + row * 3;
+    float rt[2], rp[2];
+    grid_level_pair(g, tp, level, p[0], p[1], p[2], rt, rp);
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        out_th[(size_t)(level * 2 + f) * ld + row] = rt[f];
+        out_ph[(size_t)(level * 2 + f) * ld + row] = rp[f];
+    }
 }
 
 }  // namespace dfg

@@ -67,9 +67,10 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     if (gk_th) {
         const int o = 32 * gk_th;
         encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
-                   nullptr, 0, true);
+                   nullptr, 0, true, false);
     } else {
-        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld);
+        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld,
+                   false, false);
     }
     mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1});
@@ -96,9 +97,10 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     if (gk_ph) {
         const int o = 32 * gk_ph;
         encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
-                   nullptr, 0, true);
+                   nullptr, 0, true, false);
     } else {
-        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld);
+        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
+                   false, false);
     }
     mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1});
@@ -123,7 +125,8 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         // the radiance cache keeps its (widest) encoded input in fp16
         _Float16* const xh = (_Float16*)smem;
         _Float16* const xhrow = xh + lane * a.xs_nrc;
-        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; });
+        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; },
+                   nullptr, 0, false, false);
         mlp_forward_tile<1, true>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
         if (valid && io.rgb) {
             io.rgb[3 * r + 0] = lrow[0];
@@ -134,7 +137,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
 }
 
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
-__global__ __launch_bounds__(64) void query_fused_kernel(const QueryArgs a) {
+__global__ __launch_bounds__(64, 3) void query_fused_kernel(const QueryArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const RowsIO io{a.feat, a.stride, a.u, a.guided, a.wi, a.rgb, a.n, 0};
     query_tile<NTH, NPH, WITH_DF, WITH_NRC>(a, io, (int64_t)blockIdx.x * 64, smem);
@@ -266,6 +269,26 @@ __global__ __launch_bounds__(256) void grid_encode_kernel(const NetSpec* __restr
     grid_encode_block<F>(*dns, table, pos, n_dev ? min(n, *n_dev) : n, ld, out, n_groups, chunks);
 }
 
+__global__ __launch_bounds__(256) void grid_encode_pair_kernel(const NetSpec* __restrict__ dns, const uint2* tp,
+                                                               const float* pos, int64_t n, int64_t ld, float* out_th,
+                                                               float* out_ph, int n_groups, int chunks,
+                                                               const int64_t* n_dev) {
+    grid_encode_pair_block(*dns, tp, pos, n_dev ? min(n, *n_dev) : n, ld, out_th, out_ph, n_groups, chunks);
+}
+
+// Interleaved copy {theta f0 f1, phi f0 f1} of two F = 2 grid tables.
+__global__ void interleave_tables_kernel(const uint32_t* __restrict__ th, const uint32_t* __restrict__ ph,
+                                         uint2* __restrict__ tp, int64_t n_entries) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n_entries) tp[e] = make_uint2(th[e], ph[e]);
+}
+
+void launch_interleave_tables(const _Float16* th, const _Float16* ph, uint2* tp, int64_t n_entries, hipStream_t s) {
+    if (n_entries <= 0) return;
+    hipLaunchKernelGGL(interleave_tables_kernel, dim3((unsigned)((n_entries + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t*)th, (const uint32_t*)ph, tp, n_entries);
+}
+
 __global__ void extract_pos_kernel(const float* __restrict__ feat, int stride, int pos_dim, int64_t n,
                                    float* __restrict__ pos, const int64_t* n_dev) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,6 +309,17 @@ void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, f
     if (n <= 0) return;
     hipLaunchKernelGGL(extract_pos_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, s, feat, stride,
                        pos_dim, n, pos, n_dev);
+}
+
+void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2* tp, const float* pos, int64_t n,
+                             int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev) {
+    if (n <= 0 || !hs.grid.present) return;
+    const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
+    const int phases = (hs.grid.n_levels + groups - 1) / groups;
+    const int chunks = (int)((n + 255) / 256);
+    const unsigned blocks = (unsigned)((int64_t)chunks * groups * phases);
+    hipLaunchKernelGGL(grid_encode_pair_kernel, dim3(blocks), dim3(256), 0, s, dns, tp, pos, n, ld, out_th, out_ph,
+                       groups, chunks, n_dev);
 }
 
 void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,

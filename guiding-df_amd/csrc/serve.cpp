@@ -73,7 +73,14 @@ static void wavefront_enqueue(dfg_wavefront* w, hipStream_t st, int n_bounces, c
     a.stride = w->c_stride;
     a.n = w->capacity;
     const int64_t* n_dev = &w->wl.p->total_rows;
-    if (w->tune.grid_prepass) {
+    if (w->tune.grid_prepass && c->pair_ok) {
+        launch_extract_pos(w->c_feat, w->c_stride, grid_pos_dim(c->net[0].spec), w->capacity, w->pos.p, st, n_dev);
+        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, w->pos.p, w->capacity, ld,
+                                w->gth.p, w->gph.p, st, n_dev);
+        a.gfeat_th = w->gth.p;
+        a.gfeat_ph = w->gph.p;
+        a.gld = ld;
+    } else if (w->tune.grid_prepass) {
         int pd = -1;
         for (int i = 0; i < 2; ++i) {
             NetState& s = c->net[i];

@@ -381,7 +381,7 @@ def main():
         "pdf_rel_err_vs_fp64_after_training": err_trained,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic() if n == 1 << 20 else None,
-                     "kernel": "one dfg_query call: extract_pos_kernel + 2 x grid_encode_kernel<2> + "
+                     "kernel": "one dfg_query call: extract_pos_kernel + grid_encode_pair_kernel (theta+phi) + "
                                "query_fused_kernel<32,32,true,true>",
                      "kernel_ms": kernel_ms,
                      "bytes_per_query": bytes_q, "flops_per_query": fl_q,

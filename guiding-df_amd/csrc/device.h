@@ -355,15 +355,26 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
                 for (int d = 0; d < P.n_dims; ++d) out(ob + d, in(P.dim_begin + d) * P.id_scale + P.id_offset);
                 break;
             case ENC_ONEBLOB: {
-                // kernel_one_blob_soa, tcnn oneblob.h:98-127
+                // kernel_one_blob_soa, tcnn oneblob.h:98-127: bin k gets
+                // S(b_{k+1}) - S(b_k), S(b) = sum over shifts s in {0,-1,+1} of
+                // Q(b - x + s).  Q is monotone and saturates (0 / 1) outside
+                // |t| < 1/n_bins, so for n_bins >= 2 only the shift nearest zero
+                // is unsaturated: S(b) = (1 + r) + Q(t - r), t = b - x,
+                // r = rint(t) clamped to [-1, 1] (t - r is exact, so Q sees the
+                // same argument as in the three-term sum).
                 const int nb = P.param;
                 const float fnb = (float)nb;
                 for (int d = 0; d < P.n_dims; ++d) {
                     const float x = in(P.dim_begin + d);
-                    float left = quartic_cdf(-x, fnb) + quartic_cdf(-x - 1.0f, fnb) + quartic_cdf(-x + 1.0f, fnb);
+                    auto S = [&](float b) {
+                        const float t = b - x;
+                        if (nb < 2) return quartic_cdf(t, fnb) + quartic_cdf(t - 1.0f, fnb) + quartic_cdf(t + 1.0f, fnb);
+                        const float r = fminf(fmaxf(__builtin_rintf(t), -1.0f), 1.0f);
+                        return (1.0f + r) + quartic_cdf(t - r, fnb);
+                    };
+                    float left = S(0.0f);
                     for (int k = 0; k < nb; ++k) {
-                        const float rb = (float)(k + 1) / fnb;
-                        const float right = quartic_cdf(rb - x, fnb) + quartic_cdf(rb - x - 1.0f, fnb) + quartic_cdf(rb - x + 1.0f, fnb);
+                        const float right = S((float)(k + 1) / fnb);
                         out(ob + d * nb + k, right - left);
                         left = right;
                     }
@@ -444,6 +455,12 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
 
 // softmax2 + softmaxMultFactor_F, train.h:844-893 (fp32 storage): in-place on
 // registers z[N] -> N * softmax(z), with the max initialised to 0.
+// softmax2 / softmaxMultFactor_F (train.h:844-893): N exp(z_i - m) / sum_j
+// exp(z_j - m), m = max(0, max z).  The reference normalises with a second
+// exponential, exp(z_i - m - log sum); here each exponential is taken once and
+// scaled by N / sum (the two agree to a few ulp; the oracle's FP32 mode does
+// the same).  A sum that underflowed to 0 gives +inf bins, as the reference's
+// log(0) does.
 template <int N>
 __device__ __forceinline__ void softmax_times_n(float (&z)[N]) {
 #pragma clang fp contract(off)
@@ -452,10 +469,13 @@ __device__ __forceinline__ void softmax_times_n(float (&z)[N]) {
     for (int i = 0; i < N; ++i) mx = fmaxf(mx, z[i]);
     float s = 0.0f;
 #pragma unroll
-    for (int i = 0; i < N; ++i) s += det_expf(z[i] - mx);
-    const float ls = det_logf(s);
+    for (int i = 0; i < N; ++i) {
+        z[i] = det_expf(z[i] - mx);
+        s += z[i];
+    }
+    const float scale = (float)N / s;
 #pragma unroll
-    for (int i = 0; i < N; ++i) z[i] = det_expf((z[i] - mx) - ls) * (float)N;
+    for (int i = 0; i < N; ++i) z[i] = s > 0.0f ? z[i] * scale : __builtin_inff();
 }
 
 // Training variant: the reference stores the softmax in fp16 and then

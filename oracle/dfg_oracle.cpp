@@ -352,14 +352,21 @@ void mlp_row(const OrcNet& nt, const R* params, const R* x, R* out, R* acts) {
 // ---------------------------------------------------------------------------
 // DF head (train.h:345-520, 649-681, 793-812, 844-893)
 // ---------------------------------------------------------------------------
+// max initialised to 0 (train.h:846).  N exp(z_i - m) / sum: the reference
+// divides through a second exponential, exp(z_i - m - log sum); the product
+// with N / sum is the same quantity (exact in FP64, within a few ulp in FP32)
+// and is what the library computes, so the FP32 mode stays its bit-exact twin.
 template <typename R>
-void softmax_times_n(const R* z, int N, R* p) {  // max initialised to 0 (train.h:846)
+void softmax_times_n(const R* z, int N, R* p) {
     R mx = 0;
     for (int i = 0; i < N; ++i) mx = std::fmax(mx, z[i]);
     R s = 0;
-    for (int i = 0; i < N; ++i) s += r_exp<R>(z[i] - mx);
-    const R ls = r_log<R>(s);
-    for (int i = 0; i < N; ++i) p[i] = r_exp<R>((z[i] - mx) - ls) * (R)N;
+    for (int i = 0; i < N; ++i) {
+        p[i] = r_exp<R>(z[i] - mx);
+        s += p[i];
+    }
+    const R scale = (R)N / s;
+    for (int i = 0; i < N; ++i) p[i] = s > 0 ? p[i] * scale : (R)INFINITY;
 }
 
 template <typename R>

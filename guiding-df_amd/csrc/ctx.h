@@ -102,6 +102,10 @@ struct dfg_ctx {
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
     DevBuf<float> tmp, pos, mean_ws;
+    // training rows generated from records, and their per-step losses
+    DevBuf<float> rec_feat, rec_tgt, rec_loss;
+    DevBuf<uint32_t> rec_local;
+    DevBuf<int64_t> rec_blk, rec_count;
     // bucketed grid-backward workspace (shared by the nets, used in stream order)
     DevBuf<int64_t> gb_offs, gb_cursor;
     DevBuf<uint32_t> gb_idx, gb_cnt;
@@ -115,6 +119,8 @@ struct dfg_ctx {
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
         pos.release(); mean_ws.release();
+        rec_feat.release(); rec_tgt.release(); rec_loss.release(); rec_local.release(); rec_blk.release();
+        rec_count.release();
     }
 };
 

@@ -1,0 +1,188 @@
+// Training rows from recorded path vertices: the step before the training
+// step (generate_training_data_AR, train.cu:6-96, and
+// generate_training_data_nrc, train.cu:98-151).  Rows are compacted in a
+// deterministic order (training pixel t, then depth) by count / scan / write:
+//   count : per 256-pixel block, valid vertices per pixel and their in-block
+//           exclusive offsets; the block total
+//   scan  : one block, exclusive offsets of the block totals; the row count
+//   write : each pixel writes its rows at block offset + in-block offset
+#include "../../include/dfg.h"
+#include "device.h"
+#include "internal.h"
+
+namespace dfg {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647693f;
+constexpr float kEps = 1e-5f;  // M_EPSILON, constants.h:19
+
+// sphericalToCartesian, math_utils.h:198-208
+__device__ __forceinline__ void sph_to_cart(float theta, float phi, float (&v)[3]) {
+    const float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+    v[0] = st * cp;
+    v[1] = st * sp;
+    v[2] = ct;
+}
+
+struct Row {
+    float feat[15];
+    float tgt[9];
+};
+
+// One record (depth k, pixel p) as a training row; false when the reference
+// skips it.  mode 0: DF row (feat 9, target 9); 1: NRC row (feat 15, target 3).
+__device__ bool record_row(const dfg_records& r, int k, int64_t p, const float* __restrict__ aabb, int mode, Row& o) {
+    const int64_t i = (int64_t)k * r.n_pixels + p;
+    if (r.delta[i]) return false;
+    if (mode == 1 && r.miss[i]) return false;
+    // normalizeSpatialCoord (train.h:272-276): the box inflated by 0.5 % of its diagonal
+    float d[3];
+    for (int c = 0; c < 3; ++c) d[c] = aabb[3 + c] - aabb[c];
+    const float infl = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * 0.005f;
+    for (int c = 0; c < 3; ++c) {
+        const float lo = aabb[c] - infl, hi = aabb[3 + c] + infl;
+        o.feat[c] = 0.5f + (r.pos[3 * i + c] - (lo + hi) * 0.5f) / (hi - lo);
+    }
+    // warp_direction_for_sh of the recorded wo (train.h:1243-1245)
+    float w[3];
+    sph_to_cart(r.wo[2 * i], r.wo[2 * i + 1], w);
+    for (int c = 0; c < 3; ++c) o.feat[3 + c] = (w[c] + 1.0f) * 0.5f;
+    o.feat[6] = r.normal[2 * i] * (1.0f / kPi);
+    o.feat[7] = r.normal[2 * i + 1] * (1.0f / kTwoPi);
+    o.feat[8] = 1.0f - expf(-r.roughness[i]);  // warp_roughness_for_ob, train.h:1247-1249
+    bool bad = false;
+    for (int c = 0; c < 6; ++c) bad |= o.feat[c] != o.feat[c];
+    // local radiance: L / thp per channel where thp > eps, times the MIS weight
+    float L[3];
+    const float mis = r.mis_weight[i];
+    for (int c = 0; c < 3; ++c) {
+        const float t = r.thp[3 * i + c];
+        L[c] = (t > kEps ? r.L[3 * i + c] / t : 0.0f) * mis;
+    }
+    const float pdf = r.wi_pdf[i];
+    if (mode == 0) {
+        // uniformSphereToSquare(sphericalToCartesian(dir)), math_utils.h:233-240
+        float v[3];
+        sph_to_cart(r.dir[2 * i], r.dir[2 * i + 1], v);
+        const float n = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int c = 0; c < 3; ++c) v[c] /= n;
+        float ph = atan2f(v[1], v[0]) * (1.0f / kTwoPi);
+        if (ph < 0.0f) ph += 1.0f;
+        o.tgt[0] = (1.0f - v[2]) * 0.5f;
+        o.tgt[1] = ph;
+        // the reference tests L before the bsdfVal product (train.cu:80-84)
+        bad |= o.tgt[0] != o.tgt[0] || o.tgt[1] != o.tgt[1] || pdf != pdf || pdf == 0.0f;
+        for (int c = 0; c < 3; ++c) bad |= L[c] != L[c];
+        for (int c = 0; c < 3; ++c) o.tgt[2 + c] = L[c] * r.bsdf_val[3 * i + c];
+        o.tgt[5] = pdf;
+        o.tgt[6] = o.tgt[7] = o.tgt[8] = 0.0f;
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            o.feat[9 + c] = r.diffuse[3 * i + c];
+            o.feat[12 + c] = r.specular[3 * i + c];
+            o.tgt[c] = L[c] * r.bsdf_val[3 * i + c] / (pdf + kEps);
+            bad |= o.tgt[c] != o.tgt[c];
+        }
+    }
+    return !bad;
+}
+
+__device__ __forceinline__ int pixel_depth(const dfg_records& r, int64_t p, int mode) {
+    int d = (int)r.cur_depth[p];
+    if (d > r.max_depth) d = r.max_depth;
+    // the NRC pass has no radiance estimate for the last recorded vertex
+    return mode == 1 ? d - 1 : d;
+}
+
+__global__ __launch_bounds__(256) void records_count_kernel(const dfg_records r, int64_t n_train, uint32_t offset,
+                                                            uint32_t stride, const float* __restrict__ aabb, int mode,
+                                                            uint32_t* __restrict__ local, int64_t* __restrict__ block_tot) {
+    __shared__ uint32_t s[256];
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t cnt = 0;
+    if (t < n_train) {
+        const int64_t p = (int64_t)offset + t * (int64_t)stride;
+        const int depth = pixel_depth(r, p, mode);
+        Row o;
+        for (int k = 0; k < depth; ++k) cnt += record_row(r, k, p, aabb, mode, o) ? 1u : 0u;
+    }
+    s[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint32_t y = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += y;
+        __syncthreads();
+    }
+    if (t < n_train) local[t] = s[threadIdx.x] - cnt;
+    if (threadIdx.x == 255) block_tot[blockIdx.x] = s[255];
+}
+
+// one block: block totals -> exclusive offsets; the row count to *count
+__global__ __launch_bounds__(1024) void records_scan_kernel(int64_t* __restrict__ v, int64_t m,
+                                                            int64_t* __restrict__ count) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (m + 1023) / 1024;
+    int64_t b = (int64_t)t * per;
+    if (b > m) b = m;
+    int64_t e = b + per;
+    if (e > m) e = m;
+    int64_t sum = 0;
+    for (int64_t i = b; i < e; ++i) sum += v[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int64_t y = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;
+    for (int64_t i = b; i < e; ++i) {
+        const int64_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+    if (t == 1023) *count = part[1023];
+}
+
+__global__ __launch_bounds__(256) void records_write_kernel(const dfg_records r, int64_t n_train, uint32_t offset,
+                                                            uint32_t stride, const float* __restrict__ aabb, int mode,
+                                                            const uint32_t* __restrict__ local,
+                                                            const int64_t* __restrict__ block_off,
+                                                            float* __restrict__ feat, float* __restrict__ tgt,
+                                                            int64_t capacity) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_train) return;
+    const int64_t p = (int64_t)offset + t * (int64_t)stride;
+    const int depth = pixel_depth(r, p, mode);
+    int64_t at = block_off[blockIdx.x] + local[t];
+    const int nf = mode == 0 ? 9 : 15, nt = mode == 0 ? 9 : 3;
+    Row o;
+    for (int k = 0; k < depth; ++k) {
+        if (!record_row(r, k, p, aabb, mode, o)) continue;
+        if (at < capacity) {
+            for (int c = 0; c < nf; ++c) feat[at * nf + c] = o.feat[c];
+            for (int c = 0; c < nt; ++c) tgt[at * nt + c] = o.tgt[c];
+        }
+        ++at;
+    }
+}
+
+}  // namespace
+
+void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t offset, uint32_t stride,
+                             const float* aabb, int mode, float* feat, float* tgt, int64_t capacity,
+                             uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s) {
+    const unsigned blocks = (unsigned)((n_train + 255) / 256);
+    hipLaunchKernelGGL(records_count_kernel, dim3(blocks), dim3(256), 0, s, r, n_train, offset, stride, aabb, mode,
+                       local, block_tot);
+    hipLaunchKernelGGL(records_scan_kernel, dim3(1), dim3(1024), 0, s, block_tot, (int64_t)blocks, count);
+    hipLaunchKernelGGL(records_write_kernel, dim3(blocks), dim3(256), 0, s, r, n_train, offset, stride, aabb, mode,
+                       local, block_tot, feat, tgt, capacity);
+}
+
+}  // namespace dfg

@@ -1,0 +1,165 @@
+// Training from recorded path vertices: row generation entry points and the
+// reference's per-frame training passes (LearnedDistribution::trainStep,
+// learnedDistribution.cpp:208-293; NRC::trainStep, nrc.cpp:114-170).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace dfg;
+
+namespace dfg {
+void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t offset, uint32_t stride,
+                             const float* aabb, int mode, float* feat, float* tgt, int64_t capacity,
+                             uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s);
+}
+
+namespace {
+
+constexpr int64_t kMinTrainBatch = 65536;  // MIN_TRAIN_BATCH_SIZE, parameters.h:55
+
+void check_records(const dfg_records* r, int64_t n_train, uint32_t offset, uint32_t stride, const float* aabb,
+                   int mode) {
+    if (!r || !aabb) throw Error(DFG_ERR_INVALID_ARG, "null records or aabb");
+    if (r->max_depth < 1 || r->n_pixels < 1) throw Error(DFG_ERR_BAD_SIZE, "empty record buffer");
+    if (n_train < 0) throw Error(DFG_ERR_BAD_SIZE, "n_train < 0");
+    if (n_train > 0 && stride < 1) throw Error(DFG_ERR_BAD_SIZE, "pixel_stride < 1");
+    if (n_train > 0 && (int64_t)offset + (n_train - 1) * (int64_t)stride >= r->n_pixels)
+        throw Error(DFG_ERR_BAD_SIZE, "training pixels beyond n_pixels");
+    const void* need[] = {r->cur_depth, r->pos, r->wo, r->normal, r->roughness, r->L, r->thp, r->bsdf_val,
+                          r->wi_pdf, r->mis_weight, r->delta};
+    for (const void* p : need)
+        if (!p) throw Error(DFG_ERR_INVALID_ARG, "null record field");
+    if (mode == 0 && !r->dir) throw Error(DFG_ERR_INVALID_ARG, "null record field dir");
+    if (mode == 1 && (!r->diffuse || !r->specular || !r->miss)) throw Error(DFG_ERR_INVALID_ARG, "null NRC record field");
+}
+
+void targets(dfg_ctx* c, hipStream_t st, const dfg_records* r, int64_t n_train, uint32_t offset, uint32_t stride,
+             const float* aabb, int mode, float* feat, float* tgt, int64_t capacity, int64_t* count) {
+    if (n_train == 0) {
+        launch_fill_u32((unsigned int*)count, 0u, 2, st);
+        return;
+    }
+    const int64_t blocks = (n_train + 255) / 256;
+    c->rec_local.ensure((size_t)n_train);
+    c->rec_blk.ensure((size_t)blocks);
+    launch_training_targets(*r, n_train, offset, stride, aabb, mode, feat, tgt, capacity, c->rec_local.p,
+                            c->rec_blk.p, count, st);
+    check_launch();
+}
+
+// rows into the context's training buffer; the row count on the host
+int64_t targets_to_buffer(dfg_ctx* c, hipStream_t st, const dfg_records* r, int64_t n_train, uint32_t offset,
+                          uint32_t stride, const float* aabb, int mode) {
+    const int64_t cap = n_train * r->max_depth;
+    const int nf = mode == 0 ? 9 : 15, nt = mode == 0 ? 9 : 3;
+    c->rec_feat.ensure((size_t)std::max<int64_t>(cap, 1) * nf);
+    c->rec_tgt.ensure((size_t)std::max<int64_t>(cap, 1) * nt);
+    c->rec_count.ensure(1);
+    targets(c, st, r, n_train, offset, stride, aabb, mode, c->rec_feat.p, c->rec_tgt.p, cap, c->rec_count.p);
+    int64_t rows = 0;
+    HIPX(hipMemcpyAsync(&rows, c->rec_count.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    return std::min(rows, cap);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfg_training_targets(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train, uint32_t pixel_offset,
+                         uint32_t pixel_stride, const float* aabb, float* feat, float* target, int64_t capacity,
+                         int64_t* count) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_records(rec, n_train, pixel_offset, pixel_stride, aabb, 0);
+    if (!count || capacity < 0 || (capacity > 0 && (!feat || !target))) throw Error(DFG_ERR_INVALID_ARG, "bad output");
+    targets(ctx, (hipStream_t)stream, rec, n_train, pixel_offset, pixel_stride, aabb, 0, feat, target, capacity, count);
+    API_END(ctx)
+}
+
+int dfg_nrc_training_targets(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
+                             uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, float* feat,
+                             float* target_rgb, int64_t capacity, int64_t* count) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_records(rec, n_train, pixel_offset, pixel_stride, aabb, 1);
+    if (!count || capacity < 0 || (capacity > 0 && (!feat || !target_rgb)))
+        throw Error(DFG_ERR_INVALID_ARG, "bad output");
+    targets(ctx, (hipStream_t)stream, rec, n_train, pixel_offset, pixel_stride, aabb, 1, feat, target_rgb, capacity,
+            count);
+    API_END(ctx)
+}
+
+int dfg_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train, uint32_t pixel_offset,
+                      uint32_t pixel_stride, const float* aabb, int64_t batch_size, int batch_per_frame,
+                      int64_t* n_rows, int* n_steps, float* loss) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    check_records(rec, n_train, pixel_offset, pixel_stride, aabb, 0);
+    if (batch_size < 128 || batch_per_frame < 1) throw Error(DFG_ERR_BAD_SIZE, "bad batch schedule");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t rows = targets_to_buffer(ctx, st, rec, n_train, pixel_offset, pixel_stride, aabb, 0);
+    // learnedDistribution.cpp:226-236
+    const int planned = (int)std::min<int64_t>(rows / batch_size + 1, batch_per_frame);
+    ctx->rec_loss.ensure((size_t)planned);
+    int run = 0;
+    for (int it = 0; it < planned; ++it) {
+        int64_t b = std::min<int64_t>(rows - (int64_t)it * batch_size, batch_size);
+        b -= b % 128;
+        if (b < kMinTrainBatch) break;
+        const int64_t r0 = (int64_t)it * batch_size;
+        const int rc = dfg_train_step(ctx, st, b, ctx->rec_feat.p + r0 * 9, 9, ctx->rec_tgt.p + r0 * 9,
+                                      ctx->rec_loss.p + it);
+        if (rc != DFG_OK) throw Error(rc, ctx->err);
+        ++run;
+    }
+    std::vector<float> l((size_t)std::max(run, 1), 0.0f);
+    if (run) HIPX(hipMemcpyAsync(l.data(), ctx->rec_loss.p, sizeof(float) * run, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    float sum = 0.0f;
+    for (int i = 0; i < run; ++i) sum += l[i];
+    if (n_rows) *n_rows = rows;
+    if (n_steps) *n_steps = run;
+    if (loss) *loss = sum / (float)planned;
+    API_END(ctx)
+}
+
+int dfg_nrc_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train, uint32_t pixel_offset,
+                          uint32_t pixel_stride, const float* aabb, int64_t batch_size, int batch_per_frame,
+                          int64_t* n_rows, int* n_steps, float* loss) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    check_records(rec, n_train, pixel_offset, pixel_stride, aabb, 1);
+    if (batch_size < 128 || batch_per_frame < 1) throw Error(DFG_ERR_BAD_SIZE, "bad batch schedule");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t rows = targets_to_buffer(ctx, st, rec, n_train, pixel_offset, pixel_stride, aabb, 1);
+    // nrc.cpp:127-138, including its slice offset (step k starts at k times its size)
+    const int planned = (int)std::max<int64_t>(rows / batch_size + 1, batch_per_frame);
+    const int64_t eff = std::min<int64_t>(rows / planned, batch_size);
+    ctx->rec_loss.ensure((size_t)planned);
+    int run = 0;
+    for (int it = 0; it < planned; ++it) {
+        int64_t b = std::min<int64_t>(rows - (int64_t)it * eff, eff);
+        b -= b % 128;
+        if (b < kMinTrainBatch) break;
+        const int64_t r0 = (int64_t)it * b;
+        const int rc = dfg_nrc_train_step(ctx, st, b, ctx->rec_feat.p + r0 * 15, 15, ctx->rec_tgt.p + r0 * 3,
+                                          ctx->rec_loss.p + it);
+        if (rc != DFG_OK) throw Error(rc, ctx->err);
+        ++run;
+    }
+    std::vector<float> l((size_t)std::max(run, 1), 0.0f);
+    if (run) HIPX(hipMemcpyAsync(l.data(), ctx->rec_loss.p, sizeof(float) * run, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    float sum = 0.0f;
+    for (int i = 0; i < run; ++i) sum += l[i];
+    if (n_rows) *n_rows = rows;
+    if (n_steps) *n_steps = run;
+    if (loss) *loss = sum / (float)planned;
+    API_END(ctx)
+}
+
+}  // extern "C"

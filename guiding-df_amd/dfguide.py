@@ -35,7 +35,8 @@ EXPORTS = [
     "dfg_create", "dfg_destroy", "dfg_last_error", "dfg_get_info", "dfg_get_params", "dfg_get_grad",
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
-    "dfg_nrc_train_step_plain", "dfg_prepare_features",
+    "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
+    "dfg_nrc_train_records", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -84,6 +85,27 @@ class DfgServerStats(ctypes.Structure):
                 ("busy_s", ctypes.c_double)]
 
 
+RECORD_FIELDS = ("cur_depth", "pos", "dir", "wo", "normal", "roughness", "L", "thp", "bsdf_val", "diffuse",
+                 "specular", "wi_pdf", "mis_weight", "delta", "miss")
+
+
+class DfgRecords(ctypes.Structure):
+    _fields_ = [("max_depth", ctypes.c_int), ("n_pixels", ctypes.c_int64)] + \
+               [(f, ctypes.c_void_p) for f in RECORD_FIELDS]
+
+
+def records_struct(rec):
+    """dfg_records view of a dict of device tensors shaped [max_depth, n_pixels(, k)]
+    (cur_depth: [n_pixels] uint32 / int32)."""
+    d, n = rec["pos"].shape[0], rec["pos"].shape[1]
+    r = DfgRecords()
+    r.max_depth, r.n_pixels = d, n
+    for f in RECORD_FIELDS:
+        t = rec.get(f)
+        setattr(r, f, None if t is None else t.data_ptr())
+    return r
+
+
 _lib = None
 
 
@@ -96,7 +118,7 @@ def load_library(path=LIB_PATH):
         raise FileNotFoundError(
             "libdfg.so not built (%s); run python guiding-df_amd/build.py" % path)
     lib = ctypes.CDLL(path)
-    P, I, I64, SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t
+    P, I, I64, SZ, U32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_size_t, ctypes.c_uint32
     sig = {
         "dfg_create": ([ctypes.c_char_p, ctypes.POINTER(P)], I),
         "dfg_destroy": ([P], I),
@@ -111,10 +133,18 @@ def load_library(path=LIB_PATH):
         "dfg_query_plain": ([P, P, I64, P, I, P, P, P, P], I),
         "dfg_pdf": ([P, P, I64, P, I, P, P], I),
         "dfg_train_step": ([P, P, I64, P, I, P, P], I),
-"dfg_train_step_plain": ([P, P, I64, P, I, P, P], I),
+        "dfg_train_step_plain": ([P, P, I64, P, I, P, P], I),
         "dfg_nrc_infer": ([P, P, I64, P, I, P], I),
         "dfg_nrc_train_step": ([P, P, I64, P, I, P, P], I),
-"dfg_nrc_train_step_plain": ([P, P, I64, P, I, P, P], I),
+        "dfg_nrc_train_step_plain": ([P, P, I64, P, I, P, P], I),
+        "dfg_training_targets": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P, P, P, I64, P], I),
+        "dfg_nrc_training_targets": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P, P, P, I64, P], I),
+        "dfg_train_records": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P, I64, I,
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_float)], I),
+        "dfg_nrc_train_records": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P, I64, I,
+                                   ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_float)], I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
         "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
@@ -339,6 +369,37 @@ class GuidingField:
         """dfg_nrc_train_step on the plain FP32 per-batch path (the comparator)."""
         self._c(self.lib.dfg_nrc_train_step_plain(self.ctx, _stream(stream), feat.shape[0], _ptr(feat),
                                                   feat.shape[1], _ptr(target_rgb), _ptr(loss_out)))
+
+    # --- training from recorded path vertices ----------------------------------
+    def training_targets(self, rec, n_train, aabb, feat, target, count, offset=0, stride=1, stream=None):
+        r = records_struct(rec)
+        self._c(self.lib.dfg_training_targets(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset, stride,
+                                              _ptr(aabb), _ptr(feat), _ptr(target), feat.shape[0], _ptr(count)))
+
+    def nrc_training_targets(self, rec, n_train, aabb, feat, target_rgb, count, offset=0, stride=1, stream=None):
+        r = records_struct(rec)
+        self._c(self.lib.dfg_nrc_training_targets(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset,
+                                                  stride, _ptr(aabb), _ptr(feat), _ptr(target_rgb), feat.shape[0],
+                                                  _ptr(count)))
+
+    def _train_records(self, fn, rec, n_train, aabb, batch_size, batch_per_frame, offset, stride, stream):
+        r = records_struct(rec)
+        rows, steps, loss = ctypes.c_int64(0), ctypes.c_int(0), ctypes.c_float(0.0)
+        self._c(fn(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset, stride, _ptr(aabb), batch_size,
+                   batch_per_frame, ctypes.byref(rows), ctypes.byref(steps), ctypes.byref(loss)))
+        return rows.value, steps.value, loss.value
+
+    def train_records(self, rec, n_train, aabb, batch_size=524288, batch_per_frame=5, offset=0, stride=1,
+                      stream=None):
+        """LearnedDistribution::trainStep from records: (rows, steps run, loss)."""
+        return self._train_records(self.lib.dfg_train_records, rec, n_train, aabb, batch_size, batch_per_frame,
+                                   offset, stride, stream)
+
+    def nrc_train_records(self, rec, n_train, aabb, batch_size=1048576, batch_per_frame=4, offset=0, stride=1,
+                          stream=None):
+        """NRC::trainStep from records: (rows, steps run, loss)."""
+        return self._train_records(self.lib.dfg_nrc_train_records, rec, n_train, aabb, batch_size,
+                                   batch_per_frame, offset, stride, stream)
 
     def nrc_infer(self, feat, rgb, stream=None):
         self._c(self.lib.dfg_nrc_infer(self.ctx, _stream(stream), feat.shape[0], _ptr(feat), feat.shape[1],

@@ -163,6 +163,67 @@ int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* fea
 int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
                              const float* target_rgb, float* loss_out);
 
+/* ---- Training records: the step before dfg_train_step ----------------------
+ * Path vertices recorded for the training pixels, one SoA slice per depth as
+ * GuidedPixelStateBuffer holds them (src/render/guided/workqueue.h:26-144):
+ * element (depth d, pixel p) of a field sits at index d * n_pixels + p
+ * (vectors as 3 or 2 consecutive floats).  All pointers are device pointers. */
+typedef struct dfg_records {
+    int max_depth;             /* depth slices (MAX_TRAIN_DEPTH, parameters.h:24)        */
+    int64_t n_pixels;          /* pixels per slice                                      */
+    const uint32_t* cur_depth; /* [n_pixels] vertices recorded per pixel                 */
+    const float* pos;          /* [3] world-space vertex                                */
+    const float* dir;          /* [2] spherical (theta, phi) of the scattered direction */
+    const float* wo;           /* [2] spherical outgoing direction                      */
+    const float* normal;       /* [2] spherical shading normal                          */
+    const float* roughness;    /* [1]                                                   */
+    const float* L;            /* [3] radiance gathered after the vertex                */
+    const float* thp;          /* [3] throughput up to the vertex                       */
+    const float* bsdf_val;     /* [3]                                                   */
+    const float* diffuse;      /* [3]                                                   */
+    const float* specular;     /* [3]                                                   */
+    const float* wi_pdf;       /* [1] pdf of the scattered direction                    */
+    const float* mis_weight;   /* [1]                                                   */
+    const uint8_t* delta;      /* [1] sampled from a delta lobe                         */
+    const uint8_t* miss;       /* [1] the path left the scene there                     */
+} dfg_records;
+
+/* DF training rows (generate_training_data_AR, train.cu:6-96).  For the
+ * training pixels p = pixel_offset + t * pixel_stride, t < n_train, every
+ * recorded vertex that is not delta, has finite inputs and a non-zero wiPdf
+ * becomes one row: feat [9] (as dfg_prepare_features builds them) and target
+ * [9] = (wi square coords (2), L / thp * mis * bsdfVal (3), wiPdf, bsdfVal
+ * slot left 0 as in the reference).  Rows are written pixel-major, depth
+ * inner (a deterministic order; the reference appends through an atomic).
+ * aabb[6] = scene (min xyz, max xyz), device.  *count (device int64) gets the
+ * number of valid rows; rows past `capacity` are not written. */
+int dfg_training_targets(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
+                         uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, float* feat,
+                         float* target, int64_t capacity, int64_t* count);
+/* NRC training rows (generate_training_data_nrc, train.cu:98-151): vertices
+ * before the last of each pixel, neither delta nor miss: feat [15] (the 9 DF
+ * features, diffuse, specular), target_rgb [3] = L / thp * mis * bsdfVal /
+ * (wiPdf + 1e-5). */
+int dfg_nrc_training_targets(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
+                             uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, float* feat,
+                             float* target_rgb, int64_t capacity, int64_t* count);
+/* One DF training pass from records (LearnedDistribution::trainStep,
+ * learnedDistribution.cpp:208-293): rows into the context's training buffer,
+ * then min(rows / batch_size + 1, batch_per_frame) steps on consecutive
+ * slices of batch_size rows, each rounded down to a multiple of 128; a slice
+ * under 65,536 rows ends the pass.  Synchronises `stream`; on the host:
+ * n_rows, n_steps run, and the reference's loss (sum of the per-step mean
+ * losses divided by the planned step count). */
+int dfg_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
+                      uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, int64_t batch_size,
+                      int batch_per_frame, int64_t* n_rows, int* n_steps, float* loss);
+/* NRC::trainStep (nrc.cpp:114-170) from records: its own schedule, max(rows /
+ * batch_size + 1, batch_per_frame) steps of min(rows / steps, batch_size)
+ * rows rounded down to 128, slice k starting at k times its size. */
+int dfg_nrc_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
+                          uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, int64_t batch_size,
+                          int batch_per_frame, int64_t* n_rows, int* n_steps, float* loss);
+
 /* Feature preparation (generate_inference_data_AR, train.cu:334-362):
  * shading points -> DF feature rows [n][9].
  *   pos [n][3] world space, aabb[6] = (min xyz, max xyz), wo [n][3] (any length),

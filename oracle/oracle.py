@@ -377,3 +377,126 @@ def adam_ema_f64(n_matrix, lr, b1, b2, eps, l2, decay, step, grad, master, m, v,
     lib().orc_adam_ema_f64(master.size, n_matrix, lr, b1, b2, eps, l2, decay, step, _p(grad), _p(master), _p(m),
                            _p(v), _p(steps), _p(wl), _p(ema))
     return master, m, v, steps, ema
+
+
+# ----------------------------------------------------------------------------
+# Training records -> training rows (generate_training_data_AR, train.cu:6-96;
+# generate_training_data_nrc, train.cu:98-151), FP64.  Records are the SoA
+# fields of GuidedPixelStateBuffer (workqueue.h:26-144) as [max_depth][n_pixels]
+# arrays; rows come out pixel-major (depth inner) for the sampled training
+# pixels offset + t * stride.
+# ----------------------------------------------------------------------------
+M_EPSILON = 1e-5
+
+
+def spherical_to_cartesian(theta, phi):  # math_utils.h:198-208
+    st, ct, sp, cp = np.sin(theta), np.cos(theta), np.sin(phi), np.cos(phi)
+    return np.stack([st * cp, st * sp, ct], -1)
+
+
+def uniform_sphere_to_square(v):  # math_utils.h:233-240
+    vn = v / np.linalg.norm(v, axis=-1, keepdims=True)
+    phi = np.arctan2(vn[..., 1], vn[..., 0]) / (2 * np.pi)
+    return np.stack([(1.0 - vn[..., 2]) * 0.5, np.where(phi < 0, phi + 1.0, phi)], -1)
+
+
+def square_to_uniform_sphere(u):  # math_utils.h:226-231
+    z = 1.0 - 2.0 * u[..., 0]
+    r = np.sqrt(np.maximum(1.0 - z * z, 0.0))
+    phi = 2 * np.pi * u[..., 1]
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], -1)
+
+
+def normalize_spatial(p, aabb):  # train.h:272-276 (AABB inflated by 0.5 % of its diagonal)
+    lo, hi = np.asarray(aabb[:3], np.float64), np.asarray(aabb[3:], np.float64)
+    d = np.linalg.norm(hi - lo) * 0.005
+    lo, hi = lo - d, hi + d
+    return 0.5 + (p - (lo + hi) * 0.5) / (hi - lo)
+
+
+def _vertices(rec, n_train, offset, stride, skip_last):
+    """(depth, pixel) of every vertex considered, pixel-major with depth inner."""
+    D = rec["pos"].shape[0]
+    pix = offset + np.arange(n_train, dtype=np.int64) * stride
+    depth = np.minimum(rec["cur_depth"][pix].astype(np.int64), D) - (1 if skip_last else 0)
+    k = np.arange(D)[None, :].repeat(n_train, 0)
+    p = pix[:, None].repeat(D, 1)
+    m = k < depth[:, None]
+    return k[m], p[m]
+
+
+def _record_features(rec, k, p, aabb):
+    pos = normalize_spatial(rec["pos"][k, p].astype(np.float64), aabb)
+    wo = rec["wo"][k, p].astype(np.float64)
+    dsh = (spherical_to_cartesian(wo[:, 0], wo[:, 1]) + 1.0) * 0.5  # warp_direction_for_sh, train.h:1243-1245
+    nrm = rec["normal"][k, p].astype(np.float64)
+    rough = 1.0 - np.exp(-rec["roughness"][k, p].astype(np.float64))  # warp_roughness_for_ob, train.h:1247-1249
+    return np.concatenate([pos, dsh, (nrm[:, 0] / np.pi)[:, None], (nrm[:, 1] / (2 * np.pi))[:, None],
+                           rough[:, None]], 1)
+
+
+def _local_radiance(rec, k, p):
+    L = rec["L"][k, p].astype(np.float64)
+    thp = rec["thp"][k, p].astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        L = np.where(thp > M_EPSILON, L / np.where(thp > M_EPSILON, thp, 1.0), 0.0)
+    return L * rec["mis_weight"][k, p].astype(np.float64)[:, None]
+
+
+def training_targets_f64(rec, n_train, offset, stride, aabb):
+    """DF training rows: features [m][9] and targets [m][9] = (wi square 2,
+    L 3, wiPdf, bsdfVal 3 left at 0 as in the reference), pixel-major."""
+    k, p = _vertices(rec, n_train, offset, stride, False)
+    keep = rec["delta"][k, p] == 0
+    k, p = k[keep], p[keep]
+    f = _record_features(rec, k, p, aabb)
+    d = rec["dir"][k, p].astype(np.float64)
+    wi = uniform_sphere_to_square(spherical_to_cartesian(d[:, 0], d[:, 1])).reshape(-1, 2)
+    L = _local_radiance(rec, k, p)
+    pdf = rec["wi_pdf"][k, p].astype(np.float64)
+    # the reference tests L before the bsdfVal product (train.cu:80-84)
+    ok = ~(np.isnan(f[:, :6]).any(1) | np.isnan(wi).any(1) | np.isnan(pdf) | (pdf == 0) | np.isnan(L).any(1))
+    L = L * rec["bsdf_val"][k, p].astype(np.float64)
+    t = np.concatenate([wi, L, pdf[:, None], np.zeros((len(pdf), 3))], 1)
+    return f[ok].reshape(-1, 9), t[ok].reshape(-1, 9)
+
+
+def nrc_training_targets_f64(rec, n_train, offset, stride, aabb):
+    """NRC training rows (the last recorded depth has no radiance estimate and
+    is skipped): features [m][15], targets [m][3] = L * bsdfVal / (wiPdf + eps)."""
+    k, p = _vertices(rec, n_train, offset, stride, True)
+    keep = (rec["delta"][k, p] == 0) & (rec["miss"][k, p] == 0)
+    k, p = k[keep], p[keep]
+    f = np.concatenate([_record_features(rec, k, p, aabb), rec["diffuse"][k, p].astype(np.float64),
+                        rec["specular"][k, p].astype(np.float64)], 1)
+    L = _local_radiance(rec, k, p) * rec["bsdf_val"][k, p].astype(np.float64)
+    L = L / (rec["wi_pdf"][k, p].astype(np.float64)[:, None] + M_EPSILON)
+    ok = ~(np.isnan(f[:, :6]).any(1) | np.isnan(L).any(1))
+    return f[ok].reshape(-1, 15), L[ok].reshape(-1, 3)
+
+
+def df_batches(n_samples, batch_size, batch_per_frame, min_batch=65536):
+    """Batch schedule of LearnedDistribution::trainStep (learnedDistribution.cpp:226-236)."""
+    out = []
+    for it in range(min(n_samples // batch_size + 1, batch_per_frame)):
+        b = min(n_samples - it * batch_size, batch_size)
+        b -= b % 128
+        if b < min_batch:
+            break
+        out.append((it * batch_size, b))
+    return out, min(n_samples // batch_size + 1, batch_per_frame)
+
+
+def nrc_batches(n_samples, batch_size, batch_per_frame, min_batch=65536):
+    """Batch schedule of NRC::trainStep (nrc.cpp:127-138): the slice of batch
+    `it` starts at it * (its own rounded size), as the reference indexes it."""
+    n_b = max(n_samples // batch_size + 1, batch_per_frame)
+    eff = min(n_samples // n_b, batch_size)
+    out = []
+    for it in range(n_b):
+        b = min(n_samples - it * eff, eff)
+        b -= b % 128
+        if b < min_batch:
+            break
+        out.append((it * b, b))
+    return out, n_b

@@ -64,3 +64,33 @@ def dev(torch, a):
 
 def host(t):
     return t.detach().cpu().numpy()
+
+
+def random_records(n_pixels, max_depth=5, seed=0, aabb=(-2.0, -1.0, 0.5, 3.0, 2.0, 4.0)):
+    """Synthetic GuidedPixelStateBuffer contents (host numpy, SoA [depth][pixel]):
+    positions inside the scene box, directions as spherical angles, radiance
+    log-normal, with delta / miss vertices, zero pdfs, tiny throughputs and NaN
+    radiance sprinkled in so that every skip rule of the reference fires."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array(aabb[:3], np.float32), np.array(aabb[3:], np.float32)
+    D, N = max_depth, n_pixels
+    sph = lambda: np.stack([rng.uniform(0, np.pi, (D, N)), rng.uniform(0, 2 * np.pi, (D, N))], -1).astype(np.float32)
+    rec = {
+        "cur_depth": rng.integers(0, D + 1, N).astype(np.uint32),
+        "pos": (lo + (hi - lo) * rng.random((D, N, 3))).astype(np.float32),
+        "dir": sph(), "wo": sph(), "normal": sph(),
+        "roughness": rng.random((D, N)).astype(np.float32),
+        "L": np.exp(rng.normal(0, 1, (D, N, 3))).astype(np.float32),
+        "thp": rng.uniform(0.05, 1.0, (D, N, 3)).astype(np.float32),
+        "bsdf_val": rng.uniform(0.0, 1.0, (D, N, 3)).astype(np.float32),
+        "diffuse": rng.random((D, N, 3)).astype(np.float32),
+        "specular": rng.random((D, N, 3)).astype(np.float32),
+        "wi_pdf": rng.uniform(0.05, 2.0, (D, N)).astype(np.float32),
+        "mis_weight": rng.uniform(0.0, 1.0, (D, N)).astype(np.float32),
+        "delta": (rng.random((D, N)) < 0.1).astype(np.uint8),
+        "miss": (rng.random((D, N)) < 0.1).astype(np.uint8),
+    }
+    rec["wi_pdf"][rng.random((D, N)) < 0.03] = 0.0
+    rec["thp"][rng.random((D, N, 3)) < 0.03] = 1e-6
+    rec["L"][rng.random((D, N, 3)) < 0.01] = np.nan
+    return rec, np.array(aabb, np.float32)

@@ -1,0 +1,130 @@
+"""GPU tests of training from recorded path vertices: row generation against
+the FP64 oracle restatement of generate_training_data_AR / _nrc, and the
+per-frame training passes against the same steps run by hand."""
+import numpy as np
+import pytest
+
+import dfguide as G
+import oracle as O
+from gpu_util import random_records, dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_records(torch, rec):
+    out = {}
+    for k, v in rec.items():
+        if k == "cur_depth":
+            out[k] = torch.from_numpy(v.view(np.int32)).cuda()
+        else:
+            out[k] = torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    return out
+
+
+@pytest.mark.parametrize("n_pixels,offset,stride", [(3000, 0, 1), (3001, 2, 3), (64, 0, 1)])
+def test_training_rows_match_oracle(torch_cuda, n_pixels, offset, stride):
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=True, log2_hashmap_size=12))
+    rec, aabb = random_records(n_pixels, seed=n_pixels + offset)
+    drec, daabb = _dev_records(torch, rec), dev(torch, aabb)
+    n_train = (n_pixels - offset + stride - 1) // stride
+    fo, to = O.training_targets_f64(rec, n_train, offset, stride, aabb)
+    cap = n_train * 5
+    feat = torch.zeros((cap, 9), device="cuda")
+    tgt = torch.zeros((cap, 9), device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    f.training_targets(drec, n_train, daabb, feat, tgt, cnt, offset=offset, stride=stride)
+    torch.cuda.synchronize()
+    m = int(host(cnt)[0])
+    assert m == len(fo), (m, len(fo))
+    assert np.allclose(host(feat)[:m], fo, rtol=1e-5, atol=2e-6)
+    assert np.allclose(host(tgt)[:m], to, rtol=1e-5, atol=2e-6)
+    # NRC rows
+    fno, tno = O.nrc_training_targets_f64(rec, n_train, offset, stride, aabb)
+    featn = torch.zeros((cap, 15), device="cuda")
+    tgtn = torch.zeros((cap, 3), device="cuda")
+    f.nrc_training_targets(drec, n_train, daabb, featn, tgtn, cnt, offset=offset, stride=stride)
+    torch.cuda.synchronize()
+    m = int(host(cnt)[0])
+    assert m == len(fno), (m, len(fno))
+    assert np.allclose(host(featn)[:m], fno, rtol=1e-5, atol=2e-6)
+    assert np.allclose(host(tgtn)[:m], tno, rtol=1e-5, atol=1e-6)
+    f.close()
+
+
+def test_training_rows_capacity_and_errors(torch_cuda):
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False, log2_hashmap_size=12))
+    rec, aabb = random_records(500, seed=5)
+    drec, daabb = _dev_records(torch, rec), dev(torch, aabb)
+    fo, to = O.training_targets_f64(rec, 500, 0, 1, aabb)
+    cap = len(fo) // 2
+    feat = torch.full((cap, 9), -7.0, device="cuda")
+    tgt = torch.full((cap, 9), -7.0, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    f.training_targets(drec, 500, daabb, feat, tgt, cnt)
+    torch.cuda.synchronize()
+    assert int(host(cnt)[0]) == len(fo)            # the count covers every valid row
+    assert np.allclose(host(feat), fo[:cap], rtol=1e-5, atol=2e-6)
+    with pytest.raises(G.DfgError):                # training pixels beyond the buffer
+        f.training_targets(drec, 501, daabb, feat, tgt, cnt)
+    with pytest.raises(G.DfgError):
+        f.training_targets(drec, 200, daabb, feat, tgt, cnt, offset=10, stride=3)
+    f.training_targets(drec, 0, daabb, feat, tgt, cnt)
+    torch.cuda.synchronize()
+    assert int(host(cnt)[0]) == 0
+    f.close()
+
+
+def test_train_records_runs_the_reference_schedule(torch_cuda):
+    """dfg_train_records = row generation + the learnedDistribution.cpp batch
+    loop; the same slices trained by hand on an identical field give the same
+    weights (up to the fp32 order of the grid-gradient sums)."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg)
+    rec, aabb = random_records(100000, seed=9)
+    drec, daabb = _dev_records(torch, rec), dev(torch, aabb)
+    bs, bpf = 65536, 5
+    rows, steps, loss = fa.train_records(drec, 100000, daabb, batch_size=bs, batch_per_frame=bpf)
+    fo, to = O.training_targets_f64(rec, 100000, 0, 1, aabb)
+    sched, planned = O.df_batches(len(fo), bs, bpf)
+    assert rows == len(fo) and steps == len(sched) and steps >= 2
+    feat = torch.zeros((100000 * 5, 9), device="cuda")
+    tgt = torch.zeros((100000 * 5, 9), device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    fb.training_targets(drec, 100000, daabb, feat, tgt, cnt)
+    losses = torch.zeros(len(sched), device="cuda")
+    for i, (r0, b) in enumerate(sched):
+        fb.train_step(feat[r0:r0 + b].contiguous(), tgt[r0:r0 + b].contiguous(), losses[i:i + 1])
+    torch.cuda.synchronize()
+    assert abs(loss - float(host(losses).sum()) / planned) <= 1e-4 * abs(loss) + 1e-6
+    # grid-gradient sums run in a scheduling-dependent fp32 order, and Adam
+    # turns a near-zero gradient of either sign into a full step of +-lr: hold
+    # almost every weight tight and every weight within 2 lr per step
+    lr = cfg["nn"]["optimizer"]["nested"]["learning_rate"]
+    for k in (0, 1):
+        a, b = fa.get_params(k, G.DFG_PARAMS_MASTER), fb.get_params(k, G.DFG_PARAMS_MASTER)
+        close = np.isclose(a, b, rtol=1e-3, atol=1e-5)
+        assert close.mean() > 0.999, "net %d: %.5f of the weights agree" % (k, close.mean())
+        assert np.abs(a - b).max() <= 2 * lr * steps + 1e-6
+    fa.close()
+    fb.close()
+
+
+def test_nrc_train_records_schedule(torch_cuda):
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=True, log2_hashmap_size=12))
+    rec, aabb = random_records(150000, seed=11)
+    drec, daabb = _dev_records(torch, rec), dev(torch, aabb)
+    w0 = f.get_params(2, G.DFG_PARAMS_MASTER)
+    rows, steps, loss = f.nrc_train_records(drec, 150000, daabb, batch_size=131072, batch_per_frame=2)
+    fo, _ = O.nrc_training_targets_f64(rec, 150000, 0, 1, aabb)
+    sched, planned = O.nrc_batches(len(fo), 131072, 2)
+    assert rows == len(fo) and steps == len(sched) and steps >= 1
+    assert np.isfinite(loss) and loss > 0 and not np.array_equal(w0, f.get_params(2, G.DFG_PARAMS_MASTER))
+    # a schedule whose slices fall under the 65,536-row minimum trains nothing
+    rows, steps, loss = f.nrc_train_records(drec, 150000, daabb, batch_size=65536, batch_per_frame=4)
+    sched, _ = O.nrc_batches(len(fo), 65536, 4)
+    assert steps == len(sched) == 0 and loss == 0.0
+    f.close()

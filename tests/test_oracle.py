@@ -476,3 +476,65 @@ def test_query_composes_encode_mlp_head():
     x1, p1, _, _ = O.head(lp, 1, u32[:, 1], u32[:, 3])
     assert np.allclose(q[:, 0], x0, rtol=0, atol=1e-6) and np.allclose(q[:, 1], x1, rtol=0, atol=1e-6)
     assert np.allclose(q[:, 2], p0, rtol=1e-6) and np.allclose(q[:, 3], p1, rtol=1e-6)
+
+
+# ----------------------------------------------------------------------------
+# training records -> rows, batch schedules
+# ----------------------------------------------------------------------------
+def test_sphere_square_maps_round_trip():
+    rng = np.random.default_rng(50)
+    u = rng.random((1000, 2)) * 0.998 + 0.001
+    v = O.square_to_uniform_sphere(u)
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0)
+    assert np.allclose(O.uniform_sphere_to_square(v), u, atol=1e-12)
+    assert np.allclose(O.square_to_uniform_sphere(np.array([[0.0, 0.3], [1.0, 0.7], [0.5, 0.0], [0.5, 0.25]])),
+                       [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0]], atol=1e-12)
+    # sphericalToCartesian: theta from +z, phi from +x
+    assert np.allclose(O.spherical_to_cartesian(np.array([0.0, np.pi / 2, np.pi / 2]), np.array([1.0, 0.0, np.pi / 2])),
+                       [[0, 0, 1], [1, 0, 0], [0, 1, 0]], atol=1e-12)
+
+
+def test_spatial_normalisation_closed_form():
+    aabb = np.array([-2.0, -1.0, 0.5, 3.0, 2.0, 4.0])
+    lo, hi = aabb[:3], aabb[3:]
+    assert np.allclose(O.normalize_spatial((lo + hi) / 2, aabb), 0.5)
+    d = np.linalg.norm(hi - lo) * 0.005
+    assert np.allclose(O.normalize_spatial(lo, aabb), 0.5 - (hi - lo) / 2 / (hi - lo + 2 * d))
+    assert np.allclose(O.normalize_spatial(hi, aabb), 0.5 + (hi - lo) / 2 / (hi - lo + 2 * d))
+
+
+def test_training_rows_follow_the_skip_and_radiance_rules():
+    from gpu_util import random_records
+    rec, aabb = random_records(40, seed=51)
+    f, t = O.training_targets_f64(rec, 40, 0, 1, aabb)
+    # count by the rules, written out directly
+    expect, rows = 0, []
+    for p in range(40):
+        for k in range(min(int(rec["cur_depth"][p]), 5)):
+            if rec["delta"][k, p] or rec["wi_pdf"][k, p] == 0:
+                continue
+            thp, L = rec["thp"][k, p].astype(float), rec["L"][k, p].astype(float)
+            loc = np.where(thp > 1e-5, L / thp, 0.0) * float(rec["mis_weight"][k, p])
+            if np.isnan(loc).any():
+                continue
+            expect += 1
+            rows.append(loc * rec["bsdf_val"][k, p])
+    assert f.shape == (expect, 9) and t.shape == (expect, 9)
+    assert np.allclose(t[:, 2:5], np.array(rows)) and np.all(t[:, 6:] == 0)
+    assert np.all((t[:, :2] >= 0) & (t[:, :2] <= 1)) and np.all((f[:, :3] > 0) & (f[:, :3] < 1))
+    fn, tn = O.nrc_training_targets_f64(rec, 40, 0, 1, aabb)
+    assert fn.shape[1] == 15 and tn.shape[1] == 3 and len(fn) < len(f) + 40
+
+
+def test_batch_schedules():
+    sched, planned = O.df_batches(200000, 524288, 5)
+    assert planned == 1 and sched == [(0, 199936)]
+    sched, planned = O.df_batches(600000, 524288, 5)
+    assert planned == 2 and sched == [(0, 524288), (524288, 75648)]
+    sched, planned = O.df_batches(530000, 524288, 5)
+    assert planned == 2 and sched == [(0, 524288)]            # the 5,712-row tail is under 65,536
+    sched, planned = O.df_batches(50000, 524288, 5)
+    assert planned == 1 and sched == []
+    sched, planned = O.nrc_batches(1000000, 1048576, 4)
+    assert planned == 4 and [b for _, b in sched] == [249984] * 4
+    assert [o for o, _ in sched] == [0, 249984, 499968, 749952]

@@ -32,11 +32,12 @@ struct Row {
 };
 
 // One record (depth k, pixel p) as a training row; false when the reference
-// skips it.  mode 0: DF row (feat 9, target 9); 1: NRC row (feat 15, target 3).
+// skips it.  mode 0: DF row (feat 9, target 9); 1: NRC row (feat 15, target 3);
+// 2: NRC inference input (feat 15; the NRC row's validity).
 __device__ bool record_row(const dfg_records& r, int k, int64_t p, const float* __restrict__ aabb, int mode, Row& o) {
     const int64_t i = (int64_t)k * r.n_pixels + p;
     if (r.delta[i]) return false;
-    if (mode == 1 && r.miss[i]) return false;
+    if (mode != 0 && r.miss[i]) return false;
     // normalizeSpatialCoord (train.h:272-276): the box inflated by 0.5 % of its diagonal
     float d[3];
     for (int c = 0; c < 3; ++c) d[c] = aabb[3 + c] - aabb[c];
@@ -92,7 +93,7 @@ __device__ bool record_row(const dfg_records& r, int k, int64_t p, const float* 
 __device__ __forceinline__ int pixel_depth(const dfg_records& r, int64_t p, int mode) {
     int d = (int)r.cur_depth[p];
     if (d > r.max_depth) d = r.max_depth;
-    // the NRC pass has no radiance estimate for the last recorded vertex
+    // the NRC training rows have no radiance estimate for the last recorded vertex
     return mode == 1 ? d - 1 : d;
 }
 
@@ -149,12 +150,15 @@ __global__ __launch_bounds__(1024) void records_scan_kernel(int64_t* __restrict_
     if (t == 1023) *count = part[1023];
 }
 
+// mode 2 writes the vertex (depth, pixel) of each row to vidx instead of a
+// target, then lowers the pixel's cur_depth by one (read before, by this
+// thread alone)
 __global__ __launch_bounds__(256) void records_write_kernel(const dfg_records r, int64_t n_train, uint32_t offset,
                                                             uint32_t stride, const float* __restrict__ aabb, int mode,
                                                             const uint32_t* __restrict__ local,
                                                             const int64_t* __restrict__ block_off,
                                                             float* __restrict__ feat, float* __restrict__ tgt,
-                                                            int64_t capacity) {
+                                                            int2* __restrict__ vidx, int64_t capacity) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= n_train) return;
     const int64_t p = (int64_t)offset + t * (int64_t)stride;
@@ -166,23 +170,78 @@ __global__ __launch_bounds__(256) void records_write_kernel(const dfg_records r,
         if (!record_row(r, k, p, aabb, mode, o)) continue;
         if (at < capacity) {
             for (int c = 0; c < nf; ++c) feat[at * nf + c] = o.feat[c];
-            for (int c = 0; c < nt; ++c) tgt[at * nt + c] = o.tgt[c];
+            if (mode == 2) vidx[at] = make_int2(k, (int)p);
+            else
+                for (int c = 0; c < nt; ++c) tgt[at * nt + c] = o.tgt[c];
         }
         ++at;
+    }
+    // the reference's unsigned decrement would wrap a pixel with no vertex to
+    // 2^32 - 1; it stays at 0 here
+    if (mode == 2 && r.cur_depth[p] > 0u) r.cur_depth[p] = r.cur_depth[p] - 1u;
+}
+
+// GuidedPixelStateBuffer::isValidRecord (workqueue.h:113-115)
+__device__ __forceinline__ bool valid_record(const dfg_records& r, int64_t p, int d) {
+    return d >= 0 && d < r.max_depth && (uint32_t)d < r.cur_depth[p];
+}
+
+// replaceIndirectRadianceSkipDelta (workqueue.h:131-138) for each inferred
+// vertex k: the first non-delta vertex at or below k - 1 takes
+// thp_{k-1} * (R_k + Le_{k-1})
+__global__ void nrc_apply_kernel(const dfg_records r, const int2* __restrict__ vidx, const float* __restrict__ rgb,
+                                 const int64_t* __restrict__ count, int64_t capacity) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t m = *count < capacity ? *count : capacity;
+    if (i >= m) return;
+    const int k = vidx[i].x;
+    const int64_t p = vidx[i].y;
+    const int init = k - 1;
+    int d = init;
+    while (d >= 0 && r.delta[(int64_t)d * r.n_pixels + p]) --d;
+    if (!valid_record(r, p, d)) return;
+    const int64_t a = (int64_t)init * r.n_pixels + p, b = (int64_t)d * r.n_pixels + p;
+    for (int c = 0; c < 3; ++c) {
+        const float le = r.Le ? r.Le[3 * a + c] : 0.0f;
+        r.L[3 * b + c] = r.thp[3 * a + c] * (rgb[3 * i + c] + le);
+    }
+}
+
+// divideRadiance (workqueue.h:117-123): L_k /= R_k where R_k > eps
+__global__ void nrc_divide_kernel(const dfg_records r, const int2* __restrict__ vidx, const float* __restrict__ rgb,
+                                  const int64_t* __restrict__ count, int64_t capacity) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t m = *count < capacity ? *count : capacity;
+    if (i >= m) return;
+    const int k = vidx[i].x;
+    const int64_t p = vidx[i].y;
+    if (!valid_record(r, p, k)) return;
+    const int64_t a = (int64_t)k * r.n_pixels + p;
+    for (int c = 0; c < 3; ++c) {
+        const float R = rgb[3 * i + c];
+        if (R > kEps) r.L[3 * a + c] = r.L[3 * a + c] / R;
     }
 }
 
 }  // namespace
 
 void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t offset, uint32_t stride,
-                             const float* aabb, int mode, float* feat, float* tgt, int64_t capacity,
+                             const float* aabb, int mode, float* feat, float* tgt, int2* vidx, int64_t capacity,
                              uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s) {
     const unsigned blocks = (unsigned)((n_train + 255) / 256);
     hipLaunchKernelGGL(records_count_kernel, dim3(blocks), dim3(256), 0, s, r, n_train, offset, stride, aabb, mode,
                        local, block_tot);
     hipLaunchKernelGGL(records_scan_kernel, dim3(1), dim3(1024), 0, s, block_tot, (int64_t)blocks, count);
     hipLaunchKernelGGL(records_write_kernel, dim3(blocks), dim3(256), 0, s, r, n_train, offset, stride, aabb, mode,
-                       local, block_tot, feat, tgt, capacity);
+                       local, block_tot, feat, tgt, vidx, capacity);
+}
+
+void launch_nrc_substitute(const dfg_records& r, const int2* vidx, const float* rgb, const int64_t* count,
+                           int64_t capacity, hipStream_t s) {
+    if (capacity <= 0) return;
+    const dim3 grid((unsigned)((capacity + 255) / 256));
+    hipLaunchKernelGGL(nrc_apply_kernel, grid, dim3(256), 0, s, r, vidx, rgb, count, capacity);
+    hipLaunchKernelGGL(nrc_divide_kernel, grid, dim3(256), 0, s, r, vidx, rgb, count, capacity);
 }
 
 }  // namespace dfg

@@ -11,8 +11,10 @@ using namespace dfg;
 
 namespace dfg {
 void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t offset, uint32_t stride,
-                             const float* aabb, int mode, float* feat, float* tgt, int64_t capacity,
+                             const float* aabb, int mode, float* feat, float* tgt, int2* vidx, int64_t capacity,
                              uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s);
+void launch_nrc_substitute(const dfg_records& r, const int2* vidx, const float* rgb, const int64_t* count,
+                           int64_t capacity, hipStream_t s);
 }
 
 namespace {
@@ -32,7 +34,7 @@ void check_records(const dfg_records* r, int64_t n_train, uint32_t offset, uint3
     for (const void* p : need)
         if (!p) throw Error(DFG_ERR_INVALID_ARG, "null record field");
     if (mode == 0 && !r->dir) throw Error(DFG_ERR_INVALID_ARG, "null record field dir");
-    if (mode == 1 && (!r->diffuse || !r->specular || !r->miss)) throw Error(DFG_ERR_INVALID_ARG, "null NRC record field");
+    if (mode != 0 && (!r->diffuse || !r->specular || !r->miss)) throw Error(DFG_ERR_INVALID_ARG, "null NRC record field");
 }
 
 void targets(dfg_ctx* c, hipStream_t st, const dfg_records* r, int64_t n_train, uint32_t offset, uint32_t stride,
@@ -44,7 +46,7 @@ void targets(dfg_ctx* c, hipStream_t st, const dfg_records* r, int64_t n_train, 
     const int64_t blocks = (n_train + 255) / 256;
     c->rec_local.ensure((size_t)n_train);
     c->rec_blk.ensure((size_t)blocks);
-    launch_training_targets(*r, n_train, offset, stride, aabb, mode, feat, tgt, capacity, c->rec_local.p,
+    launch_training_targets(*r, n_train, offset, stride, aabb, mode, feat, tgt, nullptr, capacity, c->rec_local.p,
                             c->rec_blk.p, count, st);
     check_launch();
 }
@@ -159,6 +161,44 @@ int dfg_nrc_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, in
     if (n_rows) *n_rows = rows;
     if (n_steps) *n_steps = run;
     if (loss) *loss = sum / (float)planned;
+    API_END(ctx)
+}
+
+int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, int64_t n_train, uint32_t pixel_offset,
+                                uint32_t pixel_stride, const float* aabb, int64_t* n_inferred) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!ctx->net[2].present) throw Error(DFG_ERR_INVALID_ARG, "no NRC net configured");
+    check_records(rec, n_train, pixel_offset, pixel_stride, aabb, 2);
+    hipStream_t st = (hipStream_t)stream;
+    int64_t m = 0;
+    if (n_train > 0) {
+        const int64_t cap = n_train * rec->max_depth;
+        const int64_t blocks = (n_train + 255) / 256;
+        ctx->rec_local.ensure((size_t)n_train);
+        ctx->rec_blk.ensure((size_t)blocks);
+        ctx->rec_feat.ensure((size_t)cap * 15);
+        ctx->rec_tgt.ensure((size_t)cap * 3);
+        ctx->rec_vidx.ensure((size_t)cap);
+        ctx->rec_count.ensure(1);
+        // 1. NRC inputs of the vertices below cur_depth; cur_depth lowered by one
+        launch_training_targets(*rec, n_train, pixel_offset, pixel_stride, aabb, 2, ctx->rec_feat.p, nullptr,
+                                ctx->rec_vidx.p, cap, ctx->rec_local.p, ctx->rec_blk.p, ctx->rec_count.p, st);
+        check_launch();
+        HIPX(hipMemcpyAsync(&m, ctx->rec_count.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        m = std::min(m, cap);
+        // 2. the radiance cache at those vertices
+        if (m > 0) {
+            const int rc = dfg_nrc_infer(ctx, st, m, ctx->rec_feat.p, 15, ctx->rec_tgt.p);
+            if (rc != DFG_OK) throw Error(rc, ctx->err);
+            // 3. replaceIndirectRadianceSkipDelta, then divideRadiance
+            launch_nrc_substitute(*rec, ctx->rec_vidx.p, ctx->rec_tgt.p, ctx->rec_count.p, m, st);
+            check_launch();
+        }
+        HIPX(hipStreamSynchronize(st));
+    }
+    if (n_inferred) *n_inferred = m;
     API_END(ctx)
 }
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
     "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
-    "dfg_nrc_train_records", "dfg_prepare_features",
+    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -86,7 +86,7 @@ class DfgServerStats(ctypes.Structure):
 
 
 RECORD_FIELDS = ("cur_depth", "pos", "dir", "wo", "normal", "roughness", "L", "thp", "bsdf_val", "diffuse",
-                 "specular", "wi_pdf", "mis_weight", "delta", "miss")
+                 "specular", "wi_pdf", "mis_weight", "delta", "miss", "Le")
 
 
 class DfgRecords(ctypes.Structure):
@@ -145,6 +145,8 @@ def load_library(path=LIB_PATH):
         "dfg_nrc_train_records": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P, I64, I,
                                    ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_float)], I),
+        "dfg_nrc_substitute_radiance": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P,
+                                         ctypes.POINTER(ctypes.c_int64)], I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
         "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
@@ -388,6 +390,15 @@ class GuidingField:
         self._c(fn(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset, stride, _ptr(aabb), batch_size,
                    batch_per_frame, ctypes.byref(rows), ctypes.byref(steps), ctypes.byref(loss)))
         return rows.value, steps.value, loss.value
+
+    def nrc_substitute_radiance(self, rec, n_train, aabb, offset=0, stride=1, stream=None):
+        """Replaces the records' indirect radiance by the radiance cache (writes
+        rec["L"] and rec["cur_depth"]); returns the number of NRC queries."""
+        r = records_struct(rec)
+        m = ctypes.c_int64(0)
+        self._c(self.lib.dfg_nrc_substitute_radiance(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset,
+                                                     stride, _ptr(aabb), ctypes.byref(m)))
+        return m.value
 
     def train_records(self, rec, n_train, aabb, batch_size=524288, batch_per_frame=5, offset=0, stride=1,
                       stream=None):

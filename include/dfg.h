@@ -171,13 +171,15 @@ int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float*
 typedef struct dfg_records {
     int max_depth;             /* depth slices (MAX_TRAIN_DEPTH, parameters.h:24)        */
     int64_t n_pixels;          /* pixels per slice                                      */
-    const uint32_t* cur_depth; /* [n_pixels] vertices recorded per pixel                 */
+    uint32_t* cur_depth;       /* [n_pixels] vertices recorded per pixel (written only by
+                                  dfg_nrc_substitute_radiance)                          */
     const float* pos;          /* [3] world-space vertex                                */
     const float* dir;          /* [2] spherical (theta, phi) of the scattered direction */
     const float* wo;           /* [2] spherical outgoing direction                      */
     const float* normal;       /* [2] spherical shading normal                          */
     const float* roughness;    /* [1]                                                   */
-    const float* L;            /* [3] radiance gathered after the vertex                */
+    float* L;                  /* [3] radiance gathered after the vertex (written only by
+                                  dfg_nrc_substitute_radiance)                          */
     const float* thp;          /* [3] throughput up to the vertex                       */
     const float* bsdf_val;     /* [3]                                                   */
     const float* diffuse;      /* [3]                                                   */
@@ -186,6 +188,7 @@ typedef struct dfg_records {
     const float* mis_weight;   /* [1]                                                   */
     const uint8_t* delta;      /* [1] sampled from a delta lobe                         */
     const uint8_t* miss;       /* [1] the path left the scene there                     */
+    const float* Le;           /* [3] emission seen at the vertex (may be NULL: zero)   */
 } dfg_records;
 
 /* DF training rows (generate_training_data_AR, train.cu:6-96).  For the
@@ -223,6 +226,22 @@ int dfg_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_
 int dfg_nrc_train_records(dfg_ctx* ctx, void* stream, const dfg_records* rec, int64_t n_train,
                           uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb, int64_t batch_size,
                           int batch_per_frame, int64_t* n_rows, int* n_steps, float* loss);
+
+/* Radiance-cache substitution of the training records (the reference's
+ * OPTIMIZE_ON_NRC = 4 path: generate_training_data_nrc_with_inference,
+ * train.cu:156-213, NRC::inferenceStepFromTrainingData, nrc.cpp:183-216,
+ * apply_NRCoutput_to_training_data and divide_training_data_by_NRCoutput,
+ * train.cu:265-308).  For every recorded vertex k < cur_depth that is neither
+ * delta nor miss, the NRC radiance R_k is inferred; then cur_depth drops by one
+ * per pixel; then, for each such k, the first non-delta vertex j <= k - 1 gets
+ * L_j = thp_{k-1} * (R_k + Le_{k-1}); then L_k /= R_k per channel where
+ * R_k > 1e-5 (only for vertices still below cur_depth).  Writes rec->L and
+ * rec->cur_depth.  Run it after the NRC training pass and before the DF one,
+ * as the reference's trainStep does (integrator.cpp:1202-1237).  Synchronises
+ * `stream`; *n_inferred (host, may be NULL) gets the number of NRC queries. */
+int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, int64_t n_train,
+                                uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb,
+                                int64_t* n_inferred);
 
 /* Feature preparation (generate_inference_data_AR, train.cu:334-362):
  * shading points -> DF feature rows [n][9].

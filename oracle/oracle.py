@@ -500,3 +500,44 @@ def nrc_batches(n_samples, batch_size, batch_per_frame, min_batch=65536):
             break
         out.append((it * b, b))
     return out, n_b
+
+
+def nrc_substitute_f64(rec, n_train, offset, stride, aabb, nrc_rgb):
+    """The records after the radiance-cache substitution (OPTIMIZE_ON_NRC = 4:
+    train.cu:156-213, 265-308; workqueue.h:113-138), given the cache's output
+    nrc_rgb [m][3] for the inference inputs in the order this returns them.
+    Returns (inputs [m][15], vertices [m][2] (depth, pixel), new L, new cur_depth)."""
+    D = rec["pos"].shape[0]
+    k, p = _vertices(rec, n_train, offset, stride, False)
+    keep = (rec["delta"][k, p] == 0) & (rec["miss"][k, p] == 0)
+    k, p = k[keep], p[keep]
+    f = np.concatenate([_record_features(rec, k, p, aabb), rec["diffuse"][k, p].astype(np.float64),
+                        rec["specular"][k, p].astype(np.float64)], 1)
+    L = _local_radiance(rec, k, p) * rec["bsdf_val"][k, p].astype(np.float64)
+    L = L / (rec["wi_pdf"][k, p].astype(np.float64)[:, None] + M_EPSILON)
+    ok = ~(np.isnan(f[:, :6]).any(1) | np.isnan(L).any(1))
+    k, p, f = k[ok], p[ok], f[ok]
+    depth = rec["cur_depth"].astype(np.int64).copy()
+    pix = offset + np.arange(n_train, dtype=np.int64) * stride
+    depth[pix] = np.maximum(depth[pix] - 1, 0)
+    newL = rec["L"].astype(np.float64).copy()
+    Le = rec.get("Le")
+    valid = lambda pp, d: 0 <= d < D and d < depth[pp]
+    if nrc_rgb is None:
+        return f, np.stack([k, p], 1), newL, depth
+    for i in range(len(k)):   # replaceIndirectRadianceSkipDelta(p, k - 1, R)
+        kk, pp = int(k[i]), int(p[i])
+        d = kk - 1
+        while d >= 0 and rec["delta"][d, pp]:
+            d -= 1
+        if not valid(pp, d):
+            continue
+        le = Le[kk - 1, pp].astype(np.float64) if Le is not None else 0.0
+        newL[d, pp] = rec["thp"][kk - 1, pp].astype(np.float64) * (nrc_rgb[i] + le)
+    for i in range(len(k)):   # divideRadiance(p, k, R)
+        kk, pp = int(k[i]), int(p[i])
+        if not valid(pp, kk):
+            continue
+        R = nrc_rgb[i]
+        newL[kk, pp] = np.where(R > M_EPSILON, newL[kk, pp] / np.where(R > M_EPSILON, R, 1.0), newL[kk, pp])
+    return f, np.stack([k, p], 1), newL, depth

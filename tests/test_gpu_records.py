@@ -128,3 +128,46 @@ def test_nrc_train_records_schedule(torch_cuda):
     sched, _ = O.nrc_batches(len(fo), 65536, 4)
     assert steps == len(sched) == 0 and loss == 0.0
     f.close()
+
+
+def test_nrc_substitution_matches_oracle(torch_cuda):
+    """Radiance-cache substitution of the records (the reference's
+    OPTIMIZE_ON_NRC = 4 path): new cur_depth exact; new L against the oracle's
+    apply / divide rules fed with the cache's output for the same inputs
+    (computed from the oracle's inputs, so within the cache's fp16 input
+    rounding of the library's own)."""
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=True, log2_hashmap_size=12))
+    rec, aabb = random_records(2000, seed=21)
+    rec["Le"] = np.random.default_rng(22).uniform(0, 0.5, rec["thp"].shape).astype(np.float32)
+    rec["cur_depth"] = np.minimum(rec["cur_depth"], 4)
+    rec["cur_depth"][:7] = 0                       # pixels without vertices stay at 0
+    drec, daabb = _dev_records(torch, rec), dev(torch, aabb)
+    fo, vk, _, _ = O.nrc_substitute_f64(rec, 1999, 1, 1, aabb, None)
+    # the library's own cache inputs for those vertices: the NRC training rows
+    # of the records with one more vertex per pixel are the same set, in order
+    plus = dict(drec)
+    plus["cur_depth"] = dev(torch, (rec["cur_depth"] + 1).view(np.int32))
+    featn = torch.zeros((1999 * 5, 15), device="cuda")
+    tgtn = torch.zeros((1999 * 5, 3), device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    f.nrc_training_targets(plus, 1999, daabb, featn, tgtn, cnt, offset=1, stride=1)
+    torch.cuda.synchronize()
+    assert int(host(cnt)[0]) == len(vk)
+    assert np.allclose(host(featn)[:len(vk)], fo, rtol=1e-5, atol=2e-6)
+    rgb = torch.zeros((len(vk), 3), device="cuda")
+    f.nrc_infer(featn[:len(vk)].contiguous(), rgb)
+    m = f.nrc_substitute_radiance(drec, 1999, daabb, offset=1, stride=1)
+    torch.cuda.synchronize()
+    assert m == len(vk)
+    _, _, L_o, depth_o = O.nrc_substitute_f64(rec, 1999, 1, 1, aabb, host(rgb).astype(np.float64))
+    depth_g = host(drec["cur_depth"]).view(np.uint32)
+    assert np.array_equal(depth_g.astype(np.int64), depth_o)
+    assert depth_g[0] == rec["cur_depth"][0]        # outside the training pixels: untouched
+    L_g = host(drec["L"])
+    L0 = rec["L"].astype(np.float64)
+    same = (L_o == L0) | (np.isnan(L_o) & np.isnan(L0))
+    assert np.array_equal(L_g[same], rec["L"][same], equal_nan=True)
+    assert np.allclose(L_g[~same], L_o[~same], rtol=1e-5, atol=1e-7, equal_nan=True)
+    assert (~same).sum() > 100
+    f.close()

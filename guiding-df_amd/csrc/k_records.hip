@@ -225,6 +225,67 @@ __global__ void nrc_divide_kernel(const dfg_records r, const int2* __restrict__ 
 
 }  // namespace
 
+namespace {
+
+// AutoRegressive::sample / pdf (AR.h:25-48) with squareToUniformSphere
+// (math_utils.h:226-231)
+__global__ void directions_kernel(int64_t n, const float* __restrict__ wi, float* __restrict__ dir,
+                                  float* __restrict__ pdf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 w = *(const float4*)(wi + 4 * i);
+    if (dir) {
+        const float z = 1.0f - 2.0f * w.x;
+        const float r = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+        const float ph = kTwoPi * w.y;
+        dir[3 * i + 0] = r * cosf(ph);
+        dir[3 * i + 1] = r * sinf(ph);
+        dir[3 * i + 2] = z;
+    }
+    if (pdf) pdf[i] = fabsf(w.z) * fabsf(w.w) * DFG_INV_4PI;
+}
+
+// the mixture pdf and throughput update of handleIntersections (integrator.cpp:203-299)
+__global__ void scatter_kernel(int64_t n, float f, const float* __restrict__ wi, const uint8_t* __restrict__ guided,
+                               const uint8_t* __restrict__ mix, const float* __restrict__ bsdf_pdf,
+                               const float* __restrict__ bsdf_val, const float* __restrict__ cos_theta,
+                               float* __restrict__ thp, float* __restrict__ spdf, float* __restrict__ mis,
+                               uint8_t* __restrict__ alive) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float pg = fabsf(wi[4 * i + 2]) * fabsf(wi[4 * i + 3]) * DFG_INV_4PI;
+    const float pb = bsdf_pdf[i];
+    float p;
+    if (guided && guided[i]) p = (1.0f - f) * pg + f * pb;
+    else if (mix && mix[i]) p = pb * f + (1.0f - f) * pg;
+    else p = pb;
+    const float b0 = bsdf_val[3 * i], b1 = bsdf_val[3 * i + 1], b2 = bsdf_val[3 * i + 2];
+    const bool ok = p != 0.0f && (b0 != 0.0f || b1 != 0.0f || b2 != 0.0f);
+    if (spdf) spdf[i] = p;
+    if (alive) alive[i] = ok ? 1 : 0;
+    if (!ok) return;
+    const float c = fabsf(cos_theta[i]);
+    thp[3 * i + 0] = thp[3 * i + 0] * b0 * c / (p + kEps);
+    thp[3 * i + 1] = thp[3 * i + 1] * b1 * c / (p + kEps);
+    thp[3 * i + 2] = thp[3 * i + 2] * b2 * c / (p + kEps);
+    if (mis) mis[i] = (1.0f - f) * pg / p;
+}
+
+}  // namespace
+
+void launch_directions(int64_t n, const float* wi, float* dir, float* pdf, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(directions_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, wi, dir, pdf);
+}
+
+void launch_scatter(int64_t n, float f, const float* wi, const uint8_t* guided, const uint8_t* mix,
+                    const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta, float* thp, float* spdf,
+                    float* mis, uint8_t* alive, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, f, wi, guided, mix,
+                       bsdf_pdf, bsdf_val, cos_theta, thp, spdf, mis, alive);
+}
+
 void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t offset, uint32_t stride,
                              const float* aabb, int mode, float* feat, float* tgt, int2* vidx, int64_t capacity,
                              uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s) {

@@ -15,6 +15,10 @@ void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t off
                              uint32_t* local, int64_t* block_tot, int64_t* count, hipStream_t s);
 void launch_nrc_substitute(const dfg_records& r, const int2* vidx, const float* rgb, const int64_t* count,
                            int64_t capacity, hipStream_t s);
+void launch_directions(int64_t n, const float* wi, float* dir, float* pdf, hipStream_t s);
+void launch_scatter(int64_t n, float f, const float* wi, const uint8_t* guided, const uint8_t* mix,
+                    const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta, float* thp, float* spdf,
+                    float* mis, uint8_t* alive, hipStream_t s);
 }
 
 namespace {
@@ -200,6 +204,26 @@ int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, in
     }
     if (n_inferred) *n_inferred = m;
     API_END(ctx)
+}
+
+int dfg_directions(void* stream, int64_t n, const float* wi, float* dir, float* pdf) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (n == 0) return DFG_OK;
+    if (!wi) return DFG_ERR_INVALID_ARG;
+    launch_directions(n, wi, dir, pdf, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+int dfg_scatter(void* stream, int64_t n, float bsdf_fraction, const float* wi, const uint8_t* guided,
+                const uint8_t* mix, const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta,
+                float* thp, float* scatter_pdf, float* mis, uint8_t* alive) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (!(bsdf_fraction >= 0.0f && bsdf_fraction <= 1.0f)) return DFG_ERR_INVALID_ARG;
+    if (n == 0) return DFG_OK;
+    if (!wi || !bsdf_pdf || !bsdf_val || !cos_theta || !thp) return DFG_ERR_INVALID_ARG;
+    launch_scatter(n, bsdf_fraction, wi, guided, mix, bsdf_pdf, bsdf_val, cos_theta, thp, scatter_pdf, mis, alive,
+                   (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
 }
 
 }  // extern "C"

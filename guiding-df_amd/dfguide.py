@@ -36,7 +36,7 @@ EXPORTS = [
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
     "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
-    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_prepare_features",
+    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_directions", "dfg_scatter", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -147,6 +147,8 @@ def load_library(path=LIB_PATH):
                                    ctypes.POINTER(ctypes.c_float)], I),
         "dfg_nrc_substitute_radiance": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P,
                                          ctypes.POINTER(ctypes.c_int64)], I),
+        "dfg_directions": ([P, I64, P, P, P], I),
+        "dfg_scatter": ([P, I64, ctypes.c_float, P, P, P, P, P, P, P, P, P, P], I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
         "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
@@ -538,6 +540,25 @@ def df_head(logits_theta, logits_phi, u, wi, guided=None, interpolation=1, bins_
                            _ptr(bins_theta), _ptr(bins_phi), _ptr(seg))
     if code != 0:
         raise DfgError(code, "dfg_df_head")
+
+
+def directions(wi, dir=None, pdf=None, stream=None):
+    """Directions (squareToUniformSphere) and solid-angle pdfs of query results."""
+    lib = load_library()
+    rc = lib.dfg_directions(_stream(stream), wi.shape[0], _ptr(wi), _ptr(dir), _ptr(pdf))
+    if rc != 0:
+        raise DfgError(rc, "dfg_directions")
+
+
+def scatter(wi, bsdf_pdf, bsdf_val, cos_theta, thp, bsdf_fraction=0.5, guided=None, mix=None, scatter_pdf=None,
+            mis=None, alive=None, stream=None):
+    """Mixture pdf and throughput update of the scatter step (thp updated in place)."""
+    lib = load_library()
+    rc = lib.dfg_scatter(_stream(stream), wi.shape[0], bsdf_fraction, _ptr(wi), _ptr(guided), _ptr(mix),
+                         _ptr(bsdf_pdf), _ptr(bsdf_val), _ptr(cos_theta), _ptr(thp), _ptr(scatter_pdf), _ptr(mis),
+                         _ptr(alive))
+    if rc != 0:
+        raise DfgError(rc, "dfg_scatter")
 
 
 def prepare_features(pos, aabb, wo, normal, roughness, feat, stream=None):

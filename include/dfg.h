@@ -243,6 +243,27 @@ int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, in
                                 uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb,
                                 int64_t* n_inferred);
 
+/* ---- Around the query in a wavefront renderer -------------------------------
+ * Directions and solid-angle pdfs of query results (AutoRegressive::sample and
+ * pdf, AR.h:25-48): dir[i] = squareToUniformSphere(wi[i][0], wi[i][1])
+ * (math_utils.h:226-231); pdf[i] = |p_theta| |p_phi| / (4 pi).  dir or pdf may
+ * be NULL. */
+int dfg_directions(void* stream, int64_t n, const float* wi, float* dir, float* pdf);
+/* The scatter step of GuidedPathTracer::handleIntersections
+ * (integrator.cpp:203-299) once the BSDF is evaluated: per row, the guide pdf
+ * pg = |p_theta| |p_phi| / (4 pi) from wi [n][4], the BSDF pdf pb, value
+ * bsdf_val [n][3] and |cos| at the scattered direction come from the caller.
+ * With f = bsdf_fraction: guided rows scatter with (1 - f) pg + f pb; other
+ * rows with pb, or with f pb + (1 - f) pg when mix[i] != 0 (guiding on, f < 1
+ * and a smooth BSDF).  Where that pdf is non-zero and bsdf_val is not all
+ * zero: thp[i] *= bsdf_val |cos| / (pdf + 1e-5), alive[i] = 1, and
+ * mis[i] = (1 - f) pg / pdf (the training records' MIS weight); otherwise
+ * alive[i] = 0 and thp is left.  scatter_pdf [n] receives the pdf.  guided and
+ * mix may be NULL (all 0). */
+int dfg_scatter(void* stream, int64_t n, float bsdf_fraction, const float* wi, const uint8_t* guided,
+                const uint8_t* mix, const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta,
+                float* thp, float* scatter_pdf, float* mis, uint8_t* alive);
+
 /* Feature preparation (generate_inference_data_AR, train.cu:334-362):
  * shading points -> DF feature rows [n][9].
  *   pos [n][3] world space, aabb[6] = (min xyz, max xyz), wo [n][3] (any length),

@@ -541,3 +541,22 @@ def nrc_substitute_f64(rec, n_train, offset, stride, aabb, nrc_rgb):
         R = nrc_rgb[i]
         newL[kk, pp] = np.where(R > M_EPSILON, newL[kk, pp] / np.where(R > M_EPSILON, R, 1.0), newL[kk, pp])
     return f, np.stack([k, p], 1), newL, depth
+
+
+def scatter_f64(f, wi, guided, mix, bsdf_pdf, bsdf_val, cos_theta, thp):
+    """handleIntersections' scatter step (integrator.cpp:203-299) once the BSDF
+    is evaluated: (thp', scatter pdf, MIS weight, alive)."""
+    wi = np.asarray(wi, np.float64)
+    pg = np.abs(wi[:, 2]) * np.abs(wi[:, 3]) * np.float64(np.float32(0.07957747154594766788))
+    pb = np.asarray(bsdf_pdf, np.float64)
+    g = np.zeros(len(pb), bool) if guided is None else np.asarray(guided) != 0
+    m = np.zeros(len(pb), bool) if mix is None else np.asarray(mix) != 0
+    p = np.where(g, (1 - f) * pg + f * pb, np.where(m, pb * f + (1 - f) * pg, pb))
+    bv = np.asarray(bsdf_val, np.float64)
+    alive = (p != 0) & (bv != 0).any(1)
+    thp = np.asarray(thp, np.float64).copy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        upd = thp * bv * np.abs(np.asarray(cos_theta, np.float64))[:, None] / (p[:, None] + M_EPSILON)
+        mis = (1 - f) * pg / p
+    thp = np.where(alive[:, None], upd, thp)
+    return thp, p, mis, alive

@@ -171,3 +171,38 @@ def test_nrc_substitution_matches_oracle(torch_cuda):
     assert np.allclose(L_g[~same], L_o[~same], rtol=1e-5, atol=1e-7, equal_nan=True)
     assert (~same).sum() > 100
     f.close()
+
+
+def test_directions_and_scatter_match_oracle(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(70)
+    n = 5000
+    wi = np.concatenate([rng.random((n, 2)), rng.uniform(0.1, 3, (n, 2))], 1).astype(np.float32)
+    d_wi = dev(torch, wi)
+    d_dir = torch.zeros((n, 3), device="cuda")
+    d_pdf = torch.zeros(n, device="cuda")
+    G.directions(d_wi, d_dir, d_pdf)
+    torch.cuda.synchronize()
+    v = O.square_to_uniform_sphere(wi[:, :2].astype(np.float64))
+    assert np.allclose(host(d_dir), v, atol=2e-6)
+    assert np.allclose(O.uniform_sphere_to_square(host(d_dir).astype(np.float64))[:, 0], wi[:, 0], atol=1e-5)
+    assert np.allclose(host(d_pdf), wi[:, 2] * wi[:, 3] / (4 * np.pi), rtol=1e-6)
+    pb = rng.uniform(0.0, 2, n).astype(np.float32)
+    pb[:50] = 0.0
+    bv = rng.random((n, 3)).astype(np.float32)
+    bv[50:100] = 0.0
+    c = rng.uniform(-1, 1, n).astype(np.float32)
+    thp = rng.random((n, 3)).astype(np.float32)
+    guided = (rng.random(n) < 0.5).astype(np.uint8)
+    mix = (rng.random(n) < 0.5).astype(np.uint8)
+    d_thp = dev(torch, thp)
+    d_p, d_m = torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    d_alive = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    G.scatter(d_wi, dev(torch, pb), dev(torch, bv), dev(torch, c), d_thp, bsdf_fraction=0.3, guided=dev(torch, guided),
+              mix=dev(torch, mix), scatter_pdf=d_p, mis=d_m, alive=d_alive)
+    torch.cuda.synchronize()
+    t, p, mis, alive = O.scatter_f64(0.3, wi, guided, mix, pb, bv, c, thp)
+    assert np.array_equal(host(d_alive) != 0, alive)
+    assert np.allclose(host(d_p), p, rtol=2e-6, atol=1e-9)
+    assert np.allclose(host(d_thp), t, rtol=2e-5, atol=1e-7)
+    assert np.allclose(host(d_m)[alive], mis[alive], rtol=2e-5)

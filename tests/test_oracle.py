@@ -538,3 +538,27 @@ def test_batch_schedules():
     sched, planned = O.nrc_batches(1000000, 1048576, 4)
     assert planned == 4 and [b for _, b in sched] == [249984] * 4
     assert [o for o, _ in sched] == [0, 249984, 499968, 749952]
+
+
+def test_scatter_mixture_closed_forms():
+    rng = np.random.default_rng(60)
+    n = 64
+    wi = np.concatenate([rng.random((n, 2)), rng.uniform(0.1, 3, (n, 2))], 1)
+    pb, bv, c, thp = rng.uniform(0.1, 2, n), rng.random((n, 3)), rng.random(n), rng.random((n, 3))
+    pg = wi[:, 2] * wi[:, 3] * np.float64(np.float32(1 / (4 * np.pi)))
+    ones = np.ones(n, np.uint8)
+    # f = 0: guided rows scatter with the guide alone and carry MIS weight 1
+    t, p, mis, alive = O.scatter_f64(0.0, wi, ones, None, pb, bv, c, thp)
+    assert np.allclose(p, pg) and np.allclose(mis, 1.0) and alive.all()
+    assert np.allclose(t, thp * bv * c[:, None] / (pg[:, None] + 1e-5))
+    # f = 1, unguided, no mixing: the BSDF alone, MIS weight 0
+    t, p, mis, _ = O.scatter_f64(1.0, wi, None, None, pb, bv, c, thp)
+    assert np.allclose(p, pb) and np.allclose(mis, 0.0)
+    # a mixed unguided row equals a guided row (same mixture)
+    _, p1, _, _ = O.scatter_f64(0.3, wi, None, ones, pb, bv, c, thp)
+    _, p2, _, _ = O.scatter_f64(0.3, wi, ones, None, pb, bv, c, thp)
+    assert np.allclose(p1, p2)
+    # a zero BSDF value ends the path and leaves the throughput
+    bv0 = bv.copy(); bv0[:5] = 0
+    t, _, _, alive = O.scatter_f64(0.5, wi, ones, None, pb, bv0, c, thp)
+    assert not alive[:5].any() and np.array_equal(t[:5], thp[:5])

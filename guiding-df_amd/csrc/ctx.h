@@ -107,6 +107,7 @@ struct dfg_ctx {
     DevBuf<uint32_t> rec_local;
     DevBuf<int64_t> rec_blk, rec_count;
     DevBuf<int2> rec_vidx;
+    DevBuf<float> img_rows, img_wi2;  // rows of a pdf image
     // bucketed grid-backward workspace (shared by the nets, used in stream order)
     DevBuf<int64_t> gb_offs, gb_cursor;
     DevBuf<uint32_t> gb_idx, gb_cnt;
@@ -121,7 +122,7 @@ struct dfg_ctx {
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
         pos.release(); mean_ws.release();
         rec_feat.release(); rec_tgt.release(); rec_loss.release(); rec_local.release(); rec_blk.release();
-        rec_count.release(); rec_vidx.release();
+        rec_count.release(); rec_vidx.release(); img_rows.release(); img_wi2.release();
     }
 };
 

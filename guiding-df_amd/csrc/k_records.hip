@@ -271,7 +271,24 @@ __global__ void scatter_kernel(int64_t n, float f, const float* __restrict__ wi,
     if (mis) mis[i] = (1.0f - f) * pg / p;
 }
 
+// rows of a pdf image: the point's features broadcast, (u0, u1) at pixel centres
+__global__ void pdf_image_rows_kernel(const float* __restrict__ feat, int D, int rt, int rp, float* __restrict__ rows,
+                                      float* __restrict__ wi2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rt * rp) return;
+    for (int d = 0; d < D; ++d) rows[i * D + d] = feat[d];
+    const int a = (int)(i / rp), b = (int)(i % rp);
+    wi2[2 * i] = ((float)a + 0.5f) / (float)rt;
+    wi2[2 * i + 1] = ((float)b + 0.5f) / (float)rp;
+}
+
 }  // namespace
+
+void launch_pdf_image_rows(const float* feat, int D, int rt, int rp, float* rows, float* wi2, hipStream_t s) {
+    const int64_t n = (int64_t)rt * rp;
+    hipLaunchKernelGGL(pdf_image_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, feat, D, rt, rp,
+                       rows, wi2);
+}
 
 void launch_directions(int64_t n, const float* wi, float* dir, float* pdf, hipStream_t s) {
     if (n <= 0) return;

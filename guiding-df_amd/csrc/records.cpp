@@ -16,6 +16,7 @@ void launch_training_targets(const dfg_records& r, int64_t n_train, uint32_t off
 void launch_nrc_substitute(const dfg_records& r, const int2* vidx, const float* rgb, const int64_t* count,
                            int64_t capacity, hipStream_t s);
 void launch_directions(int64_t n, const float* wi, float* dir, float* pdf, hipStream_t s);
+void launch_pdf_image_rows(const float* feat, int D, int rt, int rp, float* rows, float* wi2, hipStream_t s);
 void launch_scatter(int64_t n, float f, const float* wi, const uint8_t* guided, const uint8_t* mix,
                     const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta, float* thp, float* spdf,
                     float* mis, uint8_t* alive, hipStream_t s);
@@ -203,6 +204,23 @@ int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, in
         HIPX(hipStreamSynchronize(st));
     }
     if (n_inferred) *n_inferred = m;
+    API_END(ctx)
+}
+
+int dfg_pdf_image(dfg_ctx* ctx, void* stream, const float* feat, int res_theta, int res_phi, float* image) {
+    if (!ctx) return DFG_ERR_NOT_INIT;
+    API_BEGIN
+    if (!feat || !image) throw Error(DFG_ERR_INVALID_ARG, "null argument");
+    if (res_theta < 1 || res_phi < 1 || res_theta > 4096 || res_phi > 4096)
+        throw Error(DFG_ERR_BAD_SIZE, "image resolution must be 1..4096");
+    const int64_t n = (int64_t)res_theta * res_phi;
+    ctx->img_rows.ensure((size_t)n * ctx->D);
+    ctx->img_wi2.ensure((size_t)n * 2);
+    hipStream_t st = (hipStream_t)stream;
+    launch_pdf_image_rows(feat, ctx->D, res_theta, res_phi, ctx->img_rows.p, ctx->img_wi2.p, st);
+    check_launch();
+    const int rc = dfg_pdf(ctx, st, n, ctx->img_rows.p, ctx->D, ctx->img_wi2.p, image);
+    if (rc != DFG_OK) throw Error(rc, ctx->err);
     API_END(ctx)
 }
 

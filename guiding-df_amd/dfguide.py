@@ -36,7 +36,7 @@ EXPORTS = [
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
     "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
-    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_directions", "dfg_scatter", "dfg_prepare_features",
+    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_pdf_image", "dfg_directions", "dfg_scatter", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -147,6 +147,7 @@ def load_library(path=LIB_PATH):
                                    ctypes.POINTER(ctypes.c_float)], I),
         "dfg_nrc_substitute_radiance": ([P, P, ctypes.POINTER(DfgRecords), I64, U32, U32, P,
                                          ctypes.POINTER(ctypes.c_int64)], I),
+        "dfg_pdf_image": ([P, P, P, I, I, P], I),
         "dfg_directions": ([P, I64, P, P, P], I),
         "dfg_scatter": ([P, I64, ctypes.c_float, P, P, P, P, P, P, P, P, P, P], I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
@@ -392,6 +393,11 @@ class GuidingField:
         self._c(fn(self.ctx, _stream(stream), ctypes.byref(r), n_train, offset, stride, _ptr(aabb), batch_size,
                    batch_per_frame, ctypes.byref(rows), ctypes.byref(steps), ctypes.byref(loss)))
         return rows.value, steps.value, loss.value
+
+    def pdf_image(self, feat, image, stream=None):
+        """Solid-angle pdf over the (u0, u1) pixel-centre grid of image [res_theta, res_phi]."""
+        self._c(self.lib.dfg_pdf_image(self.ctx, _stream(stream), _ptr(feat), image.shape[0], image.shape[1],
+                                       _ptr(image)))
 
     def nrc_substitute_radiance(self, rec, n_train, aabb, offset=0, stride=1, stream=None):
         """Replaces the records' indirect radiance by the radiance cache (writes

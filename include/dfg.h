@@ -243,6 +243,13 @@ int dfg_nrc_substitute_radiance(dfg_ctx* ctx, void* stream, dfg_records* rec, in
                                 uint32_t pixel_offset, uint32_t pixel_stride, const float* aabb,
                                 int64_t* n_inferred);
 
+/* Dense pdf image of one shading point (LearnedDistribution::
+ * mouseOverInference / mouseOverInferenceOptimized, learnedDistribution.cpp:
+ * 386-587): image[i][j] = the solid-angle pdf (as dfg_pdf) at the square
+ * coordinates u0 = (i + 0.5) / res_theta, u1 = (j + 0.5) / res_phi.  feat:
+ * device [n_input_dims].  res_theta, res_phi in 1..4096. */
+int dfg_pdf_image(dfg_ctx* ctx, void* stream, const float* feat, int res_theta, int res_phi, float* image);
+
 /* ---- Around the query in a wavefront renderer -------------------------------
  * Directions and solid-angle pdfs of query results (AutoRegressive::sample and
  * pdf, AR.h:25-48): dir[i] = squareToUniformSphere(wi[i][0], wi[i][1])

@@ -206,3 +206,34 @@ def test_directions_and_scatter_match_oracle(torch_cuda):
     assert np.allclose(host(d_p), p, rtol=2e-6, atol=1e-9)
     assert np.allclose(host(d_thp), t, rtol=2e-5, atol=1e-7)
     assert np.allclose(host(d_m)[alive], mis[alive], rtol=2e-5)
+
+
+def test_pdf_image_matches_oracle_and_integrates_to_one(torch_cuda):
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    from gpu_util import df_features
+    feat = df_features(1, 80)[0]
+    rt, rp = 128, 96
+    img = torch.zeros((rt, rp), device="cuda")
+    f.pdf_image(dev(torch, feat), img)
+    torch.cuda.synchronize()
+    u0 = (np.arange(rt) + 0.5) / rt
+    u1 = (np.arange(rp) + 0.5) / rp
+    wi = np.zeros((rt * rp, 4), np.float32)
+    wi[:, 0] = np.repeat(u0, rp)
+    wi[:, 1] = np.tile(u1, rt)
+    wi[:, 2:] = 1.0
+    o = O.query_f64(th, ph, f.get_params(0, G.DFG_PARAMS_INFERENCE), f.get_params(1, G.DFG_PARAMS_INFERENCE),
+                    np.repeat(feat[None], rt * rp, 0), 9, np.zeros((rt * rp, 4), np.float32), wi, 1,
+                    guided=np.zeros(rt * rp, np.uint8))
+    ref = (np.abs(o[:, 2].astype(np.float64)) * np.abs(o[:, 3]) / (4 * np.pi)).reshape(rt, rp)
+    g = host(img).astype(np.float64)
+    rel = np.abs(g - ref) / ref
+    assert np.median(rel) <= 1e-5 and rel.max() <= 1e-3
+    # the density integrates to one over the sphere (midpoint rule on the square, 4 pi sr)
+    assert abs(g.sum() * 4 * np.pi / (rt * rp) - 1.0) < 1e-2
+    with pytest.raises(G.DfgError):
+        f.pdf_image(dev(torch, feat), torch.zeros((0, 4), device="cuda"))
+    f.close()

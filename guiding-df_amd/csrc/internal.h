@@ -114,6 +114,7 @@ void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* t
 void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2* tp, const float* pos, int64_t n,
                              int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev = nullptr);
 void launch_interleave_tables(const _Float16* th, const _Float16* ph, uint2* tp, int64_t n_entries, hipStream_t s);
+void launch_extract_pos_worklist(const WorkList* wl, int pos_dim, int64_t rows, float* pos, hipStream_t s);
 void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, float* pos, hipStream_t s,
                         const int64_t* n_dev = nullptr);
 int grid_pos_dim(const NetSpec& hs);

@@ -302,6 +302,8 @@ struct dfg_server {
     double first_submit = -1.0, last_done = 0.0;
     bool nrc = false;
     int nslots = 0;
+    // per launch slot: compact positions and the paired grid pre-pass output
+    std::vector<DevBuf<float>> slot_pos, slot_gth, slot_gph;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
@@ -315,6 +317,9 @@ struct dfg_server {
         if (host_wl) (void)hipHostFree(host_wl);
         wls.release();
         counters.release();
+        for (auto& b : slot_pos) b.release();
+        for (auto& b : slot_gth) b.release();
+        for (auto& b : slot_gph) b.release();
     }
 };
 
@@ -390,7 +395,7 @@ bool server_launch(dfg_server* s) {
         g.rgb = p.r.nrc_rgb;
         g.n = p.r.n;
         g.tile_begin = tiles;
-        g.gbase = 0;
+        g.gbase = l.rows;  // row of the segment in the launch's pre-pass buffers
         tiles += (p.r.n + 63) / 64;
         l.rows += p.r.n;
         l.reqs.push_back(p);
@@ -414,7 +419,22 @@ bool server_launch(dfg_server* s) {
     a.stride = l.reqs[0].r.feat_stride;
     a.n = l.rows;
     a.rgb = s->nrc ? l.reqs[0].r.nrc_rgb : nullptr;
-    fill_lds_strides(s->ctx, a, s->nrc, true);
+    dfg_ctx* c = s->ctx;
+    if (c->pair_ok && l.rows >= c->prepass_min_rows) {
+        // the XCD-aware paired grid pre-pass over every row of the launch
+        const int64_t ld = round_up(l.rows, 64);
+        const size_t cols = (size_t)c->net[0].spec.grid.n_levels * 2;
+        s->slot_pos[slot].ensure((size_t)l.rows * 3);
+        s->slot_gth[slot].ensure(cols * ld);
+        s->slot_gph[slot].ensure(cols * ld);
+        launch_extract_pos_worklist(dwl, grid_pos_dim(c->net[0].spec), l.rows, s->slot_pos[slot].p, st);
+        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, s->slot_pos[slot].p, l.rows, ld,
+                                s->slot_gth[slot].p, s->slot_gph[slot].p, st);
+        a.gfeat_th = s->slot_gth[slot].p;
+        a.gfeat_ph = s->slot_gph[slot].p;
+        a.gld = ld;
+    }
+    fill_lds_strides(c, a, s->nrc, true);
     const int64_t want_blocks = std::min<int64_t>(tiles, (int64_t)cu_count() * 4);
     launch_query_worklist(a, s->ctx->n_theta, s->ctx->n_phi, dwl, ctr, (int)std::max<int64_t>(1, want_blocks), 1, st);
     check_launch();
@@ -459,6 +479,9 @@ int dfg_server_create(dfg_ctx* ctx, const dfg_server_config* cfg, dfg_server** o
     }
     HIPX(hipStreamCreateWithFlags(&s->train_stream, hipStreamNonBlocking));
     s->nslots = cfg->n_streams * 2;
+    s->slot_pos.resize(s->nslots);
+    s->slot_gth.resize(s->nslots);
+    s->slot_gph.resize(s->nslots);
     s->wls.ensure(s->nslots);
     s->counters.ensure(s->nslots);
     HIPX(hipHostMalloc((void**)&s->host_wl, sizeof(WorkList) * s->nslots, hipHostMallocDefault));

@@ -97,11 +97,16 @@ def _requests(torch, n_req, seed, nrc=False, lo=1024, hi=40000):
     return reqs
 
 
-@pytest.mark.parametrize("n_streams,budget_rows", [(2, 1 << 22), (1, 50000), (3, 120000)])
-def test_server_alone_equals_mix(torch_cuda, n_streams, budget_rows):
+@pytest.mark.parametrize("n_streams,budget_rows,prepass_min", [(2, 1 << 22, 1 << 30), (1, 50000, 1 << 30),
+                                                                (3, 120000, 1 << 30), (2, 1 << 22, 1024),
+                                                                (3, 120000, 1024)])
+def test_server_alone_equals_mix(torch_cuda, n_streams, budget_rows, prepass_min):
+    """Every request equals the same request queried alone, with in-kernel grid
+    gathers (prepass_min huge) and with the paired grid pre-pass over the
+    packed launch (prepass_min 1024: alone and packed both use it)."""
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True)
-    cfg["dfg"]["grid_prepass_min_rows"] = 1 << 30
+    cfg["dfg"]["grid_prepass_min_rows"] = prepass_min
     f = G.GuidingField(cfg)
     reqs = _requests(torch, 24, 90 + n_streams, nrc=True)
     alone = {}

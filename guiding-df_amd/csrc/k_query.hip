@@ -122,12 +122,14 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
 
     // ---- radiance cache
     if constexpr (WITH_NRC) {
-        // the radiance cache keeps its (widest) encoded input in fp16
+        // the radiance cache runs at the reference's precision: fp16 encoded
+        // input and fp16 hidden activations (fp32 accumulation).  Its output is
+        // radiance, not a pdf, and feeds training targets only.
         _Float16* const xh = (_Float16*)smem;
         _Float16* const xhrow = xh + lane * a.xs_nrc;
         encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; },
                    nullptr, 0, false, false);
-        mlp_forward_tile<1, true>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
+        mlp_forward_tile<1, false, 2>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
         if (valid && io.rgb) {
             io.rgb[3 * r + 0] = lrow[0];
             io.rgb[3 * r + 1] = lrow[1];

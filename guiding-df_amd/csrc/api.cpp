@@ -385,6 +385,18 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         c->seed = (uint64_t)d.number_or("seed", 7272);
         c->D = (int)d.number_or("n_input_dims", 9);
         c->prepass_min_rows = (int64_t)d.number_or("grid_prepass_min_rows", 32768);
+        c->query_chunks = (int)d.number_or("query_chunks", 1);
+        c->prepass_blocks_per_cu = (int)d.number_or("prepass_blocks_per_cu", 0);
+        c->pipeline_min_rows = (int64_t)d.number_or("pipeline_min_rows", 262144);
+        if (c->query_chunks < 1 || c->query_chunks > 64) throw Error(DFG_ERR_CONFIG, "query_chunks must be in 1..64");
+        if (c->prepass_blocks_per_cu < 0 || c->prepass_blocks_per_cu > 64)
+            throw Error(DFG_ERR_CONFIG, "prepass_blocks_per_cu must be in 0..64");
+        {
+            int dev = 0, cus = 0;
+            HIPX(hipGetDevice(&dev));
+            HIPX(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            if (cus > 0) c->n_cus = cus;
+        }
         const std::string gbw = lower(d.string_or("grid_backward", "bucketed"));
         if (gbw != "bucketed" && gbw != "atomic") throw Error(DFG_ERR_CONFIG, "grid_backward must be bucketed or atomic");
         c->grid_bwd_bucketed = gbw == "bucketed";
@@ -640,6 +652,66 @@ void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, i
     a.gld = ld;
 }
 
+// Chunked query of a large batch: the paired grid pre-pass of every chunk is
+// queued on a side stream, and the query tiles of chunk k (on the caller's
+// stream) wait only for the pre-pass of chunk k.  The pre-pass is bound by L2
+// requests (random table lines) and the tiles by VALU work, so the pre-pass of
+// the next chunks runs on the CUs beside the tiles of the current one.  The
+// per-row results are those of the unchunked query.  Returns false (nothing
+// launched) when the batch is not chunked.
+static bool query_chunked(dfg_ctx* c, const QueryArgs& base, int64_t n, hipStream_t st) {
+    if (!c->pair_ok || c->query_chunks < 2 || n < c->pipeline_min_rows || n < c->prepass_min_rows) return false;
+    if (!c->side) {
+        HIPX(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPX(hipEventCreateWithFlags(&c->pipe_start, hipEventDisableTiming));
+    }
+    while ((int)c->pipe_ev.size() < c->query_chunks) {
+        hipEvent_t e;
+        HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->pipe_ev.push_back(e);
+    }
+    const int64_t ld = round_up(n, 64);
+    const size_t cols = (size_t)c->net[0].spec.grid.n_levels * 2;
+    c->net[0].gfeat.ensure(cols * ld);
+    c->net[1].gfeat.ensure(cols * ld);
+    float* gth = c->net[0].gfeat.p;
+    float* gph = c->net[1].gfeat.p;
+    const float* pos = compact_pos(c, base.feat, base.stride, grid_pos_dim(c->net[0].spec), n, st);
+    // the side stream starts after everything queued so far on the caller's
+    // stream (an earlier query may still read the feature buffers)
+    HIPX(hipEventRecord(c->pipe_start, st));
+    HIPX(hipStreamWaitEvent(c->side, c->pipe_start, 0));
+    const int64_t m = round_up((n + c->query_chunks - 1) / c->query_chunks, 256);
+    const int max_blocks = c->prepass_blocks_per_cu * c->n_cus;
+    int k = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += m, ++k) {
+        const int64_t rows = std::min(m, n - r0);
+        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, pos + 3 * r0, rows, ld, gth + r0,
+                                gph + r0, c->side, nullptr, max_blocks);
+        HIPX(hipEventRecord(c->pipe_ev[k], c->side));
+    }
+    k = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += m, ++k) {
+        const int64_t rows = std::min(m, n - r0);
+        HIPX(hipStreamWaitEvent(st, c->pipe_ev[k], 0));
+        QueryArgs a = base;
+        a.feat = base.feat + r0 * base.stride;
+        a.n = rows;
+        if (base.u) a.u = base.u + 4 * r0;
+        if (base.guided) a.guided = base.guided + r0;
+        if (base.wi) a.wi = base.wi + 4 * r0;
+        if (base.rgb) a.rgb = base.rgb + 3 * r0;
+        if (base.wi2) a.wi2 = base.wi2 + 2 * r0;
+        if (base.pdf_out) a.pdf_out = base.pdf_out + r0;
+        a.gfeat_th = gth + r0;
+        a.gfeat_ph = gph + r0;
+        a.gld = ld;
+        fill_lds_strides(c, a, base.rgb != nullptr, true);
+        launch_query(a, c->n_theta, c->n_phi, st);
+    }
+    return true;
+}
+
 void dfg::check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride) {
     if (n < 0 || n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "n out of range");
     if (n > 0 && !feat) throw Error(DFG_ERR_INVALID_ARG, "null features");
@@ -659,6 +731,10 @@ int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat
     if (n == 0) return DFG_OK;
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.u = u; a.guided = guided; a.wi = wi; a.rgb = nrc_rgb;
+    if (query_chunked(ctx, a, n, (hipStream_t)stream)) {
+        check_launch();
+        return DFG_OK;
+    }
     grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
     fill_lds_strides(ctx, a, nrc_rgb != nullptr, true);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);
@@ -674,6 +750,10 @@ int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_s
     if (n == 0) return DFG_OK;
     QueryArgs a = base_args(ctx);
     a.feat = feat; a.stride = feat_stride; a.n = n; a.wi2 = wi2; a.pdf_out = pdf;
+    if (query_chunked(ctx, a, n, (hipStream_t)stream)) {
+        check_launch();
+        return DFG_OK;
+    }
     grid_prepass(ctx, a, n, feat, feat_stride, (hipStream_t)stream);
     fill_lds_strides(ctx, a, false, true);
     launch_query(a, ctx->n_theta, ctx->n_phi, (hipStream_t)stream);

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/dfg.h"
 #include "internal.h"
@@ -94,6 +95,15 @@ struct dfg_ctx {
     uint64_t seed = 7272;
     int64_t prepass_min_rows = 32768;  // queries from which the XCD-aware grid pre-pass is used
     bool grid_bwd_bucketed = true;     // bucketed (LDS) grid backward; false: fp32 atomic scatter
+    // chunked query: the paired grid pre-pass of the next row chunk runs on a
+    // side stream beside the query tiles of the current one
+    int query_chunks = 1;              // chunks of a large query (1: no chunking)
+    int prepass_blocks_per_cu = 0;     // persistent pre-pass grid per CU in chunked mode (0: one block per item)
+    int64_t pipeline_min_rows = 262144;
+    int n_cus = 256;
+    hipStream_t side = nullptr;
+    hipEvent_t pipe_start = nullptr;
+    std::vector<hipEvent_t> pipe_ev;
     // theta and phi grids of identical structure (F = 2): their inference
     // tables are also kept interleaved, one 8-byte entry per vertex for both
     bool pair_ok = false;
@@ -115,6 +125,9 @@ struct dfg_ctx {
     DevBuf<int4> gb_items;
     DevBuf<int> gb_n_items;
     ~dfg_ctx() {
+        for (auto e : pipe_ev) (void)hipEventDestroy(e);
+        if (pipe_start) (void)hipEventDestroy(pipe_start);
+        if (side) (void)hipStreamDestroy(side);
         tab_pair.release();
         gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
         gb_items.release(); gb_n_items.release();

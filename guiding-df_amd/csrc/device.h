@@ -21,12 +21,14 @@ namespace dfg {
 #define DFG_INV_4PI 0.07957747154594766788f  // M_INV_4PI, krrmath/constants.h:17
 
 // ---------------------------------------------------------------------------
-// Deterministic exp / log.  Both use only IEEE-754 +, -, *, / and exact
-// bit manipulation, with FMA contraction disabled, so the host restatement of
-// the same steps produces bit-identical results.
+// Deterministic exp / log.  Both use only correctly rounded IEEE-754
+// operations (+, -, *, /, explicit fused multiply-add, ldexp) and exact bit
+// manipulation, with implicit FMA contraction disabled, so the host
+// restatement of the same steps produces bit-identical results.
 //   exp: Cody & Waite additive range reduction x = k ln2 + r with ln2 split
 //        into a 15-bit head (k*ln2_hi exact) and a tail, then the Taylor
-//        polynomial of e^r through r^7 (|r| <= ln2/2), then scaling by 2^k.
+//        polynomial of e^r through r^7 (|r| <= ln2/2) in Horner form, then
+//        scaling by 2^k.
 //   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
 //        s = f/(2+f), log m = f - (f^2/2 - s (f^2/2 + R(s^2))) with R the
 //        atanh series 2s^2/3 + ... + 2s^10/11 (the reduction of fdlibm's
@@ -39,23 +41,23 @@ __device__ __forceinline__ float pow2i(int k) {  // 2^k for k in [-126, 127]
 
 __device__ inline float det_expf(float x) {
 #pragma clang fp contract(off)
-    if (x != x) return x;
-    if (x > 88.72283935546875f) return __builtin_inff();
-    if (x < -103.972084045410156f) return 0.0f;
-    const float k = __builtin_rintf(x * 1.44269502162933349609f);
-    const float r = (x - k * 0.693145751953125f) - k * 1.42860676533018704504e-06f;
+    // k = rint(x / ln 2) clamped to a range where the int conversion is
+    // defined (outside it the guards below decide); r = x - k ln2 with ln2
+    // split hi + lo (k * hi is exact); Taylor through r^7 by explicit FMAs;
+    // scaling by 2^k with one ldexp (correctly rounded, subnormals included).
+    const float k = fminf(fmaxf(__builtin_rintf(x * 1.44269502162933349609f), -200.0f), 200.0f);
+    const float r = __builtin_fmaf(-k, 1.42860676533018704504e-06f, __builtin_fmaf(-k, 0.693145751953125f, x));
     float p = 1.98412698412698412698e-04f;
-    p = p * r + 1.38888888888888888889e-03f;
-    p = p * r + 8.33333333333333333333e-03f;
-    p = p * r + 4.16666666666666666667e-02f;
-    p = p * r + 1.66666666666666666667e-01f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    const int ki = (int)k;
-    if (ki > 127) return (p * pow2i(127)) * pow2i(ki - 127);
-    if (ki < -126) return (p * pow2i(-126)) * pow2i(ki + 126);
-    return p * pow2i(ki);
+    p = __builtin_fmaf(p, r, 1.38888888888888888889e-03f);
+    p = __builtin_fmaf(p, r, 8.33333333333333333333e-03f);
+    p = __builtin_fmaf(p, r, 4.16666666666666666667e-02f);
+    p = __builtin_fmaf(p, r, 1.66666666666666666667e-01f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const float e = __builtin_ldexpf(p, (int)k);
+    if (x != x) return x;
+    return x > 88.72283935546875f ? __builtin_inff() : (x < -103.972084045410156f ? 0.0f : e);
 }
 
 __device__ inline float det_logf(float x) {
@@ -94,13 +96,16 @@ __device__ inline float det_logf(float x) {
 // reference under src/ext/tcnn).
 // ---------------------------------------------------------------------------
 
-// quartic_cdf, tcnn common_device.h:492-497
+// quartic_cdf, tcnn common_device.h:492-497: clamp(15/16 u (1 - 2u^2/3 + u^4/5)
+// + 1/2, 0, 1), u = x * inv_radius; the polynomial by explicit fused
+// multiply-adds (the same value to an ulp or two, fewer instructions).
 __device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
 #pragma clang fp contract(off)
     const float u = x * inv_radius;
     const float u2 = u * u;
     const float u4 = u2 * u2;
-    return fmaxf(0.0f, fminf(1.0f, 0.9375f * u * (1.0f - 0.666666686534881591797f * u2 + 0.200000002980232238770f * u4) + 0.5f));
+    const float poly = __builtin_fmaf(0.200000002980232238770f, u4, __builtin_fmaf(-0.666666686534881591797f, u2, 1.0f));
+    return fmaxf(0.0f, fminf(1.0f, __builtin_fmaf(0.9375f * u, poly, 0.5f)));
 }
 
 // grid_index with the CoherentPrime hash, tcnn grid.h:111-128, :164-186.
@@ -716,11 +721,15 @@ struct ActStore {
     int64_t row0;       // first batch row of this tile
 };
 
-// fp32 -> (hi, lo) fp16 pair with hi + lo == v to ~2^-22 relative: the MFMA
-// then sees the activation at fp32-level precision through two products.
+// fp32 -> (hi, lo) fp16 pair with hi + lo == v to ~2^-21 relative: the MFMA
+// then sees the activation at fp32-level precision through two products.  hi
+// keeps the sign, exponent and top 10 mantissa bits of v (a value fp16 holds
+// exactly over its normal range), so v - hi is exact in fp32 and only lo is
+// rounded; below the fp16 normal range the pair is exact to ~2^-25 absolute.
 __device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
-    hi = (_Float16)v;
-    lo = (_Float16)(v - (float)hi);
+    const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & 0xffffe000u);
+    hi = (_Float16)h;
+    lo = (_Float16)(v - h);
 }
 
 // relu(C) of 4 M-tiles -> (hi, lo) operands of the next layer.
@@ -929,9 +938,8 @@ __device__ __forceinline__ void grid_encode_block(const NetSpec& ns, const _Floa
 __device__ __forceinline__ void grid_encode_pair_block(const NetSpec& ns, const uint2* __restrict__ tp,
                                                        const float* __restrict__ pos, int64_t n, int64_t ld,
                                                        float* __restrict__ out_th, float* __restrict__ out_ph,
-                                                       int n_groups, int chunks) {
+                                                       int n_groups, int chunks, int b) {
     const GridSpec& g = ns.grid;
-    const int b = blockIdx.x;
     const int phase = b / (n_groups * chunks);
     const int rem = b - phase * n_groups * chunks;
     const int level = phase * n_groups + rem % n_groups;

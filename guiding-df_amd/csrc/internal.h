@@ -110,9 +110,12 @@ void launch_worklist_setup(WorkList* wl, const int64_t* sizes, int n_bounces, in
 void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,
                         int64_t ld, float* out, hipStream_t s, const int64_t* n_dev = nullptr);
 // Theta + phi pre-pass in one launch from their interleaved table (grids of
-// identical structure, F = 2), and the interleaved copy itself.
+// identical structure, F = 2), and the interleaved copy itself.  max_blocks > 0
+// caps the grid (a persistent grid of that many workgroups, rounded down to a
+// multiple of the XCD count).
 void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2* tp, const float* pos, int64_t n,
-                             int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev = nullptr);
+                             int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev = nullptr,
+                             int max_blocks = 0);
 void launch_interleave_tables(const _Float16* th, const _Float16* ph, uint2* tp, int64_t n_entries, hipStream_t s);
 void launch_extract_pos_worklist(const WorkList* wl, int pos_dim, int64_t rows, float* pos, hipStream_t s);
 void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, float* pos, hipStream_t s,

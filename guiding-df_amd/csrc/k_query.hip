@@ -271,11 +271,18 @@ __global__ __launch_bounds__(256) void grid_encode_kernel(const NetSpec* __restr
     grid_encode_block<F>(*dns, table, pos, n_dev ? min(n, *n_dev) : n, ld, out, n_groups, chunks);
 }
 
+// One block item (level, 256 rows) per workgroup, or, with a grid smaller than
+// the item count, a persistent grid striding over the items: the grid size is
+// then a multiple of the XCD count, so a workgroup keeps its XCD's level in
+// every phase, and the grid holds only part of the chip's wave slots (the rest
+// stays free for query tiles running beside it on another stream).
 __global__ __launch_bounds__(256) void grid_encode_pair_kernel(const NetSpec* __restrict__ dns, const uint2* tp,
                                                                const float* pos, int64_t n, int64_t ld, float* out_th,
                                                                float* out_ph, int n_groups, int chunks,
-                                                               const int64_t* n_dev) {
-    grid_encode_pair_block(*dns, tp, pos, n_dev ? min(n, *n_dev) : n, ld, out_th, out_ph, n_groups, chunks);
+                                                               const int64_t* n_dev, int items) {
+    const int64_t nn = n_dev ? min(n, *n_dev) : n;
+    for (int b = blockIdx.x; b < items; b += gridDim.x)
+        grid_encode_pair_block(*dns, tp, pos, nn, ld, out_th, out_ph, n_groups, chunks, b);
 }
 
 // Interleaved copy {theta f0 f1, phi f0 f1} of two F = 2 grid tables.
@@ -337,14 +344,18 @@ void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, f
 }
 
 void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2* tp, const float* pos, int64_t n,
-                             int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev) {
+                             int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev,
+                             int max_blocks) {
     if (n <= 0 || !hs.grid.present) return;
     const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
     const int phases = (hs.grid.n_levels + groups - 1) / groups;
     const int chunks = (int)((n + 255) / 256);
-    const unsigned blocks = (unsigned)((int64_t)chunks * groups * phases);
-    hipLaunchKernelGGL(grid_encode_pair_kernel, dim3(blocks), dim3(256), 0, s, dns, tp, pos, n, ld, out_th, out_ph,
-                       groups, chunks, n_dev);
+    const int64_t items = (int64_t)chunks * groups * phases;  // <= 2^26 for n <= 2^30
+    int64_t blocks = items;
+    if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks / kXcdGroups * kXcdGroups;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(grid_encode_pair_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dns, tp, pos, n, ld, out_th,
+                       out_ph, groups, chunks, n_dev, (int)items);
 }
 
 void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,

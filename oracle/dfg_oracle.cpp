@@ -66,30 +66,32 @@ const double kPi = 3.14159265358979323846;
 // ---------------------------------------------------------------------------
 // Deterministic fp32 exp / log (restatement of the same published steps the
 // device code uses: Cody-Waite reduction + Taylor through r^7; fdlibm e_log.c
-// reduction f - (f^2/2 - s (f^2/2 + R)) with the atanh series through s^11).  Plain IEEE ops, no contraction (-ffp-contract=off).
+// reduction f - (f^2/2 - s (f^2/2 + R)) with the atanh series through s^11).
+// Correctly rounded IEEE ops only (explicit fmaf / ldexpf where the steps name
+// them), no implicit contraction (-ffp-contract=off).
 // ---------------------------------------------------------------------------
 inline float bits_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
 inline uint32_t f_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
 inline float p2(int k) { return bits_f((uint32_t)(k + 127) << 23); }
 
 float det_expf(float x) {
+    // rint(x / ln 2) clamped to +-200 (the guards decide outside that range),
+    // r = x - k ln2 (hi + lo, two fused multiply-adds), Taylor through r^7 by
+    // fused multiply-adds, one correctly rounded ldexp
+    const float k = fminf(fmaxf(rintf(x * 1.44269502162933349609f), -200.0f), 200.0f);
+    const float r = fmaf(-k, 1.42860676533018704504e-06f, fmaf(-k, 0.693145751953125f, x));
+    float p = 1.98412698412698412698e-04f;
+    p = fmaf(p, r, 1.38888888888888888889e-03f);
+    p = fmaf(p, r, 8.33333333333333333333e-03f);
+    p = fmaf(p, r, 4.16666666666666666667e-02f);
+    p = fmaf(p, r, 1.66666666666666666667e-01f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
     if (x != x) return x;
     if (x > 88.72283935546875f) return INFINITY;
     if (x < -103.972084045410156f) return 0.0f;
-    const float k = rintf(x * 1.44269502162933349609f);
-    const float r = (x - k * 0.693145751953125f) - k * 1.42860676533018704504e-06f;
-    float p = 1.98412698412698412698e-04f;
-    p = p * r + 1.38888888888888888889e-03f;
-    p = p * r + 8.33333333333333333333e-03f;
-    p = p * r + 4.16666666666666666667e-02f;
-    p = p * r + 1.66666666666666666667e-01f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    const int ki = (int)k;
-    if (ki > 127) return (p * p2(127)) * p2(ki - 127);
-    if (ki < -126) return (p * p2(-126)) * p2(ki + 126);
-    return p * p2(ki);
+    return ldexpf(p, (int)k);
 }
 
 float det_logf(float x) {

@@ -927,7 +927,9 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
     const GridBucketPlan plan = gp >= 0 ? grid_bucket_plan(ns, n) : GridBucketPlan{};
     if (gp >= 0 && c->grid_bwd_bucketed && plan.ok) {
         const int F = ns.grid.n_feat;
-        c->gb_offs.ensure((size_t)plan.n_slices + 1);
+        // slice totals / offsets, then the per-level maxima (DFG_MAX_LEVELS uint32)
+        c->gb_offs.ensure((size_t)plan.n_slices + 1 + DFG_MAX_LEVELS / 2);
+        c->gb_part.ensure((size_t)plan.max_partials * kGridSliceFloats);
         c->gb_cursor.ensure((size_t)plan.n_slices);
         c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
         if (F != 2) c->gb_idx.ensure((size_t)plan.records);
@@ -953,6 +955,8 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         gb.rec_val = c->gb_val.p;
         gb.items = c->gb_items.p;
         gb.n_items = c->gb_n_items.p;
+        gb.lvl_max = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
+        gb.partial = c->gb_part.p;
         launch_grid_bwd_bucketed(gb, ns, plan, st);
     } else if (gp >= 0) {
         const size_t ngrid = (size_t)ns.n_params - ns.grid.param_begin;

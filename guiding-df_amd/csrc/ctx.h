@@ -124,13 +124,14 @@ struct dfg_ctx {
     DevBuf<float> gb_val;
     DevBuf<int4> gb_items;
     DevBuf<int> gb_n_items;
+    DevBuf<unsigned long long> gb_part;  // fixed-point partial sums of multi-part slices
     ~dfg_ctx() {
         for (auto e : pipe_ev) (void)hipEventDestroy(e);
         if (pipe_start) (void)hipEventDestroy(pipe_start);
         if (side) (void)hipStreamDestroy(side);
         tab_pair.release();
         gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
-        gb_items.release(); gb_n_items.release();
+        gb_items.release(); gb_n_items.release(); gb_part.release();
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
         pos.release(); mean_ws.release();

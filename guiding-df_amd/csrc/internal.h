@@ -189,9 +189,10 @@ struct GridBwdArgs {
 void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s);
 
 // Bucketed grid backward: the table gradient is built slice by slice in LDS
-// (one slice = kGridSliceFloats floats of one level) from compact update
-// records, then written once; no global atomics, no zero fill.
-constexpr int kGridSliceFloats = 16384;       // 64 KB of LDS per slice
+// (one slice = kGridSliceFloats gradient values of one level) from compact
+// update records, as 64-bit fixed-point sums, then written once; no global
+// atomics, no zero fill.
+constexpr int kGridSliceFloats = 8192;        // 64 KB of int64 LDS accumulators per slice
 constexpr int kGridMaxSlicesPerLevel = 2048;
 constexpr int kGridChunkRows = 4096;
 constexpr int kGridPartRecords = 65536;       // records reduced by one block
@@ -202,6 +203,7 @@ struct GridBucketPlan {
     int n_chunks;            // row chunks of kGridChunkRows
     int64_t records;         // n * levels * corners
     int64_t max_items;       // bound on (slice, part) work items
+    int64_t max_partials;    // bound on the parts of slices cut into several parts
 };
 GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n);
 struct GridBucketArgs {
@@ -219,8 +221,11 @@ struct GridBucketArgs {
     int64_t* cursor;         // [n_slices] next free record of each slice
     uint32_t* rec_idx;       // [records] entry within the slice (F != 2)
     float* rec_val;          // F == 2: [records] 16-byte {entry, g0, g1, -}; else [records][F]
-    int4* items;             // [max_items] (slice, part, parts of the slice, -)
+    int4* items;             // [max_items] (slice, part, parts of the slice, partial slot)
     int* n_items;            // [1]
+    uint32_t* lvl_max;       // [levels] bits of max |update| per level (zeroed with offs)
+    unsigned long long* partial;  // [max_partials][kGridSliceFloats] fixed-point sums of
+                                  // the parts of slices cut into several parts
 };
 void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s);
 

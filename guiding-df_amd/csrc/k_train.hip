@@ -407,12 +407,42 @@ void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s) {
 // scatter of kernel_grid_backward (grid.h:394-520) is restructured as a
 // counting sort by table slice:
 //   count   : per (row chunk, level) block, updates per slice of the level
-//   scan    : exclusive offsets in (slice, chunk) order
-//   scatter : records {entry in slice, w * dL/dy[f]} to their slice's range
-//   reduce  : one block per slice sums its records in LDS and writes the
-//             slice of the gradient once (every table entry is written, so
-//             the gradient needs no zero fill)
+//   scan    : exclusive offsets in (slice, chunk) order, the work list
+//   scatter : records {entry in slice, w * dL/dy[f]} to their slice's range,
+//             and the largest |update| of each level
+//   reduce  : one block per (slice, part) sums its records in LDS and writes
+//             the slice of the gradient once (every table entry is written, so
+//             the gradient needs no zero fill); parts of a slice cut into
+//             several parts write partial sums, which combine adds in order
+// The LDS sums are 64-bit fixed point: fp32 LDS atomics are serialised per
+// lane on gfx950 (~0.2 T updates/s device-wide) while 64-bit integer LDS
+// atomics run at ~2.4 T/s.  A slice with R records and level maximum
+// |update| < 2^e uses the scale 2^S, S = 62 - bits(R) - e, so no partial sum
+// can overflow and every update keeps ~40 or more bits below the level's
+// largest; integer sums do not depend on the order of the additions, so the
+// grid gradient is deterministic.  A level whose maximum is not finite gets
+// NaN gradients.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ long long fix64(float v, int S) {
+    return (long long)__builtin_rint(__builtin_ldexp((double)v, S));
+}
+
+__device__ __forceinline__ float unfix64(unsigned long long q, int S) {
+    return (float)__builtin_ldexp((double)(long long)q, -S);
+}
+
+// Scale exponent of slice s (level `level`); finite = false when the level's
+// largest update is inf or NaN.
+__device__ __forceinline__ int slice_scale(const GridBucketArgs& a, int level, int s, bool& finite) {
+    const float mx = __uint_as_float(a.lvl_max[level]);
+    finite = mx <= 3.402823466e+38f;
+    if (!finite || !(mx > 0.0f)) return 0;
+    int e;
+    (void)frexpf(mx, &e);  // mx < 2^e
+    const int64_t R = a.offs[s + 1] - a.offs[s];
+    const int lb = R > 0 ? 64 - __clzll((unsigned long long)R) : 0;  // R < 2^lb
+    return 62 - lb - e;
+}
 __device__ __forceinline__ int level_slices(const GridSpec& g, int l, int shift) {
     const uint32_t e = g.offset[l + 1] - g.offset[l];
     return (int)((e + (1u << shift) - 1) >> shift);
@@ -491,23 +521,28 @@ __global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArg
     if (b > n_slices) b = n_slices;
     int e = b + per;
     if (e > n_slices) e = n_slices;
-    int64_t sum = 0, np = 0;
+    int64_t sum = 0, np = 0, nm = 0;
     for (int i = b; i < e; ++i) {
         const int64_t c = a.offs[i];
         sum += c;
-        np += c > 0 ? (c + kGridPartRecords - 1) / kGridPartRecords : 1;
+        const int64_t parts = c > 0 ? (c + kGridPartRecords - 1) / kGridPartRecords : 1;
+        np += parts;
+        if (parts > 1) nm += parts;
     }
     const int64_t inc = block_scan_1024(part, sum);
     __syncthreads();
     const int64_t inc_p = block_scan_1024(part, np);
-    int64_t run = inc - sum, it = inc_p - np;
+    __syncthreads();
+    const int64_t inc_m = block_scan_1024(part, nm);
+    int64_t run = inc - sum, it = inc_p - np, slot = inc_m - nm;
     for (int i = b; i < e; ++i) {
         const int64_t c = a.offs[i];
         a.offs[i] = run;
         a.cursor[i] = run;
         run += c;
         const int parts = c > 0 ? (int)((c + kGridPartRecords - 1) / kGridPartRecords) : 1;
-        for (int p = 0; p < parts; ++p) a.items[it++] = make_int4(i, p, parts, 0);
+        for (int p = 0; p < parts; ++p) a.items[it++] = make_int4(i, p, parts, parts > 1 ? (int)(slot + p) : -1);
+        if (parts > 1) slot += parts;
     }
     if (t == 1023) {
         a.offs[n_slices] = inc;
@@ -520,18 +555,22 @@ __global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArg
 __global__ __launch_bounds__(256) void grid_bwd_scatter_kernel(const GridBucketArgs a) {
     __shared__ uint32_t fill[kGridMaxSlicesPerLevel];
     __shared__ int64_t base[kGridMaxSlicesPerLevel];
+    __shared__ uint32_t bmax;
     const GridSpec& gs = a.ns->grid;
     const int F = gs.n_feat;
     const int level = blockIdx.y;
     const int c = blockIdx.x;
     const int nsl = level_slices(gs, level, a.slice_shift);
     const int sb = level_slice_base(gs, level, a.slice_shift);
+    if (threadIdx.x == 0) bmax = 0u;
     for (int i = threadIdx.x; i < nsl; i += blockDim.x) {
         fill[i] = 0u;
         const uint32_t h = a.cnt[(size_t)(sb + i) * a.n_chunks + c];
         base[i] = h ? (int64_t)atomicAdd((unsigned long long*)&a.cursor[sb + i], (unsigned long long)h) : 0;
     }
     __syncthreads();
+    // largest |update| (bits of |v|: monotone for non-negative floats, NaN above inf)
+    uint32_t umax = 0u;
     const int64_t r0 = (int64_t)c * kGridChunkRows;
     const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
     const uint32_t emask = (1u << a.slice_shift) - 1u;
@@ -545,6 +584,7 @@ __global__ __launch_bounds__(256) void grid_bwd_scatter_kernel(const GridBucketA
             const uint32_t sl = cr.idx[0] >> a.slice_shift;
             const int64_t pos = base[sl] + atomicAdd(&fill[sl], 1u);
             put_record(a, F, pos, cr.idx[0] & emask, 1.0f, d);
+            for (int f = 0; f < F; ++f) umax = max(umax, __float_as_uint(fabsf(d[f])));
             continue;
         }
 #pragma unroll
@@ -552,67 +592,105 @@ __global__ __launch_bounds__(256) void grid_bwd_scatter_kernel(const GridBucketA
             const uint32_t sl = cr.idx[k] >> a.slice_shift;
             const int64_t pos = base[sl] + atomicAdd(&fill[sl], 1u);
             put_record(a, F, pos, cr.idx[k] & emask, cr.w[k], d);
+            for (int f = 0; f < F; ++f) umax = max(umax, __float_as_uint(fabsf(cr.w[k] * d[f])));
         }
     }
+    atomicMax(&bmax, umax);
+    __syncthreads();
+    if (threadIdx.x == 0 && bmax) atomicMax(&a.lvl_max[level], bmax);
 }
 
-// One block per (slice, part): sum the part's records into the slice in LDS
-// (8 records in flight per thread), then write the slice (single part) or add
-// it into the zero-filled gradient (several parts).
-__global__ __launch_bounds__(512) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
-    extern __shared__ float acc[];  // [entries of the slice][F]
-    if ((int)blockIdx.x >= *a.n_items) return;
-    const int4 item = a.items[blockIdx.x];
+// Level, first entry and value count of slice s.
+__device__ __forceinline__ void slice_extent(const GridBucketArgs& a, int s, int& level, uint32_t& e0, int& nf) {
     const GridSpec& gs = a.ns->grid;
-    const int F = gs.n_feat;
-    const int s = item.x;
-    int level = 0, sb = 0;
-    for (; level < gs.n_levels; ++level) {
+    int sb = 0;
+    for (level = 0; level < gs.n_levels; ++level) {
         const int k = level_slices(gs, level, a.slice_shift);
         if (s < sb + k) break;
         sb += k;
     }
     const uint32_t lvl_entries = gs.offset[level + 1] - gs.offset[level];
-    const uint32_t e0 = (uint32_t)(s - sb) << a.slice_shift;
+    e0 = (uint32_t)(s - sb) << a.slice_shift;
     const uint32_t cap = 1u << a.slice_shift;
     const uint32_t ne = lvl_entries - e0 < cap ? lvl_entries - e0 : cap;
-    const int nf = (int)ne * F;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) acc[i] = 0.0f;
+    nf = (int)ne * gs.n_feat;
+}
+
+// One block per (slice, part): sum the part's records into the slice in LDS
+// as fixed-point integers (8 records in flight per thread), then write the
+// slice (single part) or the part's partial sums (several parts).
+__global__ __launch_bounds__(512) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
+    extern __shared__ unsigned long long acc[];  // [entries of the slice][F]
+    if ((int)blockIdx.x >= *a.n_items) return;
+    const int4 item = a.items[blockIdx.x];
+    const GridSpec& gs = a.ns->grid;
+    const int F = gs.n_feat;
+    const int s = item.x;
+    int level, nf;
+    uint32_t e0;
+    slice_extent(a, s, level, e0, nf);
+    bool finite;
+    const int S = slice_scale(a, level, s, finite);
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) acc[i] = 0ull;
     __syncthreads();
     const int64_t se = a.offs[s + 1];
     const int64_t pb = a.offs[s] + (int64_t)item.y * kGridPartRecords;
     const int64_t pe = pb + kGridPartRecords < se ? pb + kGridPartRecords : se;
     constexpr int U = 8;
-    if (F == 2) {
-        const uint4* r4 = (const uint4*)a.rec_val;
-        for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
-            uint4 r[U];
+    if (finite) {
+        if (F == 2) {
+            const uint4* r4 = (const uint4*)a.rec_val;
+            for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
+                uint4 r[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                const int64_t p = p0 + (int64_t)j * blockDim.x;
-                r[j] = p < pe ? r4[p] : make_uint4(0u, 0u, 0u, 0u);
-            }
+                for (int j = 0; j < U; ++j) {
+                    const int64_t p = p0 + (int64_t)j * blockDim.x;
+                    r[j] = p < pe ? r4[p] : make_uint4(0u, 0u, 0u, 0u);
+                }
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                if (p0 + (int64_t)j * blockDim.x < pe) {
-                    atomicAdd(&acc[2 * r[j].x], __uint_as_float(r[j].y));
-                    atomicAdd(&acc[2 * r[j].x + 1], __uint_as_float(r[j].z));
+                for (int j = 0; j < U; ++j) {
+                    if (p0 + (int64_t)j * blockDim.x < pe) {
+                        atomicAdd(&acc[2 * r[j].x], (unsigned long long)fix64(__uint_as_float(r[j].y), S));
+                        atomicAdd(&acc[2 * r[j].x + 1], (unsigned long long)fix64(__uint_as_float(r[j].z), S));
+                    }
                 }
             }
-        }
-    } else {
-        for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-            const uint32_t e = a.rec_idx[p];
-            for (int f = 0; f < F; ++f) atomicAdd(&acc[e * F + f], a.rec_val[p * F + f]);
+        } else {
+            for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+                const uint32_t e = a.rec_idx[p];
+                for (int f = 0; f < F; ++f) atomicAdd(&acc[e * F + f], (unsigned long long)fix64(a.rec_val[p * F + f], S));
+            }
         }
     }
     __syncthreads();
-    float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
     if (item.z == 1) {
-        for (int i = threadIdx.x; i < nf; i += blockDim.x) g[i] = acc[i];
-    } else {  // a slice cut into parts: add into the zero-filled gradient
-        for (int i = threadIdx.x; i < nf; i += blockDim.x)
-            if (acc[i] != 0.0f) unsafeAtomicAdd(g + i, acc[i]);
+        float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) g[i] = finite ? unfix64(acc[i], S) : __builtin_nanf("");
+    } else {
+        unsigned long long* pp = a.partial + (size_t)item.w * kGridSliceFloats;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) pp[i] = acc[i];
+    }
+}
+
+// Slices cut into several parts: the first part's block adds the parts'
+// partial sums in part order and writes the slice.
+__global__ __launch_bounds__(512) void grid_bwd_combine_kernel(const GridBucketArgs a) {
+    if ((int)blockIdx.x >= *a.n_items) return;
+    const int4 item = a.items[blockIdx.x];
+    if (item.z == 1 || item.y != 0) return;
+    const GridSpec& gs = a.ns->grid;
+    const int F = gs.n_feat;
+    int level, nf;
+    uint32_t e0;
+    slice_extent(a, item.x, level, e0, nf);
+    bool finite;
+    const int S = slice_scale(a, level, item.x, finite);
+    float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
+    const unsigned long long* pp = a.partial + (size_t)item.w * kGridSliceFloats;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        unsigned long long q = 0ull;
+        for (int p = 0; p < item.z; ++p) q += pp[(size_t)p * kGridSliceFloats + i];
+        g[i] = finite ? unfix64(q, S) : __builtin_nanf("");
     }
 }
 
@@ -635,6 +713,8 @@ GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n) {
     p.n_chunks = (int)((n + kGridChunkRows - 1) / kGridChunkRows);
     p.records = n * (int64_t)g.n_levels * (g.interp == INTERP_NEAREST ? 1 : 8);
     p.max_items = p.n_slices + (p.records + kGridPartRecords - 1) / kGridPartRecords;
+    // a slice of c > P records has ceil(c / P) < 2c / P parts
+    p.max_partials = 2 * (p.records / kGridPartRecords) + 1;
     p.ok = maxl <= kGridMaxSlicesPerLevel && p.records <= ((int64_t)1 << 31) && p.n_chunks <= 65535 &&
            p.max_items < ((int64_t)1 << 30);
     return p;
@@ -643,14 +723,14 @@ GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n) {
 void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s) {
     if (a.n <= 0 || !p.ok) return;
     const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
-    launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1), s);  // int64 slice totals
+    // int64 slice totals, then the per-level maxima right after them
+    launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + DFG_MAX_LEVELS, s);
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(256), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
     hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(256), 0, s, a);
-    const int64_t ngrid = (int64_t)(hs.n_params - hs.grid.param_begin);
-    launch_fill_f32(a.grad + hs.grid.param_begin, 0.0f, ngrid, s);
-    const size_t lds = (size_t)kGridSliceFloats * sizeof(float);
+    const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
     hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(512), lds, s, a);
+    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_items), dim3(512), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

@@ -408,8 +408,10 @@ def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
 @pytest.mark.parametrize("cfgname,B", [("hash", 70000), ("hash_nearest", 5000), ("reference", 9000)])
 def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
     """The bucketed (count / scan / scatter / LDS-reduce) grid backward gives
-    the atomic scatter's table gradient up to fp32 summation order, covers
-    every table entry (untouched entries exactly 0) and leaves the MLP
+    the atomic scatter's table gradient to 1e-5 (fixed-point sums against
+    fp32 atomics in arbitrary order), covers every table entry (untouched
+    entries exactly 0; an entry may differ in being zero only where its sum is
+    below 1e-9 of the largest, the fixed-point resolution) and leaves the MLP
     gradients untouched."""
     torch = torch_cuda
     if cfgname == "reference":
@@ -433,11 +435,37 @@ def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
         gb, ga = fb.get_grad(k).astype(np.float64), fa.get_grad(k).astype(np.float64)
         nm = L["n_matrix"]
         assert np.array_equal(gb[:nm], ga[:nm]), "net %d MLP gradients differ" % k
-        assert np.array_equal(gb[nm:] == 0, ga[nm:] == 0), "net %d touched entries differ" % k
+        zb, za = gb[nm:] == 0, ga[nm:] == 0
+        tiny = 1e-9 * np.abs(ga[nm:]).max()
+        diff = zb != za
+        assert np.all(np.abs(ga[nm:][diff]) <= tiny) and np.all(np.abs(gb[nm:][diff]) <= tiny), \
+            "net %d touched entries differ (%d entries)" % (k, diff.sum())
         r = _grad_rel(gb[nm:], ga[nm:])
         assert r <= 1e-5, "net %d grid grad bucketed vs atomic rel err %.3g" % (k, r)
     fb.close()
     fa.close()
+
+
+def test_train_steps_are_deterministic(torch_cuda):
+    """Grid gradients are fixed-point sums (order-independent) and every other
+    reduction runs in a fixed order, so two fields with the same seed trained
+    on the same batches hold bit-identical weights and gradients."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=16)
+    fs = [G.GuidingField(cfg) for _ in range(2)]
+    B = 20000
+    for step in range(3):
+        feat, tgt = dev(torch, df_features(B, 80 + step)), dev(torch, train_targets(B, 90 + step))
+        for f in fs:
+            f.train_step(feat, tgt)
+    torch.cuda.synchronize()
+    for k in (0, 1):
+        assert np.array_equal(fs[0].get_grad(k), fs[1].get_grad(k)), "net %d gradients differ" % k
+        for which in (0, 1, 2):
+            assert np.array_equal(fs[0].get_params(k, which), fs[1].get_params(k, which)), \
+                "net %d params (%d) differ" % (k, which)
+    for f in fs:
+        f.close()
 
 
 def test_plain_nrc_train_step_vs_fp64(torch_cuda):

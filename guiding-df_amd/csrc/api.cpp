@@ -611,12 +611,16 @@ static const float* compact_pos(dfg_ctx* c, const float* feat, int stride, int p
     return c->pos.p;
 }
 
+// Rebuilds both interleaved tables from the nets' own (after parameters were
+// set or loaded; the training step's Adam keeps them current itself).
 void dfg::refresh_pair_table(dfg_ctx* c, hipStream_t st) {
     if (!c->pair_ok) return;
     const NetSpec& ns = c->net[0].spec;
     const int64_t entries = ns.grid.offset[ns.grid.n_levels];
     c->tab_pair.ensure((size_t)entries);
+    c->tab_pair_train.ensure((size_t)entries);
     launch_interleave_tables(c->net[0].tab_infer(), c->net[1].tab_infer(), c->tab_pair.p, entries, st);
+    launch_interleave_tables(c->net[0].tab_train(), c->net[1].tab_train(), c->tab_pair_train.p, entries, st);
 }
 
 void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
@@ -860,8 +864,10 @@ static int grid_part_index(const NetSpec& ns) {
     return -1;
 }
 
+// grid_done: s.gfeat already holds this batch's grid features (the paired
+// theta/phi pre-pass ran).
 static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int stride, const float* target,
-                              int mode, int64_t n, int64_t ld, hipStream_t st) {
+                              int mode, int64_t n, int64_t ld, hipStream_t st, bool grid_done = false) {
     TrainFwdArgs a;
     std::memset(&a, 0, sizeof(a));
     a.ns = s.d_spec.p;
@@ -881,15 +887,31 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     a.xs = s.spec.k_steps * 32 + 4;
     a.ls = s.spec.n_out_pad + 1;
     if (s.spec.grid.present) {
-        s.gfeat.ensure((size_t)s.spec.grid.n_levels * s.spec.grid.n_feat * ld);
-        const float* pos = compact_pos(c, feat, stride, grid_pos_dim(s.spec), n, st);
-        launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), pos, n, ld, s.gfeat.p, st);
+        if (!grid_done) {
+            s.gfeat.ensure((size_t)s.spec.grid.n_levels * s.spec.grid.n_feat * ld);
+            const float* pos = compact_pos(c, feat, stride, grid_pos_dim(s.spec), n, st);
+            launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), pos, n, ld, s.gfeat.p, st);
+        }
         a.gfeat = s.gfeat.p;
     }
     launch_train_fwd(a, s.spec, st);
 }
 
-static void net_update(NetState& s, float loss_scale, hipStream_t st);
+// Grid features of both DF nets for a training batch in one XCD-aware pass
+// over the interleaved training table (returns false when the grids differ).
+static bool train_grid_pair(dfg_ctx* c, const float* feat, int stride, int64_t n, int64_t ld, hipStream_t st) {
+    if (!c->pair_ok) return false;
+    NetState& th = c->net[0];
+    NetState& ph = c->net[1];
+    const size_t cols = (size_t)th.spec.grid.n_levels * 2;
+    th.gfeat.ensure(cols * ld);
+    ph.gfeat.ensure(cols * ld);
+    const float* pos = compact_pos(c, feat, stride, grid_pos_dim(th.spec), n, st);
+    launch_grid_encode_pair(th.d_spec.p, th.spec, c->tab_pair_train.p, pos, n, ld, th.gfeat.p, ph.gfeat.p, st);
+    return true;
+}
+
+static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st);
 
 static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
                                 int n_chunks, int chunk, float loss_scale, hipStream_t st) {
@@ -935,7 +957,8 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         if (F != 2) c->gb_idx.ensure((size_t)plan.records);
         c->gb_val.ensure((size_t)plan.records * (F == 2 ? 4 : F));
         c->gb_items.ensure((size_t)plan.max_items);
-        c->gb_n_items.ensure(1);
+        c->gb_n_items.ensure(2);
+        c->gb_multi.ensure((size_t)plan.max_multi);
         GridBucketArgs gb;
         std::memset(&gb, 0, sizeof(gb));
         gb.ns = s.d_spec.p;
@@ -957,6 +980,7 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         gb.n_items = c->gb_n_items.p;
         gb.lvl_max = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
         gb.partial = c->gb_part.p;
+        gb.multi = c->gb_multi.p;
         launch_grid_bwd_bucketed(gb, ns, plan, st);
     } else if (gp >= 0) {
         const size_t ngrid = (size_t)ns.n_params - ns.grid.param_begin;
@@ -973,12 +997,12 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         gb.ld = ld;
         launch_grid_bwd(gb, ns, st);
     }
-    net_update(s, loss_scale, st);
+    net_update(c, s, loss_scale, st);
 }
 
 // Adam + EMA (EmaOptimizer::step, ema.h:102-136) on the gradient in s.grad,
 // then the fragment packs of the new weights.
-static void net_update(NetState& s, float loss_scale, hipStream_t st) {
+static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st) {
     const NetSpec& ns = s.spec;
     s.step += 1;
     AdamArgs o;
@@ -988,6 +1012,15 @@ static void net_update(NetState& s, float loss_scale, hipStream_t st) {
     o.lr = s.opt.lr;
     o.beta1 = s.opt.beta1;
     o.beta2 = s.opt.beta2;
+    o.log2_beta1 = (float)std::log2((double)s.opt.beta1);
+    o.log2_beta2 = (float)std::log2((double)s.opt.beta2);
+    const int slot = (int)(&s - c->net);
+    if (c->pair_ok && slot < 2) {
+        o.pair_w16 = (_Float16*)c->tab_pair_train.p;
+        o.pair_ema16 = (_Float16*)c->tab_pair.p;
+        o.grid_begin = ns.grid.param_begin;
+        o.pair_slot = slot;
+    }
     o.eps = s.opt.eps;
     o.l2_reg = s.opt.l2;
     o.loss_scale = loss_scale;
@@ -1023,8 +1056,9 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     ensure_train_ws(th, ld, n_chunks);
     ensure_train_ws(ph, ld, n_chunks);
     ctx->loss.ensure((size_t)ld);
-    net_forward_train(ctx, th, feat, feat_stride, nullptr, 0, B, ld, st);
-    net_forward_train(ctx, ph, feat, feat_stride, target, 0, B, ld, st);
+    const bool paired = train_grid_pair(ctx, feat, feat_stride, B, ld, st);
+    net_forward_train(ctx, th, feat, feat_stride, nullptr, 0, B, ld, st, paired);
+    net_forward_train(ctx, ph, feat, feat_stride, target, 0, B, ld, st, paired);
     DfLossArgs l;
     std::memset(&l, 0, sizeof(l));
     l.target = target;
@@ -1044,7 +1078,6 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
     net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    refresh_pair_table(ctx, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
@@ -1119,8 +1152,8 @@ static void plain_net_forward(NetState& s, const float* feat, int stride, const 
 // from the loss-scaled output gradient in s.pgout: per layer the weight
 // gradient and the ReLU-masked data gradient, the grid features' gradient,
 // the grid scatter, then Adam + EMA
-static void plain_net_backward_update(NetState& s, const float* feat, int stride, int64_t n, int64_t ld, int n_chunks,
-                                      int chunk, float loss_scale, hipStream_t st) {
+static void plain_net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
+                                      int n_chunks, int chunk, float loss_scale, hipStream_t st) {
     const NetSpec& ns = s.spec;
     const int H = ns.n_hidden;
     const int gp = grid_part_index(ns);
@@ -1157,7 +1190,7 @@ static void plain_net_backward_update(NetState& s, const float* feat, int stride
         gb.ld = ld;
         launch_grid_bwd(gb, ns, st);
     }
-    net_update(s, loss_scale, st);
+    net_update(c, s, loss_scale, st);
 }
 
 int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
@@ -1183,9 +1216,8 @@ int dfg_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float* fea
     plain_net_forward(ph, feat, feat_stride, ctx->p_theta.p, B, ld, st);
     launch_plain_df_loss(target, th.pout.p, ph.pout.p, ctx->n_theta, ctx->n_phi, ctx->linear, th.pgout.p, ph.pgout.p,
                          ctx->loss.p, 1024.0f, B, ld, st);
-    plain_net_backward_update(th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    plain_net_backward_update(ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    refresh_pair_table(ctx, st);
+    plain_net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    plain_net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
@@ -1212,7 +1244,7 @@ int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float*
     ctx->loss.ensure((size_t)ld);
     plain_net_forward(s, feat, feat_stride, nullptr, B, ld, st);
     launch_plain_nrc_loss(s.pout.p, target_rgb, s.pgout.p, ctx->loss.p, 1024.0f, B, ld, st);
-    plain_net_backward_update(s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    plain_net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);

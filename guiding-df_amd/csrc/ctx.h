@@ -107,7 +107,8 @@ struct dfg_ctx {
     // theta and phi grids of identical structure (F = 2): their inference
     // tables are also kept interleaved, one 8-byte entry per vertex for both
     bool pair_ok = false;
-    DevBuf<uint2> tab_pair;
+    DevBuf<uint2> tab_pair;        // EMA (inference) weights
+    DevBuf<uint2> tab_pair_train;  // training weights (the fp16 copy of the master)
     std::string err;
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
@@ -125,13 +126,15 @@ struct dfg_ctx {
     DevBuf<int4> gb_items;
     DevBuf<int> gb_n_items;
     DevBuf<unsigned long long> gb_part;  // fixed-point partial sums of multi-part slices
+    DevBuf<int4> gb_multi;               // the multi-part slices
     ~dfg_ctx() {
         for (auto e : pipe_ev) (void)hipEventDestroy(e);
         if (pipe_start) (void)hipEventDestroy(pipe_start);
         if (side) (void)hipStreamDestroy(side);
         tab_pair.release();
+        tab_pair_train.release();
         gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
-        gb_items.release(); gb_n_items.release(); gb_part.release();
+        gb_items.release(); gb_n_items.release(); gb_part.release(); gb_multi.release();
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
         pos.release(); mean_ws.release();

@@ -204,6 +204,7 @@ struct GridBucketPlan {
     int64_t records;         // n * levels * corners
     int64_t max_items;       // bound on (slice, part) work items
     int64_t max_partials;    // bound on the parts of slices cut into several parts
+    int64_t max_multi;       // bound on the slices cut into several parts
 };
 GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n);
 struct GridBucketArgs {
@@ -222,7 +223,8 @@ struct GridBucketArgs {
     uint32_t* rec_idx;       // [records] entry within the slice (F != 2)
     float* rec_val;          // F == 2: [records] 16-byte {entry, g0, g1, -}; else [records][F]
     int4* items;             // [max_items] (slice, part, parts of the slice, partial slot)
-    int* n_items;            // [1]
+    int* n_items;            // [2]: work items, slices cut into several parts
+    int4* multi;             // [max_multi] (slice, first partial slot, parts, -)
     uint32_t* lvl_max;       // [levels] bits of max |update| per level (zeroed with offs)
     unsigned long long* partial;  // [max_partials][kGridSliceFloats] fixed-point sums of
                                   // the parts of slices cut into several parts
@@ -232,6 +234,13 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
 struct AdamArgs {
     int64_t n_params, n_matrix;
     float lr, beta1, beta2, eps, l2_reg, loss_scale;
+    float log2_beta1, log2_beta2;
+    // optional interleaved theta/phi tables (F = 2 grids of identical
+    // structure): net `pair_slot` writes its half of each entry
+    _Float16* pair_w16;
+    _Float16* pair_ema16;
+    int64_t grid_begin;
+    int pair_slot;
     float ema_decay, ema_debias_old, ema_debias_new;
     float* master;
     _Float16* w16;

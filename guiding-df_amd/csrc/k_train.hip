@@ -521,20 +521,25 @@ __global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArg
     if (b > n_slices) b = n_slices;
     int e = b + per;
     if (e > n_slices) e = n_slices;
-    int64_t sum = 0, np = 0, nm = 0;
+    int64_t sum = 0, np = 0, nm = 0, ns = 0;
     for (int i = b; i < e; ++i) {
         const int64_t c = a.offs[i];
         sum += c;
         const int64_t parts = c > 0 ? (c + kGridPartRecords - 1) / kGridPartRecords : 1;
         np += parts;
-        if (parts > 1) nm += parts;
+        if (parts > 1) {
+            nm += parts;
+            ns += 1;
+        }
     }
     const int64_t inc = block_scan_1024(part, sum);
     __syncthreads();
     const int64_t inc_p = block_scan_1024(part, np);
     __syncthreads();
     const int64_t inc_m = block_scan_1024(part, nm);
-    int64_t run = inc - sum, it = inc_p - np, slot = inc_m - nm;
+    __syncthreads();
+    const int64_t inc_s = block_scan_1024(part, ns);
+    int64_t run = inc - sum, it = inc_p - np, slot = inc_m - nm, ms = inc_s - ns;
     for (int i = b; i < e; ++i) {
         const int64_t c = a.offs[i];
         a.offs[i] = run;
@@ -542,11 +547,15 @@ __global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArg
         run += c;
         const int parts = c > 0 ? (int)((c + kGridPartRecords - 1) / kGridPartRecords) : 1;
         for (int p = 0; p < parts; ++p) a.items[it++] = make_int4(i, p, parts, parts > 1 ? (int)(slot + p) : -1);
-        if (parts > 1) slot += parts;
+        if (parts > 1) {
+            a.multi[ms++] = make_int4(i, (int)slot, parts, 0);
+            slot += parts;
+        }
     }
     if (t == 1023) {
         a.offs[n_slices] = inc;
-        *a.n_items = (int)inc_p;
+        a.n_items[0] = (int)inc_p;
+        a.n_items[1] = (int)inc_s;
     }
 }
 
@@ -672,25 +681,35 @@ __global__ __launch_bounds__(512) void grid_bwd_reduce_kernel(const GridBucketAr
     }
 }
 
-// Slices cut into several parts: the first part's block adds the parts'
-// partial sums in part order and writes the slice.
-__global__ __launch_bounds__(512) void grid_bwd_combine_kernel(const GridBucketArgs a) {
-    if ((int)blockIdx.x >= *a.n_items) return;
-    const int4 item = a.items[blockIdx.x];
-    if (item.z == 1 || item.y != 0) return;
+// Slices cut into several parts: block (k, y) adds, for multi-part slice k
+// and the y-th stripe of its values, the parts' partial sums in part order
+// and writes the gradient.
+__global__ __launch_bounds__(256) void grid_bwd_combine_kernel(const GridBucketArgs a) {
+    if ((int)blockIdx.x >= a.n_items[1]) return;
+    const int4 m = a.multi[blockIdx.x];
     const GridSpec& gs = a.ns->grid;
     const int F = gs.n_feat;
     int level, nf;
     uint32_t e0;
-    slice_extent(a, item.x, level, e0, nf);
+    slice_extent(a, m.x, level, e0, nf);
     bool finite;
-    const int S = slice_scale(a, level, item.x, finite);
+    const int S = slice_scale(a, level, m.x, finite);
     float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
-    const unsigned long long* pp = a.partial + (size_t)item.w * kGridSliceFloats;
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
-        unsigned long long q = 0ull;
-        for (int p = 0; p < item.z; ++p) q += pp[(size_t)p * kGridSliceFloats + i];
-        g[i] = finite ? unfix64(q, S) : __builtin_nanf("");
+    const unsigned long long* pp = a.partial + (size_t)m.y * kGridSliceFloats;
+    const int stripe = (nf + gridDim.y - 1) / gridDim.y;
+    const int i0 = blockIdx.y * stripe;
+    const int i1 = min(nf, i0 + stripe);
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, q3 = 0ull;
+        int p = 0;
+        for (; p + 4 <= m.z; p += 4) {
+            q0 += pp[(size_t)p * kGridSliceFloats + i];
+            q1 += pp[(size_t)(p + 1) * kGridSliceFloats + i];
+            q2 += pp[(size_t)(p + 2) * kGridSliceFloats + i];
+            q3 += pp[(size_t)(p + 3) * kGridSliceFloats + i];
+        }
+        for (; p < m.z; ++p) q0 += pp[(size_t)p * kGridSliceFloats + i];
+        g[i] = finite ? unfix64(q0 + q1 + q2 + q3, S) : __builtin_nanf("");
     }
 }
 
@@ -715,6 +734,7 @@ GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n) {
     p.max_items = p.n_slices + (p.records + kGridPartRecords - 1) / kGridPartRecords;
     // a slice of c > P records has ceil(c / P) < 2c / P parts
     p.max_partials = 2 * (p.records / kGridPartRecords) + 1;
+    p.max_multi = p.records / kGridPartRecords + 1;  // each holds more than P records
     p.ok = maxl <= kGridMaxSlicesPerLevel && p.records <= ((int64_t)1 << 31) && p.n_chunks <= 65535 &&
            p.max_items < ((int64_t)1 << 30);
     return p;
@@ -730,7 +750,7 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(256), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
     hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(512), lds, s, a);
-    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_items), dim3(512), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -752,16 +772,27 @@ __global__ void adam_ema_kernel(const AdamArgs a) {
         a.v[i] = v;
         const uint32_t step = a.steps[i] + 1;
         a.steps[i] = step;
+        // bias corrections beta^step as 2^(step log2 beta) (one hardware exp2
+        // each instead of two powf per parameter)
         float lr = a.lr;
-        lr *= sqrtf(1.0f - powf(a.beta2, (float)step)) / (1.0f - powf(a.beta1, (float)step));
+        lr *= sqrtf(1.0f - exp2f((float)step * a.log2_beta2)) / (1.0f - exp2f((float)step * a.log2_beta1));
         const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
         const float nw = wf - elr * m;
         a.master[i] = nw;
         a.w16[i] = (_Float16)nw;
     }
     const float e = (float)a.ema16[i];
-    const float w = (float)a.w16[i];
-    a.ema16[i] = (_Float16)((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
+    const _Float16 w16 = a.w16[i];
+    const float w = (float)w16;
+    const _Float16 ne = (_Float16)((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
+    a.ema16[i] = ne;
+    if (a.pair_w16 && i >= a.grid_begin) {
+        // the interleaved theta/phi tables: entry {theta f0, f1, phi f0, f1}
+        const int64_t l = i - a.grid_begin;
+        const int64_t h = (l >> 1) * 4 + a.pair_slot * 2 + (l & 1);
+        a.pair_w16[h] = w16;
+        a.pair_ema16[h] = ne;
+    }
 }
 
 void launch_adam_ema(const AdamArgs& a, hipStream_t s) {

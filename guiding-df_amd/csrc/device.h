@@ -56,8 +56,10 @@ __device__ inline float det_expf(float x) {
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
     const float e = __builtin_ldexpf(p, (int)k);
-    if (x != x) return x;
-    return x > 88.72283935546875f ? __builtin_inff() : (x < -103.972084045410156f ? 0.0f : e);
+    // guards as selects (no branch per call): NaN passes through, overflow
+    // gives +inf, underflow below the smallest subnormal gives 0
+    const float g = x > 88.72283935546875f ? __builtin_inff() : (x < -103.972084045410156f ? 0.0f : e);
+    return x != x ? x : g;
 }
 
 __device__ inline float det_logf(float x) {
@@ -510,6 +512,8 @@ struct Segment {
 template <int N>
 __device__ __forceinline__ Segment search_linear(const float (&p)[N], float u) {
 #pragma clang fp contract(off)
+    // the first segment whose end exceeds u, tracked with selects (the same
+    // result as the reference's early-exit loop, without a branch per bin)
     Segment s;
     float c = p[0] * 0.5f / (float)N;
     bool found = u < c;
@@ -518,10 +522,17 @@ __device__ __forceinline__ Segment search_linear(const float (&p)[N], float u) {
 #pragma unroll
     for (int i = 1; i < N; ++i) {
         c = prev + (p[i - 1] + p[i]) / (float)(2 * N);
-        if (!found && u < c) { found = true; s.i = i; s.c_lo = prev; s.c_hi = c; }
+        const bool hit = (u < c) & !found;
+        s.i = hit ? i : s.i;
+        s.c_lo = hit ? prev : s.c_lo;
+        s.c_hi = hit ? c : s.c_hi;
+        found = found | hit;
         prev = c;
     }
-    if (!found) { s.i = N; s.c_lo = prev; s.c_hi = prev + p[N - 1] * 0.5f / (float)N; }
+    const float end = prev + p[N - 1] * 0.5f / (float)N;
+    s.i = found ? s.i : N;
+    s.c_lo = found ? s.c_lo : prev;
+    s.c_hi = found ? s.c_hi : end;
     return s;
 }
 
@@ -532,14 +543,19 @@ __device__ __forceinline__ Segment search_nearest(const float (&p)[N], float u) 
     Segment s;
     float prev = 0.0f, c = 0.0f;
     bool found = false;
-    s.i = N;
+    s.i = N; s.c_lo = 0.0f; s.c_hi = 0.0f;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         c = prev + p[i] / (float)N;
-        if (!found && u < c) { found = true; s.i = i; s.c_lo = prev; s.c_hi = c; }
+        const bool hit = (u < c) & !found;
+        s.i = hit ? i : s.i;
+        s.c_lo = hit ? prev : s.c_lo;
+        s.c_hi = hit ? c : s.c_hi;
+        found = found | hit;
         prev = c;
     }
-    if (!found) { s.c_lo = prev; s.c_hi = c; }
+    s.c_lo = found ? s.c_lo : prev;
+    s.c_hi = found ? s.c_hi : c;
     return s;
 }
 

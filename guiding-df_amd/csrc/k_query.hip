@@ -72,7 +72,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
-    mlp_forward_tile<NTH / 16, true>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
+    mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1});
     if (a.logits_th && valid) {
 #pragma unroll
@@ -102,7 +102,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
-    mlp_forward_tile<NPH / 16, true>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
+    mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1});
     if (a.logits_ph && valid) {
 #pragma unroll
@@ -139,7 +139,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
 }
 
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
-__global__ __launch_bounds__(64, 3) void query_fused_kernel(const QueryArgs a) {
+__global__ __launch_bounds__(64, 2) void query_fused_kernel(const QueryArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const RowsIO io{a.feat, a.stride, a.u, a.guided, a.wi, a.rgb, a.n, 0};
     query_tile<NTH, NPH, WITH_DF, WITH_NRC>(a, io, (int64_t)blockIdx.x * 64, smem);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(64, 3) void query_fused_kernel(const QueryArgs a) {
 // the table is exhausted.  Tiles never straddle segments, and the per-row math
 // is the one of query_fused_kernel, so results do not depend on the batching.
 template <int NTH, int NPH, bool WITH_NRC>
-__global__ __launch_bounds__(64) void query_worklist_kernel(const QueryArgs a, const WorkList* __restrict__ wl,
+__global__ __launch_bounds__(64, 2) void query_worklist_kernel(const QueryArgs a, const WorkList* __restrict__ wl,
                                                             unsigned int* __restrict__ counter, int tiles_per_fetch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;

@@ -468,7 +468,7 @@ __device__ __forceinline__ void put_record(const GridBucketArgs& a, int F, int64
 
 // Per (row chunk, level) block: updates per slice of the level, kept per
 // (slice, chunk) and added to the slice totals.
-__global__ __launch_bounds__(256) void grid_bwd_count_kernel(const GridBucketArgs a) {
+__global__ __launch_bounds__(1024) void grid_bwd_count_kernel(const GridBucketArgs a) {
     __shared__ uint32_t hist[kGridMaxSlicesPerLevel];
     const GridSpec& gs = a.ns->grid;
     const int level = blockIdx.y;
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scan_kernel(const GridBucketArg
 
 // Per (row chunk, level) block: reserve this block's range of every slice,
 // then write one record per (row, corner).
-__global__ __launch_bounds__(256) void grid_bwd_scatter_kernel(const GridBucketArgs a) {
+__global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucketArgs a) {
     __shared__ uint32_t fill[kGridMaxSlicesPerLevel];
     __shared__ int64_t base[kGridMaxSlicesPerLevel];
     __shared__ uint32_t bmax;
@@ -628,7 +628,7 @@ __device__ __forceinline__ void slice_extent(const GridBucketArgs& a, int s, int
 // One block per (slice, part): sum the part's records into the slice in LDS
 // as fixed-point integers (8 records in flight per thread), then write the
 // slice (single part) or the part's partial sums (several parts).
-__global__ __launch_bounds__(512) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
+__global__ __launch_bounds__(1024) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
     extern __shared__ unsigned long long acc[];  // [entries of the slice][F]
     if ((int)blockIdx.x >= *a.n_items) return;
     const int4 item = a.items[blockIdx.x];
@@ -745,11 +745,11 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
     // int64 slice totals, then the per-level maxima right after them
     launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + DFG_MAX_LEVELS, s);
-    hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
-    hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
-    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(512), lds, s, a);
+    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
     hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
 }
 

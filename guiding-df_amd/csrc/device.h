@@ -204,6 +204,36 @@ __device__ __forceinline__ void grid_corners(const GridSpec& g, int level, float
     }
 }
 
+// Corners 2p and 2p + 1 (the x pair of the p-th (y, z) combination) of one
+// level, by the same operations as grid_corners (identical indices and weights).
+__device__ __forceinline__ void grid_corner_pair(const GridSpec& g, int level, float px, float py, float pz, int p,
+                                                 uint32_t (&idx)[2], float (&w)[2]) {
+#pragma clang fp contract(off)
+    const float scale = g.scale[level];
+    const uint32_t hsize = g.offset[level + 1] - g.offset[level];
+    float fx, fy, fz;
+    uint32_t cx, cy, cz;
+    pos_fract(px, scale, g.interp, fx, cx);
+    pos_fract(py, scale, g.interp, fy, cy);
+    pos_fract(pz, scale, g.interp, fz, cz);
+    const bool hashed = g.hashed[level] & 1u;
+    const uint32_t yy = cy + (uint32_t)(p & 1), zz = cz + (uint32_t)(p >> 1);
+    const uint32_t ty = hashed ? yy * 2654435761u : yy * g.s1[level];
+    const uint32_t tz = hashed ? zz * 805459861u : zz * g.s2[level];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int k = 2 * p + j;
+        float ww = 1.0f;
+        if (k & 1) { ww *= fx; } else { ww *= 1.0f - fx; }
+        if (k & 2) { ww *= fy; } else { ww *= 1.0f - fy; }
+        if (k & 4) { ww *= fz; } else { ww *= 1.0f - fz; }
+        const uint32_t x = cx + (uint32_t)j;
+        const uint32_t n = hashed ? (x ^ (ty ^ tz)) : (x + (ty + tz));
+        idx[j] = level_mod(g, level, n, hsize);
+        w[j] = ww;
+    }
+}
+
 // Grid level value for one row (N-linear / smoothstep / nearest).
 template <int F>
 __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __restrict__ table, int level,

@@ -609,6 +609,99 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
     if (threadIdx.x == 0 && bmax) atomicMax(&a.lvl_max[level], bmax);
 }
 
+// Staged scatter (F = 2, 8 corners): the records of 256 rows at a time (four
+// threads per row, one x pair of corners each) are sorted by slice in LDS and
+// then written as one contiguous run per slice, instead of one scattered
+// 16-byte store per record.  The same records land in the same slice ranges
+// (only their order inside a range differs, which the order-independent
+// fixed-point reduce does not see).
+constexpr int kStageRows = 256;
+__global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
+    __shared__ int64_t base[kGridMaxSlicesPerLevel];
+    __shared__ uint32_t fill[kGridMaxSlicesPerLevel];
+    __shared__ uint32_t lcnt[kGridMaxSlicesPerLevel];
+    __shared__ uint32_t loff[kGridMaxSlicesPerLevel];
+    __shared__ uint4 stage[kStageRows * 8];
+    __shared__ uint32_t bmax;
+    const GridSpec& gs = a.ns->grid;
+    const int level = blockIdx.y;
+    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    const int nsl = level_slices(gs, level, a.slice_shift);
+    const int sb = level_slice_base(gs, level, a.slice_shift);
+    if (t == 0) bmax = 0u;
+    for (int i = t; i < nsl; i += blockDim.x) {
+        fill[i] = 0u;
+        const uint32_t h = a.cnt[(size_t)(sb + i) * a.n_chunks + c];
+        base[i] = h ? (int64_t)atomicAdd((unsigned long long*)&a.cursor[sb + i], (unsigned long long)h) : 0;
+    }
+    const int64_t r0 = (int64_t)c * kGridChunkRows;
+    const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
+    const uint32_t emask = (1u << a.slice_shift) - 1u;
+    uint32_t umax = 0u;
+    for (int64_t s0 = r0; s0 < r1; s0 += kStageRows) {
+        for (int i = t; i < nsl; i += blockDim.x) lcnt[i] = 0u;
+        __syncthreads();
+        const int64_t row = s0 + (t >> 2);
+        const bool live = row < r1;
+        uint32_t sl[2] = {0u, 0u}, lp[2] = {0u, 0u};
+        uint4 rec[2];
+        if (live) {
+            const float d0 = a.dx[(size_t)(level * 2) * a.ld + row];
+            const float d1 = a.dx[(size_t)(level * 2 + 1) * a.ld + row];
+            const float* fr = a.feat + row * a.stride + a.pos_dim;
+            uint32_t idx[2];
+            float w[2];
+            grid_corner_pair(gs, level, fr[0], fr[1], fr[2], t & 3, idx, w);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                sl[j] = idx[j] >> a.slice_shift;
+                const float v0 = w[j] * d0, v1 = w[j] * d1;
+                rec[j] = make_uint4(idx[j] & emask, __float_as_uint(v0), __float_as_uint(v1), sl[j]);
+                umax = max(umax, max(__float_as_uint(fabsf(v0)), __float_as_uint(fabsf(v1))));
+                lp[j] = atomicAdd(&lcnt[sl[j]], 1u);
+            }
+        }
+        __syncthreads();
+        // exclusive offsets of this sub-chunk's slice runs, by one wave
+        if (t < 64) {
+            const int per = (nsl + 63) / 64;
+            const int b = t * per < nsl ? t * per : nsl;
+            const int e = b + per < nsl ? b + per : nsl;
+            uint32_t sum = 0u;
+            for (int i = b; i < e; ++i) sum += lcnt[i];
+            uint32_t inc = sum;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d, 64);
+                if (t >= d) inc += y;
+            }
+            uint32_t run = inc - sum;
+            for (int i = b; i < e; ++i) {
+                loff[i] = run;
+                run += lcnt[i];
+            }
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) stage[loff[sl[j]] + lp[j]] = rec[j];
+        }
+        __syncthreads();
+        const int nrec = (int)((r1 - s0 < kStageRows ? r1 - s0 : kStageRows) * 8);
+        for (int i = t; i < nrec; i += blockDim.x) {
+            const uint4 r = stage[i];
+            const int64_t pos = base[r.w] + (int64_t)fill[r.w] + (int64_t)(i - (int)loff[r.w]);
+            ((uint4*)a.rec_val)[pos] = make_uint4(r.x, r.y, r.z, 0u);
+        }
+        __syncthreads();
+        for (int i = t; i < nsl; i += blockDim.x) fill[i] += lcnt[i];
+    }
+    atomicMax(&bmax, umax);
+    __syncthreads();
+    if (t == 0 && bmax) atomicMax(&a.lvl_max[level], bmax);
+}
+
 // Level, first entry and value count of slice s.
 __device__ __forceinline__ void slice_extent(const GridBucketArgs& a, int s, int& level, uint32_t& e0, int& nf) {
     const GridSpec& gs = a.ns->grid;
@@ -747,7 +840,10 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + DFG_MAX_LEVELS, s);
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
-    hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(1024), 0, s, a);
+    if (hs.grid.n_feat == 2 && hs.grid.interp != INTERP_NEAREST)
+        hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel, g1, dim3(1024), 0, s, a);
+    else
+        hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
     hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
     hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);

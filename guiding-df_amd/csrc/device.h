@@ -45,7 +45,7 @@ __device__ inline float det_expf(float x) {
     // defined (outside it the guards below decide); r = x - k ln2 with ln2
     // split hi + lo (k * hi is exact); Taylor through r^7 by explicit FMAs;
     // scaling by 2^k with one ldexp (correctly rounded, subnormals included).
-    const float k = fminf(fmaxf(__builtin_rintf(x * 1.44269502162933349609f), -200.0f), 200.0f);
+    const float k = __builtin_amdgcn_fmed3f(__builtin_rintf(x * 1.44269502162933349609f), -200.0f, 200.0f);
     const float r = __builtin_fmaf(-k, 1.42860676533018704504e-06f, __builtin_fmaf(-k, 0.693145751953125f, x));
     float p = 1.98412698412698412698e-04f;
     p = __builtin_fmaf(p, r, 1.38888888888888888889e-03f);
@@ -107,7 +107,7 @@ __device__ __forceinline__ float quartic_cdf(float x, float inv_radius) {
     const float u2 = u * u;
     const float u4 = u2 * u2;
     const float poly = __builtin_fmaf(0.200000002980232238770f, u4, __builtin_fmaf(-0.666666686534881591797f, u2, 1.0f));
-    return fmaxf(0.0f, fminf(1.0f, __builtin_fmaf(0.9375f * u, poly, 0.5f)));
+    return __builtin_amdgcn_fmed3f(__builtin_fmaf(0.9375f * u, poly, 0.5f), 0.0f, 1.0f);
 }
 
 // grid_index with the CoherentPrime hash, tcnn grid.h:111-128, :164-186.
@@ -406,7 +406,7 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
                     auto S = [&](float b) {
                         const float t = b - x;
                         if (nb < 2) return quartic_cdf(t, fnb) + quartic_cdf(t - 1.0f, fnb) + quartic_cdf(t + 1.0f, fnb);
-                        const float r = fminf(fmaxf(__builtin_rintf(t), -1.0f), 1.0f);
+                        const float r = __builtin_amdgcn_fmed3f(__builtin_rintf(t), -1.0f, 1.0f);
                         return (1.0f + r) + quartic_cdf(t - r, fnb);
                     };
                     float left = S(0.0f);
@@ -733,12 +733,19 @@ __device__ __forceinline__ _Float16 sat_h(float v) {
     return (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f);
 }
 
+// max(v, 0) as one integer max on the bit pattern: non-negative floats order
+// like their bits, negative ones (and -0) have the sign bit and map to +0.
+// (fmaxf costs a canonicalisation on top in IEEE mode.)
+__device__ __forceinline__ float relu(float v) {
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int, v), 0));
+}
+
 // relu(C) of 4 M-tiles (64 neurons) -> the two K-step operands of the next layer.
 __device__ __forceinline__ void relu_to_act(const f4 (&c)[4], h8 (&a)[2]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[s][j] = (_Float16)fmaxf(c[2 * s + (j >> 2)][j & 3], 0.0f);
+        for (int j = 0; j < 8; ++j) a[s][j] = (_Float16)relu(c[2 * s + (j >> 2)][j & 3]);
 }
 
 __device__ __forceinline__ void c_to_act(const f4 (&c)[4], h8 (&a)[2]) {
@@ -785,7 +792,7 @@ __device__ __forceinline__ void relu_to_act2(const f4 (&c)[4], h8 (&hi)[2], h8 (
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             _Float16 h, l;
-            split_h(fmaxf(c[2 * s + (j >> 2)][j & 3], 0.0f), h, l);
+            split_h(relu(c[2 * s + (j >> 2)][j & 3]), h, l);
             hi[s][j] = h;
             lo[s][j] = l;
         }

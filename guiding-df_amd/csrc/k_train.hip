@@ -22,7 +22,7 @@ struct RowInT {
 //  NRC: the raw network output).
 // ---------------------------------------------------------------------------
 template <int NT_OUT, int NBINS>
-__global__ __launch_bounds__(64) void train_fwd_kernel(const TrainFwdArgs a) {
+__global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* xt = (float*)smem;
     float* lo = (float*)smem;

@@ -49,16 +49,27 @@ __global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) 
     mlp_forward_tile<NT_OUT, false>(ns, a.pack, xt, a.xs, lo, a.ls, &st);
     const float* lrow = lo + lane * a.ls;
     _Float16* o = a.out + row * (NT_OUT * 16);
+    // row-major fp16 output, written 8 halves (16 bytes) at a time
     if (a.mode == 0) {
         float z[NBINS];
 #pragma unroll
         for (int i = 0; i < NBINS; ++i) z[i] = lrow[i];
         softmax_times_n_half<NBINS>(z);
 #pragma unroll
-        for (int i = 0; i < NBINS; ++i) o[i] = (_Float16)z[i];
+        for (int v = 0; v < NT_OUT * 2; ++v) {
+            h8 hv;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) hv[j] = v * 8 + j < NBINS ? (_Float16)z[v * 8 + j < NBINS ? v * 8 + j : 0] : (_Float16)0.0f;
+            *(h8*)(o + v * 8) = hv;
+        }
     } else {
 #pragma unroll
-        for (int i = 0; i < NT_OUT * 16; ++i) o[i] = (_Float16)lrow[i];
+        for (int v = 0; v < NT_OUT * 2; ++v) {
+            h8 hv;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) hv[j] = (_Float16)lrow[v * 8 + j];
+            *(h8*)(o + v * 8) = hv;
+        }
     }
 }
 
@@ -90,11 +101,18 @@ __device__ __forceinline__ void df_grad(float x, float pdf_other, float prefix, 
     const float multiplier = prefix * pdf_other * (float)N;
     float g[N];
     df_logit_grad<N>(x, multiplier, linear, [&](int k) { return (float)bins[k] * inv_res; }, g);
+    // the row-major copy as 16-byte stores (8 halves), the feature-major one
+    // element by element (consecutive rows across the lanes)
 #pragma unroll
-    for (int i = 0; i < NPAD; ++i) {
-        const _Float16 h = i < N ? sat_h(g[i < N ? i : 0]) : (_Float16)0.0f;
-        rm[row * NPAD + i] = h;
-        fm[(size_t)i * ld + row] = h;
+    for (int v = 0; v < NPAD / 8; ++v) {
+        h8 hv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = v * 8 + j;
+            hv[j] = i < N ? sat_h(g[i < N ? i : 0]) : (_Float16)0.0f;
+            fm[(size_t)i * ld + row] = hv[j];
+        }
+        *(h8*)(rm + row * NPAD + v * 8) = hv;
     }
 }
 
@@ -104,8 +122,10 @@ __global__ void df_loss_kernel(const DfLossArgs a) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= a.ld) return;
     if (row >= a.n) {
-        for (int i = 0; i < NTH; ++i) { a.gout_th_rm[row * NTH + i] = (_Float16)0.0f; a.gout_th_fm[(size_t)i * a.ld + row] = (_Float16)0.0f; }
-        for (int i = 0; i < NPH; ++i) { a.gout_ph_rm[row * NPH + i] = (_Float16)0.0f; a.gout_ph_fm[(size_t)i * a.ld + row] = (_Float16)0.0f; }
+        for (int i = 0; i < NTH; i += 8) *(h8*)(a.gout_th_rm + row * NTH + i) = h8{};
+        for (int i = 0; i < NPH; i += 8) *(h8*)(a.gout_ph_rm + row * NPH + i) = h8{};
+        for (int i = 0; i < NTH; ++i) a.gout_th_fm[(size_t)i * a.ld + row] = (_Float16)0.0f;
+        for (int i = 0; i < NPH; ++i) a.gout_ph_fm[(size_t)i * a.ld + row] = (_Float16)0.0f;
         return;
     }
     const float* t = a.target + row * 9;
@@ -146,7 +166,9 @@ __global__ void nrc_loss_kernel(const NrcLossArgs a) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= a.ld) return;
     if (row >= a.n) {
-        for (int c = 0; c < 16; ++c) { a.gout_rm[row * 16 + c] = (_Float16)0.0f; a.gout_fm[(size_t)c * a.ld + row] = (_Float16)0.0f; }
+        *(h8*)(a.gout_rm + row * 16) = h8{};
+        *(h8*)(a.gout_rm + row * 16 + 8) = h8{};
+        for (int c = 0; c < 16; ++c) a.gout_fm[(size_t)c * a.ld + row] = (_Float16)0.0f;
         return;
     }
     const _Float16* p = a.pred + row * 16;
@@ -155,6 +177,7 @@ __global__ void nrc_loss_kernel(const NrcLossArgs a) {
     const float den = lum * lum + 0.01f;
     const float n_total = (float)(a.n * 3);
     float lsum = 0.0f;
+    h8 g0 = h8{};
     for (int c = 0; c < 16; ++c) {
         _Float16 gh = (_Float16)0.0f;
         if (c < 3) {
@@ -162,10 +185,12 @@ __global__ void nrc_loss_kernel(const NrcLossArgs a) {
             lsum += diff * diff / den / 1.0f / n_total;
             const float grad = 2.0f * diff / den / 1.0f;
             gh = sat_h(a.loss_scale * grad / n_total);
+            g0[c] = gh;
         }
-        a.gout_rm[row * 16 + c] = gh;
         a.gout_fm[(size_t)c * a.ld + row] = gh;
     }
+    *(h8*)(a.gout_rm + row * 16) = g0;
+    *(h8*)(a.gout_rm + row * 16 + 8) = h8{};
     a.loss[row] = lsum;
 }
 

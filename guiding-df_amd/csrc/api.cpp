@@ -955,7 +955,7 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         c->gb_cursor.ensure((size_t)plan.n_slices);
         c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
         if (F != 2) c->gb_idx.ensure((size_t)plan.records);
-        c->gb_val.ensure((size_t)plan.records * (F == 2 ? 4 : F));
+        c->gb_val.ensure((size_t)plan.records * (F == 2 ? 3 : F));
         c->gb_items.ensure((size_t)plan.max_items);
         c->gb_n_items.ensure(2);
         c->gb_multi.ensure((size_t)plan.max_multi);

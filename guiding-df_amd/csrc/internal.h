@@ -221,7 +221,7 @@ struct GridBucketArgs {
     int64_t* offs;           // [n_slices + 1]: slice totals, then exclusive offsets
     int64_t* cursor;         // [n_slices] next free record of each slice
     uint32_t* rec_idx;       // [records] entry within the slice (F != 2)
-    float* rec_val;          // F == 2: [records] 16-byte {entry, g0, g1, -}; else [records][F]
+    float* rec_val;          // F == 2: [records] 12-byte {entry, g0, g1}; else [records][F]
     int4* items;             // [max_items] (slice, part, parts of the slice, partial slot)
     int* n_items;            // [2]: work items, slices cut into several parts
     int4* multi;             // [max_multi] (slice, first partial slot, parts, -)

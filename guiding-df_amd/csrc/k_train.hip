@@ -479,12 +479,12 @@ __device__ __forceinline__ int level_slice_base(const GridSpec& g, int l, int sh
     return b;
 }
 
-// Records: F == 2 as one 16-byte {entry, g0, g1, -} store; other F as an
+// Records: F == 2 as one 12-byte {entry, g0, g1} store; other F as an
 // entry array plus an [F] value array.
 __device__ __forceinline__ void put_record(const GridBucketArgs& a, int F, int64_t pos, uint32_t e, float w,
                                            const float (&d)[8]) {
     if (F == 2) {
-        ((uint4*)a.rec_val)[pos] = make_uint4(e, __float_as_uint(w * d[0]), __float_as_uint(w * d[1]), 0u);
+        ((uint3*)a.rec_val)[pos] = make_uint3(e, __float_as_uint(w * d[0]), __float_as_uint(w * d[1]));
     } else {
         a.rec_idx[pos] = e;
         for (int f = 0; f < F; ++f) a.rec_val[pos * F + f] = w * d[f];
@@ -637,7 +637,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 // Staged scatter (F = 2, 8 corners): the records of 256 rows at a time (four
 // threads per row, one x pair of corners each) are sorted by slice in LDS and
 // then written as one contiguous run per slice, instead of one scattered
-// 16-byte store per record.  The same records land in the same slice ranges
+// 12-byte store per record.  The same records land in the same slice ranges
 // (only their order inside a range differs, which the order-independent
 // fixed-point reduce does not see).
 constexpr int kStageRows = 256;
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         for (int i = t; i < nrec; i += blockDim.x) {
             const uint4 r = stage[i];
             const int64_t pos = base[r.w] + (int64_t)fill[r.w] + (int64_t)(i - (int)loff[r.w]);
-            ((uint4*)a.rec_val)[pos] = make_uint4(r.x, r.y, r.z, 0u);
+            ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
         }
         __syncthreads();
         for (int i = t; i < nsl; i += blockDim.x) fill[i] += lcnt[i];
@@ -766,13 +766,13 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_kernel(const GridBucketA
     constexpr int U = 8;
     if (finite) {
         if (F == 2) {
-            const uint4* r4 = (const uint4*)a.rec_val;
+            const uint3* r3 = (const uint3*)a.rec_val;
             for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
-                uint4 r[U];
+                uint3 r[U];
 #pragma unroll
                 for (int j = 0; j < U; ++j) {
                     const int64_t p = p0 + (int64_t)j * blockDim.x;
-                    r[j] = p < pe ? r4[p] : make_uint4(0u, 0u, 0u, 0u);
+                    r[j] = p < pe ? r3[p] : make_uint3(0u, 0u, 0u);
                 }
 #pragma unroll
                 for (int j = 0; j < U; ++j) {

@@ -254,6 +254,35 @@ def test_grid_prepass_matches_in_kernel_gathers(torch_cuda):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("chunks,bpc", [(4, 0), (3, 2)])
+def test_chunked_query_matches_unchunked(torch_cuda, chunks, bpc):
+    """The chunked query (pre-pass of each row chunk on a side stream, optionally
+    a persistent pre-pass grid) gives the unchunked query's results bit for bit,
+    for sampling with NRC and for given directions."""
+    torch = torch_cuda
+    cfgs = []
+    for ch in (1, chunks):
+        c = G.hash_config(with_nrc=True)
+        c["dfg"].update({"query_chunks": ch, "prepass_blocks_per_cu": bpc, "pipeline_min_rows": 1024,
+                         "grid_prepass_min_rows": 1024})
+        cfgs.append(G.GuidingField(c))
+    n = 5000
+    feat, u = dev(torch, df_features(n, 72, nrc=True)), dev(torch, uniforms(n, 73))
+    guided = dev(torch, (np.random.default_rng(74).random(n) < 0.5).astype(np.uint8))
+    outs = []
+    for f in cfgs:
+        w = dev(torch, wi_init(n))
+        rgb = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        f.query(feat, u, w, guided=guided, nrc_rgb=rgb)
+        d_pdf = torch.zeros(n, dtype=torch.float32, device="cuda")
+        f.pdf(feat, w[:, :2].contiguous(), d_pdf)
+        torch.cuda.synchronize()
+        outs.append((host(w), host(rgb), host(d_pdf)))
+        f.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_pdf_entry(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=False)

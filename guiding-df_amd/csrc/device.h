@@ -890,13 +890,29 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                     a1[(size_t)n * st->ld + st->row0 + (q0 + q) * 16 + i16] = act[q][s][j];
                 }
     }
-    // hidden layers 1..H-1
-    for (int l = 1; l < ns.n_hidden; ++l) {
+    // hidden layers 1..H-1; the weight fragments of layer l + 1 are loaded
+    // while layer l computes (they do not depend on the activations)
+    h8 wf[4][2];
+    auto load_layer = [&](int l, h8 (&w)[4][2]) {
         const _Float16* pk = fwd_pack + ns.fwd_off[l];
+        const int nt = l == ns.n_hidden ? NT_OUT : 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t < nt) {
+                w[t][0] = ld_frag(pk, t, 0, 2, lane);
+                w[t][1] = ld_frag(pk, t, 1, 2, lane);
+            }
+    };
+    load_layer(1 < ns.n_hidden ? 1 : ns.n_hidden, wf);
+    for (int l = 1; l < ns.n_hidden; ++l) {
+        h8 wc[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) { wc[t][0] = wf[t][0]; wc[t][1] = wf[t][1]; }
+        load_layer(l + 1, wf);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const h8 a0 = ld_frag(pk, t, 0, 2, lane);
-            const h8 a1 = ld_frag(pk, t, 1, 2, lane);
+            const h8 a0 = wc[t][0];
+            const h8 a1 = wc[t][1];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
@@ -926,13 +942,12 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                     }
         }
     }
-    // output layer
+    // output layer (its fragments are in wf)
     {
-        const _Float16* pk = fwd_pack + ns.fwd_off[ns.n_hidden];
 #pragma unroll
         for (int t = 0; t < NT_OUT; ++t) {
-            const h8 a0 = ld_frag(pk, t, 0, 2, lane);
-            const h8 a1 = ld_frag(pk, t, 1, 2, lane);
+            const h8 a0 = wf[t][0];
+            const h8 a1 = wf[t][1];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 f4 c = mfma16(a0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});

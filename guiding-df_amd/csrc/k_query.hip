@@ -129,7 +129,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         _Float16* const xhrow = xh + lane * a.xs_nrc;
         encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; },
                    nullptr, 0, false, false);
-        mlp_forward_tile<1, false, 2>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
+        mlp_forward_tile<1, false, 4>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
         if (valid && io.rgb) {
             io.rgb[3 * r + 0] = lrow[0];
             io.rgb[3 * r + 1] = lrow[1];

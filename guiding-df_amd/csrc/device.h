@@ -62,6 +62,33 @@ __device__ inline float det_expf(float x) {
     return x != x ? x : g;
 }
 
+// det_expf for x <= 0 or NaN (the softmax's z - max): the same value, with
+// only the underflow guard (a NaN argument already yields NaN through the
+// polynomial and ldexp, and no argument can overflow).
+__device__ __forceinline__ float det_expf_nonpos(float x) {
+#pragma clang fp contract(off)
+    const float k = __builtin_amdgcn_fmed3f(__builtin_rintf(x * 1.44269502162933349609f), -200.0f, 200.0f);
+    const float r = __builtin_fmaf(-k, 1.42860676533018704504e-06f, __builtin_fmaf(-k, 0.693145751953125f, x));
+    float p = 1.98412698412698412698e-04f;
+    p = __builtin_fmaf(p, r, 1.38888888888888888889e-03f);
+    p = __builtin_fmaf(p, r, 8.33333333333333333333e-03f);
+    p = __builtin_fmaf(p, r, 4.16666666666666666667e-02f);
+    p = __builtin_fmaf(p, r, 1.66666666666666666667e-01f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const float e = __builtin_ldexpf(p, (int)k);
+    return x < -103.972084045410156f ? 0.0f : e;
+}
+
+// max(m, z) for the softmax's running maximum, which starts at 0: as signed
+// integers the bit patterns of non-negative floats order like the floats and
+// those of negative floats fall below 0, so the maximum over {0} and z is the
+// integer maximum (fmaxf costs a canonicalisation on top in IEEE mode).
+__device__ __forceinline__ float max_from_zero(float m, float z) {
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int, m), __builtin_bit_cast(int, z)));
+}
+
 __device__ inline float det_logf(float x) {
 #pragma clang fp contract(off)
     if (x != x || x < 0.0f) return __builtin_nanf("");
@@ -503,11 +530,11 @@ __device__ __forceinline__ void softmax_times_n(float (&z)[N]) {
 #pragma clang fp contract(off)
     float mx = 0.0f;
 #pragma unroll
-    for (int i = 0; i < N; ++i) mx = fmaxf(mx, z[i]);
+    for (int i = 0; i < N; ++i) mx = max_from_zero(mx, z[i]);
     float s = 0.0f;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        z[i] = det_expf(z[i] - mx);
+        z[i] = det_expf_nonpos(z[i] - mx);
         s += z[i];
     }
     const float scale = (float)N / s;

@@ -562,3 +562,34 @@ def test_scatter_mixture_closed_forms():
     bv0 = bv.copy(); bv0[:5] = 0
     t, _, _, alive = O.scatter_f64(0.5, wi, ones, None, pb, bv0, c, thp)
     assert not alive[:5].any() and np.array_equal(t[:5], thp[:5])
+
+
+def _slice_scale(mx, R):
+    """The grid-gradient fixed-point scale exponent (k_train.hip slice_scale):
+    S = 62 - bits(R) - e with mx < 2^e (frexp) and R < 2^bits(R)."""
+    if not mx > 0:
+        return 0
+    e = np.frexp(np.float32(mx))[1]
+    lb = int(R).bit_length()
+    return 62 - lb - int(e)
+
+
+def test_fixed_point_grid_gradient_scale_cannot_overflow():
+    """Worst case of the fixed-point LDS sums: R updates, all equal to the
+    level's largest |update| mx, rounded to integers at scale 2^S, sum below
+    2^63 (the int64 accumulator), while one update keeps at least 40 bits below
+    mx for slices of up to 2^22 records."""
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        mx = float(np.float32(10.0 ** rng.uniform(-38, 38)))
+        R = int(rng.integers(1, 1 << 31))
+        S = _slice_scale(mx, R)
+        q = int(np.rint(np.ldexp(np.float64(mx), S)))  # the largest single update
+        assert R * q < (1 << 63), (mx, R, S)
+        if R <= (1 << 22):
+            assert np.ldexp(np.float64(mx), S) >= 2.0 ** 39  # resolution 2^-S <= mx * 2^-39
+    # exact powers of two and a single record
+    for mx in (2.0 ** -126, 1.0, 2.0 ** 100, np.float32(3.4028235e38)):
+        S = _slice_scale(float(mx), 1)
+        assert int(np.rint(np.ldexp(np.float64(mx), S))) < (1 << 63)
+    assert _slice_scale(0.0, 5) == 0

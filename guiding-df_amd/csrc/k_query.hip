@@ -44,7 +44,9 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 }
 
 // One 64-row tile of a query: rows [row0, row0 + 64) of `io` (lane = row).
-template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
+// LOCKSTEP: a workgroup barrier before each MLP keeps the workgroup's waves on
+// the same net's weights (every wave of the workgroup must run the tile).
+template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false>
 __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
     float* xt = (float*)smem;
     float* lo = (float*)smem;
@@ -72,6 +74,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
+    if constexpr (LOCKSTEP) __syncthreads();
     mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1});
     if (a.logits_th && valid) {
@@ -102,6 +105,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
+    if constexpr (LOCKSTEP) __syncthreads();
     mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
                                      GridDirect{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1});
     if (a.logits_ph && valid) {
@@ -129,6 +133,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         _Float16* const xhrow = xh + lane * a.xs_nrc;
         encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; },
                    nullptr, 0, false, false);
+        if constexpr (LOCKSTEP) __syncthreads();
         mlp_forward_tile<1, false, 4>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
         if (valid && io.rgb) {
             io.rgb[3 * r + 0] = lrow[0];
@@ -138,11 +143,32 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     }
 }
 
+// Bytes of LDS one 64-row tile uses (the largest of its encoded-input and
+// output tiles).
+__host__ __device__ inline size_t query_tile_lds(const QueryArgs& a) {
+    size_t lds = (size_t)64 * a.xs * 4;
+    const size_t l2 = (size_t)64 * a.ls * 4;
+    const size_t l3 = (size_t)64 * a.xs_nrc * 2;
+    if (l2 > lds) lds = l2;
+    if (l3 > lds) lds = l3;
+    return (lds + 15) / 16 * 16;
+}
+
+// Up to kQueryTilesPerBlock 64-row tiles per workgroup, one per wavefront:
+// the waves of a workgroup run the same nets at about the same time (a
+// barrier before each MLP), so the weight fragments one of them brings into
+// the CU's L1 serve the others (each net's weights exceed the L1, so tiles in
+// unrelated phases would each fetch them from L2).  The launcher picks the
+// tile count that keeps the workgroup's LDS within the per-workgroup limit.
+constexpr int kQueryTilesPerBlock = 4;
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC>
-__global__ __launch_bounds__(64, 2) void query_fused_kernel(const QueryArgs a) {
+__global__ __launch_bounds__(64 * kQueryTilesPerBlock, 2) void query_fused_kernel(const QueryArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const int w = threadIdx.x >> 6;
+    const int tpb = blockDim.x >> 6;
     const RowsIO io{a.feat, a.stride, a.u, a.guided, a.wi, a.rgb, a.n, 0};
-    query_tile<NTH, NPH, WITH_DF, WITH_NRC>(a, io, (int64_t)blockIdx.x * 64, smem);
+    query_tile<NTH, NPH, WITH_DF, WITH_NRC, true>(a, io, ((int64_t)blockIdx.x * tpb + w) * 64,
+                                                  smem + w * query_tile_lds(a));
 }
 
 // Persistent work-list kernel: every wavefront pulls 64-row tiles from a device
@@ -204,20 +230,35 @@ __global__ void worklist_setup_kernel(WorkList* __restrict__ wl, const int64_t* 
 
 
 
+// Dynamic LDS available to one workgroup on the current device.
+static size_t max_block_lds() {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            cached = v;
+        else
+            cached = 65536;
+    }
+    return (size_t)cached;
+}
+
 template <int NTH, int NPH>
 static void launch_q(const QueryArgs& a, hipStream_t s) {
-    const unsigned blocks = (unsigned)((a.n + 63) / 64);
-    size_t lds = (size_t)64 * a.xs * 4;
-    const size_t l2 = (size_t)64 * a.ls * 4;
-    const size_t l3 = (size_t)64 * a.xs_nrc * 2;
-    if (l2 > lds) lds = l2;
-    if (l3 > lds) lds = l3;
+    const int64_t tiles = (a.n + 63) / 64;
+    const size_t tl = query_tile_lds(a);
+    int tpb = kQueryTilesPerBlock;
+    while (tpb > 1 && tl * tpb > max_block_lds()) tpb /= 2;
+    const unsigned blocks = (unsigned)((tiles + tpb - 1) / tpb);
+    const size_t lds = tl * tpb;
+    const dim3 bd(64 * tpb);
     if (a.th == nullptr)
-        hipLaunchKernelGGL((query_fused_kernel<16, 16, false, true>), dim3(blocks), dim3(64), lds, s, a);
+        hipLaunchKernelGGL((query_fused_kernel<16, 16, false, true>), dim3(blocks), bd, lds, s, a);
     else if (a.rgb)
-        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, true>), dim3(blocks), dim3(64), lds, s, a);
+        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, true>), dim3(blocks), bd, lds, s, a);
     else
-        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, false>), dim3(blocks), dim3(64), lds, s, a);
+        hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, false>), dim3(blocks), bd, lds, s, a);
 }
 
 static size_t query_lds(const QueryArgs& a) {

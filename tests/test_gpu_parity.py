@@ -127,6 +127,9 @@ def _query_case(cfgname):
     if cfgname == "plain":
         cfg = G.plain_head_config()
         return cfg, 8, lambda n, s: plain_features(n, s)
+    if cfgname == "hash64":  # 64 x 64 bins: the largest LDS tiles of the fused kernel
+        cfg = G.hash_config(n_bins_theta=64, n_bins_phi=64, with_nrc=True)
+        return cfg, 9, lambda n, s: df_features(n, s, nrc=True)
     cfg = G.hash_config(with_nrc=True)
     return cfg, 9, lambda n, s: df_features(n, s, nrc=True)
 
@@ -135,7 +138,7 @@ def _rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
 
 
-@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+@pytest.mark.parametrize("cfgname", ["plain", "hash", "hash64"])
 def test_query_vs_fp64(torch_cuda, cfgname):
     torch = torch_cuda
     cfg, D, feats = _query_case(cfgname)

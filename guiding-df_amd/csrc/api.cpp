@@ -913,8 +913,9 @@ static bool train_grid_pair(dfg_ctx* c, const float* feat, int stride, int64_t n
 
 static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st);
 
-static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
-                                int n_chunks, int chunk, float loss_scale, hipStream_t st) {
+// MLP backward of one net: dL/d(activations) and dL/d(grid features) (dx),
+// then the weight gradients.
+static void net_backward_mlp(NetState& s, int64_t n, int64_t ld, int n_chunks, int chunk, hipStream_t st) {
     const NetSpec& ns = s.spec;
     const int gp = grid_part_index(ns);
     TrainBwdArgs b;
@@ -946,43 +947,64 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
     w.chunk = chunk;
     w.n_chunks = n_chunks;
     launch_wgrad(w, ns, st);
-    const GridBucketPlan plan = gp >= 0 ? grid_bucket_plan(ns, n) : GridBucketPlan{};
-    if (gp >= 0 && c->grid_bwd_bucketed && plan.ok) {
-        const int F = ns.grid.n_feat;
-        // slice totals / offsets, then the per-level maxima (DFG_MAX_LEVELS uint32)
-        c->gb_offs.ensure((size_t)plan.n_slices + 1 + DFG_MAX_LEVELS / 2);
-        c->gb_part.ensure((size_t)plan.max_partials * kGridSliceFloats);
-        c->gb_cursor.ensure((size_t)plan.n_slices);
-        c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
-        if (F != 2) c->gb_idx.ensure((size_t)plan.records);
-        c->gb_val.ensure((size_t)plan.records * (F == 2 ? 3 : F));
-        c->gb_items.ensure((size_t)plan.max_items);
-        c->gb_n_items.ensure(2);
-        c->gb_multi.ensure((size_t)plan.max_multi);
-        GridBucketArgs gb;
-        std::memset(&gb, 0, sizeof(gb));
-        gb.ns = s.d_spec.p;
-        gb.feat = feat;
-        gb.stride = stride;
-        gb.pos_dim = ns.parts[gp].dim_begin;
-        gb.dx = s.dx.p;
-        gb.grad = s.grad.p;
-        gb.n = n;
-        gb.ld = ld;
-        gb.slice_shift = plan.slice_shift;
-        gb.n_chunks = plan.n_chunks;
-        gb.offs = c->gb_offs.p;
-        gb.cursor = c->gb_cursor.p;
-        gb.cnt = c->gb_cnt.p;
-        gb.rec_idx = c->gb_idx.p;
-        gb.rec_val = c->gb_val.p;
-        gb.items = c->gb_items.p;
-        gb.n_items = c->gb_n_items.p;
-        gb.lvl_max = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
-        gb.partial = c->gb_part.p;
-        gb.multi = c->gb_multi.p;
+}
+
+// Arguments of the bucketed grid backward of net s over the context's shared
+// bucket buffers, with its records in `rec` and level maxima at `lvl_max`.
+static GridBucketArgs bucket_args(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
+                                  const GridBucketPlan& plan, float* rec, uint32_t* lvl_max) {
+    const NetSpec& ns = s.spec;
+    GridBucketArgs gb;
+    std::memset(&gb, 0, sizeof(gb));
+    gb.ns = s.d_spec.p;
+    gb.feat = feat;
+    gb.stride = stride;
+    gb.pos_dim = ns.parts[grid_part_index(ns)].dim_begin;
+    gb.dx = s.dx.p;
+    gb.grad = s.grad.p;
+    gb.n = n;
+    gb.ld = ld;
+    gb.slice_shift = plan.slice_shift;
+    gb.n_chunks = plan.n_chunks;
+    gb.offs = c->gb_offs.p;
+    gb.cursor = c->gb_cursor.p;
+    gb.cnt = c->gb_cnt.p;
+    gb.rec_idx = c->gb_idx.p;
+    gb.rec_val = rec;
+    gb.items = c->gb_items.p;
+    gb.n_items = c->gb_n_items.p;
+    gb.lvl_max = lvl_max;
+    gb.partial = c->gb_part.p;
+    gb.multi = c->gb_multi.p;
+    return gb;
+}
+
+// Shared bucket buffers for a plan (offs carries `maxima` per-level maxima
+// arrays after the slice offsets).
+static void ensure_bucket_ws(dfg_ctx* c, const GridBucketPlan& plan, int F, int maxima) {
+    c->gb_offs.ensure((size_t)plan.n_slices + 1 + (size_t)maxima * DFG_MAX_LEVELS / 2);
+    c->gb_part.ensure((size_t)plan.max_partials * kGridSliceFloats);
+    c->gb_cursor.ensure((size_t)plan.n_slices);
+    c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
+    if (F != 2) c->gb_idx.ensure((size_t)plan.records);
+    c->gb_val.ensure((size_t)plan.records * (F == 2 ? 3 : F));
+    c->gb_items.ensure((size_t)plan.max_items);
+    c->gb_n_items.ensure(2);
+    c->gb_multi.ensure((size_t)plan.max_multi);
+}
+
+static void net_grid_backward(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
+                              hipStream_t st) {
+    const NetSpec& ns = s.spec;
+    const int gp = grid_part_index(ns);
+    if (gp < 0) return;
+    const GridBucketPlan plan = grid_bucket_plan(ns, n);
+    if (c->grid_bwd_bucketed && plan.ok) {
+        ensure_bucket_ws(c, plan, ns.grid.n_feat, 1);
+        const GridBucketArgs gb = bucket_args(c, s, feat, stride, n, ld, plan, c->gb_val.p,
+                                              (uint32_t*)(c->gb_offs.p + plan.n_slices + 1));
         launch_grid_bwd_bucketed(gb, ns, plan, st);
-    } else if (gp >= 0) {
+    } else {
         const size_t ngrid = (size_t)ns.n_params - ns.grid.param_begin;
         launch_fill_f32(s.grad.p + ns.grid.param_begin, 0.0f, (int64_t)ngrid, st);
         GridBwdArgs gb;
@@ -997,6 +1019,29 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
         gb.ld = ld;
         launch_grid_bwd(gb, ns, st);
     }
+}
+
+// Grid backward of both DF nets in one bucketed pass (interleaved-table
+// grids: F = 2, identical structure); false when that does not apply.
+static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, int64_t n, int64_t ld, hipStream_t st) {
+    NetState& th = c->net[0];
+    NetState& ph = c->net[1];
+    if (!c->pair_ok || !c->grid_bwd_bucketed || th.spec.grid.interp == INTERP_NEAREST) return false;
+    const GridBucketPlan plan = grid_bucket_plan(th.spec, n);
+    if (!plan.ok) return false;
+    ensure_bucket_ws(c, plan, 2, 2);
+    c->gb_val2.ensure((size_t)plan.records * 3);
+    uint32_t* mx = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
+    const GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
+    const GridBucketArgs b = bucket_args(c, ph, feat, stride, n, ld, plan, c->gb_val2.p, mx + DFG_MAX_LEVELS);
+    launch_grid_bwd_bucketed_pair(a, b, th.spec, plan, st);
+    return true;
+}
+
+static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int stride, int64_t n, int64_t ld,
+                                int n_chunks, int chunk, float loss_scale, hipStream_t st) {
+    net_backward_mlp(s, n, ld, n_chunks, chunk, st);
+    net_grid_backward(c, s, feat, stride, n, ld, st);
     net_update(c, s, loss_scale, st);
 }
 
@@ -1076,8 +1121,20 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE, parameters.h:45
     l.linear = ctx->linear;
     launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
-    net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-    net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    if (paired) {
+        // both MLP backwards, then one grid backward for the two grids
+        net_backward_mlp(th, B, ld, n_chunks, kChunk, st);
+        net_backward_mlp(ph, B, ld, n_chunks, kChunk, st);
+        if (!grid_backward_pair(ctx, feat, feat_stride, B, ld, st)) {
+            net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
+            net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+        }
+        net_update(ctx, th, 1024.0f, st);
+        net_update(ctx, ph, 1024.0f, st);
+    } else {
+        net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+        net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    }
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);

@@ -228,8 +228,15 @@ struct GridBucketArgs {
     uint32_t* lvl_max;       // [levels] bits of max |update| per level (zeroed with offs)
     unsigned long long* partial;  // [max_partials][kGridSliceFloats] fixed-point sums of
                                   // the parts of slices cut into several parts
+    // second net of a paired scatter (same grid structure): its dL/dy, records
+    // and level maxima (null otherwise)
+    const float* dx2;
+    float* rec_val2;
+    uint32_t* lvl_max2;
 };
 void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s);
+void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const GridBucketArgs& b, const NetSpec& hs,
+                                   const GridBucketPlan& p, hipStream_t s);
 
 struct AdamArgs {
     int64_t n_params, n_matrix;

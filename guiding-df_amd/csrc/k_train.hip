@@ -640,30 +640,36 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 // 12-byte store per record.  The same records land in the same slice ranges
 // (only their order inside a range differs, which the order-independent
 // fixed-point reduce does not see).
+//
+// PAIR: the theta and phi grids of identical structure in one pass (same
+// positions, so the same corners, slices and record positions): the second
+// net's records {entry, w * dL/dy2[0], w * dL/dy2[1]} go to rec_val2 at the
+// positions of the first net's, and its level maxima to lvl_max2.
 constexpr int kStageRows = 256;
+template <bool PAIR>
 __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
-    __shared__ int64_t base[kGridMaxSlicesPerLevel];
-    __shared__ uint32_t fill[kGridMaxSlicesPerLevel];
+    // next record of each slice in this block's range (records < 2^31)
+    __shared__ uint32_t next[kGridMaxSlicesPerLevel];
     __shared__ uint32_t lcnt[kGridMaxSlicesPerLevel];
     __shared__ uint32_t loff[kGridMaxSlicesPerLevel];
     __shared__ uint4 stage[kStageRows * 8];
-    __shared__ uint32_t bmax;
+    __shared__ uint2 stage2[PAIR ? kStageRows * 8 : 1];
+    __shared__ uint32_t bmax, bmax2;
     const GridSpec& gs = a.ns->grid;
     const int level = blockIdx.y;
     const int c = blockIdx.x;
     const int t = threadIdx.x;
     const int nsl = level_slices(gs, level, a.slice_shift);
     const int sb = level_slice_base(gs, level, a.slice_shift);
-    if (t == 0) bmax = 0u;
+    if (t == 0) bmax = bmax2 = 0u;
     for (int i = t; i < nsl; i += blockDim.x) {
-        fill[i] = 0u;
         const uint32_t h = a.cnt[(size_t)(sb + i) * a.n_chunks + c];
-        base[i] = h ? (int64_t)atomicAdd((unsigned long long*)&a.cursor[sb + i], (unsigned long long)h) : 0;
+        next[i] = h ? (uint32_t)atomicAdd((unsigned long long*)&a.cursor[sb + i], (unsigned long long)h) : 0u;
     }
     const int64_t r0 = (int64_t)c * kGridChunkRows;
     const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
     const uint32_t emask = (1u << a.slice_shift) - 1u;
-    uint32_t umax = 0u;
+    uint32_t umax = 0u, umax2 = 0u;
     for (int64_t s0 = r0; s0 < r1; s0 += kStageRows) {
         for (int i = t; i < nsl; i += blockDim.x) lcnt[i] = 0u;
         __syncthreads();
@@ -671,9 +677,15 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         const bool live = row < r1;
         uint32_t sl[2] = {0u, 0u}, lp[2] = {0u, 0u};
         uint4 rec[2];
+        uint2 rec2[2];
         if (live) {
             const float d0 = a.dx[(size_t)(level * 2) * a.ld + row];
             const float d1 = a.dx[(size_t)(level * 2 + 1) * a.ld + row];
+            float e0 = 0.0f, e1 = 0.0f;
+            if constexpr (PAIR) {
+                e0 = a.dx2[(size_t)(level * 2) * a.ld + row];
+                e1 = a.dx2[(size_t)(level * 2 + 1) * a.ld + row];
+            }
             const float* fr = a.feat + row * a.stride + a.pos_dim;
             uint32_t idx[2];
             float w[2];
@@ -684,6 +696,11 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
                 const float v0 = w[j] * d0, v1 = w[j] * d1;
                 rec[j] = make_uint4(idx[j] & emask, __float_as_uint(v0), __float_as_uint(v1), sl[j]);
                 umax = max(umax, max(__float_as_uint(fabsf(v0)), __float_as_uint(fabsf(v1))));
+                if constexpr (PAIR) {
+                    const float u0 = w[j] * e0, u1 = w[j] * e1;
+                    rec2[j] = make_uint2(__float_as_uint(u0), __float_as_uint(u1));
+                    umax2 = max(umax2, max(__float_as_uint(fabsf(u0)), __float_as_uint(fabsf(u1))));
+                }
                 lp[j] = atomicAdd(&lcnt[sl[j]], 1u);
             }
         }
@@ -710,21 +727,33 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         __syncthreads();
         if (live) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j) stage[loff[sl[j]] + lp[j]] = rec[j];
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t k = loff[sl[j]] + lp[j];
+                stage[k] = rec[j];
+                if constexpr (PAIR) stage2[k] = rec2[j];
+            }
         }
         __syncthreads();
         const int nrec = (int)((r1 - s0 < kStageRows ? r1 - s0 : kStageRows) * 8);
         for (int i = t; i < nrec; i += blockDim.x) {
             const uint4 r = stage[i];
-            const int64_t pos = base[r.w] + (int64_t)fill[r.w] + (int64_t)(i - (int)loff[r.w]);
+            const int64_t pos = (int64_t)next[r.w] + (int64_t)(i - (int)loff[r.w]);
             ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
+            if constexpr (PAIR) {
+                const uint2 q = stage2[i];
+                ((uint3*)a.rec_val2)[pos] = make_uint3(r.x, q.x, q.y);
+            }
         }
         __syncthreads();
-        for (int i = t; i < nsl; i += blockDim.x) fill[i] += lcnt[i];
+        for (int i = t; i < nsl; i += blockDim.x) next[i] += lcnt[i];
     }
     atomicMax(&bmax, umax);
+    if constexpr (PAIR) atomicMax(&bmax2, umax2);
     __syncthreads();
     if (t == 0 && bmax) atomicMax(&a.lvl_max[level], bmax);
+    if constexpr (PAIR) {
+        if (t == 0 && bmax2) atomicMax(&a.lvl_max2[level], bmax2);
+    }
 }
 
 // Level, first entry and value count of slice s.
@@ -866,12 +895,37 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
     if (hs.grid.n_feat == 2 && hs.grid.interp != INTERP_NEAREST)
-        hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel, g1, dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<false>, g1, dim3(1024), 0, s, a);
     else
         hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
     hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
     hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
+}
+
+// theta (a) and phi (b) grids of identical structure (F = 2, not nearest): one
+// count, scan and staged scatter serve both (their corners, slices and record
+// positions coincide); then each net's reduce and combine.  b shares a's slice
+// offsets, cursors, work list and partial-sum buffer, and has its own records,
+// level maxima, dL/dy and gradient.
+void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const GridBucketArgs& b, const NetSpec& hs,
+                                   const GridBucketPlan& p, hipStream_t s) {
+    if (a.n <= 0 || !p.ok) return;
+    const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
+    // int64 slice totals, then both nets' per-level maxima right after them
+    launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + 2 * DFG_MAX_LEVELS, s);
+    hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
+    GridBucketArgs ab = a;
+    ab.dx2 = b.dx;
+    ab.rec_val2 = b.rec_val;
+    ab.lvl_max2 = b.lvl_max;
+    hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024), 0, s, ab);
+    const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
+    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
+    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, b);
+    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, b);
 }
 
 // ---------------------------------------------------------------------------

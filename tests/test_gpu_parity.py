@@ -437,14 +437,17 @@ def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
     f.close()
 
 
-@pytest.mark.parametrize("cfgname,B", [("hash", 70000), ("hash_nearest", 5000), ("reference", 9000)])
+@pytest.mark.parametrize("cfgname,B", [("hash", 70000), ("hash_unpaired", 20000), ("hash_nearest", 5000),
+                                        ("reference", 9000)])
 def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
     """The bucketed (count / scan / scatter / LDS-reduce) grid backward gives
     the atomic scatter's table gradient to 1e-5 (fixed-point sums against
     fp32 atomics in arbitrary order), covers every table entry (untouched
     entries exactly 0; an entry may differ in being zero only where its sum is
     below 1e-9 of the largest, the fixed-point resolution) and leaves the MLP
-    gradients untouched."""
+    gradients untouched.  "hash": theta and phi grids of one structure share
+    one count / scan / scatter pass; "hash_unpaired" gives phi's grid another
+    table size, so each net runs its own staged pass."""
     torch = torch_cuda
     if cfgname == "reference":
         cfg = G.reference_config(with_nrc=False)
@@ -454,6 +457,10 @@ def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
             for e in cfg["nn"]["encoding"]["nested"]:
                 if e["otype"] == "HashGrid":
                     e["interpolation"] = "Nearest"
+        if cfgname == "hash_unpaired":
+            pe = cfg["nn"]["phi_encoding"]["nested"]
+            grid = dict(pe[0], log2_hashmap_size=14)
+            cfg["nn"]["phi_encoding"] = {"otype": "Composite", "nested": [grid] + pe[1:]}
     import copy
     cfg_a = copy.deepcopy(cfg)
     cfg_a["dfg"]["grid_backward"] = "atomic"

@@ -391,6 +391,9 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         if (c->query_chunks < 1 || c->query_chunks > 64) throw Error(DFG_ERR_CONFIG, "query_chunks must be in 1..64");
         if (c->prepass_blocks_per_cu < 0 || c->prepass_blocks_per_cu > 64)
             throw Error(DFG_ERR_CONFIG, "prepass_blocks_per_cu must be in 0..64");
+        c->wgrad_chunk_rows = (int)d.number_or("wgrad_chunk_rows", 1024);
+        if (c->wgrad_chunk_rows < 64 || c->wgrad_chunk_rows > 65536 || c->wgrad_chunk_rows % 64)
+            throw Error(DFG_ERR_CONFIG, "wgrad_chunk_rows must be a multiple of 64 in 64..65536");
         {
             int dev = 0, cus = 0;
             HIPX(hipGetDevice(&dev));
@@ -1095,7 +1098,8 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     if (B == 0) return DFG_OK;
     hipStream_t st = (hipStream_t)stream;
     const int64_t ld = round_up(B, 64);
-    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    const int chunk = ctx->wgrad_chunk_rows;
+    const int n_chunks = (int)((ld + chunk - 1) / chunk);
     NetState& th = ctx->net[0];
     NetState& ph = ctx->net[1];
     ensure_train_ws(th, ld, n_chunks);
@@ -1123,8 +1127,8 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     launch_df_loss(l, ctx->n_theta, ctx->n_phi, st);
     if (paired) {
         // both MLP backwards, then one grid backward for the two grids
-        net_backward_mlp(th, B, ld, n_chunks, kChunk, st);
-        net_backward_mlp(ph, B, ld, n_chunks, kChunk, st);
+        net_backward_mlp(th, B, ld, n_chunks, chunk, st);
+        net_backward_mlp(ph, B, ld, n_chunks, chunk, st);
         if (!grid_backward_pair(ctx, feat, feat_stride, B, ld, st)) {
             net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
             net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
@@ -1132,8 +1136,8 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
         net_update(ctx, th, 1024.0f, st);
         net_update(ctx, ph, 1024.0f, st);
     } else {
-        net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
-        net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+        net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);
+        net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);
     }
     if (loss_out) {
         ctx->mean_ws.ensure(256);
@@ -1154,7 +1158,8 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     if (B == 0) return DFG_OK;
     hipStream_t st = (hipStream_t)stream;
     const int64_t ld = round_up(B, 64);
-    const int n_chunks = (int)((ld + kChunk - 1) / kChunk);
+    const int chunk = ctx->wgrad_chunk_rows;
+    const int n_chunks = (int)((ld + chunk - 1) / chunk);
     NetState& s = ctx->net[2];
     ensure_train_ws(s, ld, n_chunks);
     ctx->loss.ensure((size_t)ld);
@@ -1170,7 +1175,7 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     l.ld = ld;
     l.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE_NRC, parameters.h:59
     launch_nrc_loss(l, st);
-    net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, kChunk, 1024.0f, st);
+    net_backward_update(ctx, s, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);
     if (loss_out) {
         ctx->mean_ws.ensure(256);
         launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);

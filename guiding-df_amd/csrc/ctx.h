@@ -100,6 +100,7 @@ struct dfg_ctx {
     int query_chunks = 1;              // chunks of a large query (1: no chunking)
     int prepass_blocks_per_cu = 0;     // persistent pre-pass grid per CU in chunked mode (0: one block per item)
     int64_t pipeline_min_rows = 262144;
+    int wgrad_chunk_rows = 1024;       // batch rows per weight-gradient partial sum (multiple of 64)
     int n_cus = 256;
     hipStream_t side = nullptr;
     hipEvent_t pipe_start = nullptr;

@@ -335,16 +335,27 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const WgradArgs a) {
     int64_t k1 = k0 + a.chunk;
     if (k1 > a.ld) k1 = a.ld;
     const _Float16* gr = G + (size_t)(wave * 16 + i16) * a.ld;
-    for (int64_t kb = k0; kb < k1; kb += 32) {
-        const h8 af = *(const h8*)(gr + kb + g * 8);
+    // U K-steps per iteration, all of their fragments loaded before the first
+    // product; the products run in K order.  k1 - k0 is a multiple of 64.
+    auto steps = [&](int64_t kb, auto U) {
+        constexpr int NU = decltype(U)::value;
+        h8 af[NU], bf[NU][8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            if (t < nt_n) {
-                const h8 bf = *(const h8*)(A + (size_t)(t * 16 + i16) * a.ld + kb + g * 8);
-                acc[t] = mfma16(af, bf, acc[t]);
-            }
+        for (int u = 0; u < NU; ++u) {
+            af[u] = *(const h8*)(gr + kb + 32 * u + g * 8);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t < nt_n) bf[u][t] = *(const h8*)(A + (size_t)(t * 16 + i16) * a.ld + kb + 32 * u + g * 8);
         }
-    }
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t < nt_n) acc[t] = mfma16(af[u], bf[u][t], acc[t]);
+    };
+    int64_t kb = k0;
+    for (; kb + 128 <= k1; kb += 128) steps(kb, std::integral_constant<int, 4>{});
+    if (kb < k1) steps(kb, std::integral_constant<int, 2>{});
     float* out = a.partial + (size_t)c * ns.n_matrix_params + ns.layer_off[l];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {

@@ -887,7 +887,7 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     a.x = s.x.p;
     a.act = s.act.p;
     a.out = s.out.p;
-    a.xs = s.spec.k_steps * 32 + 4;
+    a.xs = s.spec.k_steps * 32 + 8;  // halves
     a.ls = s.spec.n_out_pad + 1;
     if (s.spec.grid.present) {
         if (!grid_done) {

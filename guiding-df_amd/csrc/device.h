@@ -799,7 +799,37 @@ struct ActStore {
     _Float16* act;      // hidden activations a_1..a_H, [H][64][ld]
     int64_t ld;         // leading dimension (padded batch)
     int64_t row0;       // first batch row of this tile
+    _Float16* stage;    // LDS staging tile (64 x kTileS halves) for 16-byte stores of a
+                        // whole 64-row tile, or NULL (element stores)
 };
+
+// The per-layer tiles of a 64-row block go through LDS ([feature][row], row
+// stride kTileS halves), so that global memory sees 16-byte row runs of the
+// feature-major [64][ld] activation and gradient arrays instead of one 2-byte
+// access per element.
+constexpr int kTileS = 72;
+
+// act[0..63][row0 .. row0+63] -> tile (8 lanes per feature, 8 rows each).
+__device__ __forceinline__ void tile_load(_Float16* tile, const _Float16* __restrict__ src, int64_t ld, int64_t row0,
+                                          int lane) {
+    const int c = lane & 7;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int n = (lane >> 3) + 8 * k;
+        *(h8*)(tile + n * kTileS + 8 * c) = *(const h8*)(src + (size_t)n * ld + row0 + 8 * c);
+    }
+}
+
+__device__ __forceinline__ void tile_store(const _Float16* tile, _Float16* __restrict__ dst, int64_t ld, int64_t row0,
+                                           int lane) {
+    const int c = lane & 7;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int n = (lane >> 3) + 8 * k;
+        *(h8*)(dst + (size_t)n * ld + row0 + 8 * c) = *(const h8*)(tile + n * kTileS + 8 * c);
+    }
+}
+
 
 // fp32 -> (hi, lo) fp16 pair with hi + lo == v to ~2^-21 relative: the MFMA
 // then sees the activation at fp32-level precision through two products.  hi
@@ -823,6 +853,39 @@ __device__ __forceinline__ void relu_to_act2(const f4 (&c)[4], h8 (&hi)[2], h8 (
             hi[s][j] = h;
             lo[s][j] = l;
         }
+}
+
+// One layer's activations (lane: rows q*16 + i16, neurons 16*(2s + j/4) +
+// 4g + j%4) to the feature-major [64][ld] array: through the LDS staging tile
+// as 16-byte row runs when the pass covers the whole 64-row tile, else element
+// by element.
+template <int QN>
+__device__ __forceinline__ void store_act(const ActStore& st, _Float16* dst, const h8 (&act)[QN][2], int q0, int i16,
+                                          int g, int lane) {
+    if (QN == 4 && st.stage) {
+#pragma unroll
+        for (int q = 0; q < QN; ++q)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
+                    st.stage[n * kTileS + q * 16 + i16] = act[q][s][j];
+                }
+        __syncthreads();
+        tile_store(st.stage, dst, st.ld, st.row0, lane);
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
+                dst[(size_t)n * st.ld + st.row0 + (q0 + q) * 16 + i16] = act[q][s][j];
+            }
 }
 
 // Forward of a whole MLP for a 64-row tile.  xt: encoded input in LDS as fp32,
@@ -905,18 +968,7 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
             else relu_to_act(acc[q], act[q]);
         }
     }
-    if (st) {
-        _Float16* a1 = st->act;
-#pragma unroll
-        for (int q = 0; q < QN; ++q)
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
-                    a1[(size_t)n * st->ld + st->row0 + (q0 + q) * 16 + i16] = act[q][s][j];
-                }
-    }
+    if (st) store_act<QN>(*st, st->act, act, q0, i16, g, lane);
     // hidden layers 1..H-1; the weight fragments of layer l + 1 are loaded
     // while layer l computes (they do not depend on the activations)
     h8 wf[4][2];
@@ -956,18 +1008,7 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
             if (SPLIT) relu_to_act2(acc[q], act[q], actl[q]);
             else relu_to_act(acc[q], act[q]);
         }
-        if (st) {
-            _Float16* al = st->act + (size_t)l * 64 * st->ld;
-#pragma unroll
-            for (int q = 0; q < QN; ++q)
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int n = 16 * (2 * s + (j >> 2)) + 4 * g + (j & 3);
-                        al[(size_t)n * st->ld + st->row0 + (q0 + q) * 16 + i16] = act[q][s][j];
-                    }
-        }
+        if (st) store_act<QN>(*st, st->act + (size_t)l * 64 * st->ld, act, q0, i16, g, lane);
     }
     // output layer (its fragments are in wf)
     {

@@ -61,7 +61,7 @@ struct TrainFwdArgs {
     _Float16* act;          // [H][64][ld]
     _Float16* out;          // row-major [ld][n_out_pad] (bins or prediction)
     const float* gfeat;     // grid features from the pre-pass or NULL
-    int xs, ls;
+    int xs, ls;             // LDS row strides: encoded input (halves), output (floats)
 };
 
 void launch_query(const QueryArgs& a, int n_theta, int n_phi, hipStream_t s);

@@ -24,7 +24,9 @@ struct RowInT {
 template <int NT_OUT, int NBINS>
 __global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    float* xt = (float*)smem;
+    // encoded input tile: fp16 (the values are rounded to fp16 anyway), row
+    // stride xs halves; the fp32 output tile aliases it
+    _Float16* xt = (_Float16*)smem;
     float* lo = (float*)smem;
     const NetSpec& ns = *a.ns;
     const int lane = threadIdx.x;
@@ -39,13 +41,14 @@ __global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) 
         th_in = theta_input(a.target[r * 9], a.theta_bins, a.linear);
         base = ns.in_dims - 1;
     }
-    float* const xrow = xt + lane * a.xs;
+    _Float16* const xrow = xt + lane * a.xs;
     // the training forward feeds fp16 activations, like the reference's
-    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (float)(_Float16)v; },
+    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (_Float16)v; },
                a.gfeat ? a.gfeat + r : nullptr, a.ld);
     const int kc = ns.k_steps * 32;
-    for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = (_Float16)xrow[c];
-    ActStore st{nullptr, a.act, a.ld, row0};
+    for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = xrow[c];
+    // the hidden activations are staged in the input tile (dead after layer 0)
+    ActStore st{nullptr, a.act, a.ld, row0, (_Float16*)smem};
     mlp_forward_tile<NT_OUT, false>(ns, a.pack, xt, a.xs, lo, a.ls, &st);
     const float* lrow = lo + lane * a.ls;
     _Float16* o = a.out + row * (NT_OUT * 16);
@@ -76,9 +79,11 @@ __global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) 
 void launch_train_fwd(const TrainFwdArgs& a, const NetSpec& hs, hipStream_t s) {
     if (a.n <= 0) return;
     const unsigned blocks = (unsigned)(a.ld / 64);
-    size_t lds = (size_t)64 * a.xs * 4;
+    size_t lds = (size_t)64 * a.xs * 2;
     const size_t l2 = (size_t)64 * a.ls * 4;
     if (l2 > lds) lds = l2;
+    const size_t l3 = (size_t)64 * kTileS * 2;  // activation staging tile
+    if (l3 > lds) lds = l3;
     const int nt = hs.n_out_pad / 16;
     if (a.mode == 1) {
         hipLaunchKernelGGL((train_fwd_kernel<1, 16>), dim3(blocks), dim3(64), lds, s, a);
@@ -203,22 +208,23 @@ void launch_nrc_loss(const NrcLossArgs& a, hipStream_t s) {
 // Backward data gradients for a 64-row tile (kernel_mlp_fused_backward,
 // fully_fused_mlp.cu:150-259, plus the last-layer and input-gradient GEMMs).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mask_store(f4 (&c)[4], const _Float16* __restrict__ act, _Float16* __restrict__ gst,
-                                           int64_t ld, int64_t row, int g) {
+// ReLU mask of the data gradient by the stored activation (in the tile), and
+// the masked gradient written back into the tile.
+__device__ __forceinline__ void mask_tile(f4 (&c)[4], _Float16* tile, int q, int i16, int g) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int n = 16 * t + 4 * g + r;
-            const float av = (float)act[(size_t)n * ld + row];
-            const float v = av > 0.0f ? c[t][r] : 0.0f;
+            _Float16* e = tile + (16 * t + 4 * g + r) * kTileS + q * 16 + i16;
+            const float v = (float)*e > 0.0f ? c[t][r] : 0.0f;
             c[t][r] = v;
-            gst[(size_t)n * ld + row] = sat_h(v);
+            *e = sat_h(v);
         }
 }
 
 template <int KS_OUT>
 __global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
+    __shared__ __align__(16) _Float16 tile[64 * kTileS];
     const NetSpec& ns = *a.ns;
     const int lane = threadIdx.x;
     const int i16 = lane & 15, g = lane >> 4;
@@ -228,6 +234,19 @@ __global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
     const size_t lay = (size_t)64 * a.ld;
     f4 acc[4][4];
     h8 act[4][2];
+    // layer tile: mask by a_l, write g_{l+1}, next operand
+    auto mask_layer = [&](int l) {
+        tile_load(tile, a.act + (size_t)l * lay, a.ld, row0, lane);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            mask_tile(acc[q], tile, q, i16, g);
+            c_to_act(acc[q], act[q]);
+        }
+        __syncthreads();
+        tile_store(tile, a.g + (size_t)l * lay, a.ld, row0, lane);
+        __syncthreads();
+    };
     // dL/da_H = W_H^T g_out (K = outputs, natural order)
     {
         const _Float16* pk = a.pack + ns.bwd_off[H];
@@ -251,11 +270,7 @@ __global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
                 for (int q = 0; q < 4; ++q) acc[q][t] = mfma16(w, b[q], acc[q][t]);
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            mask_store(acc[q], a.act + (size_t)(H - 1) * lay, a.g + (size_t)(H - 1) * lay, a.ld, row0 + q * 16 + i16, g);
-            c_to_act(acc[q], act[q]);
-        }
+        mask_layer(H - 1);
     }
     // hidden layers
     for (int l = H - 1; l >= 1; --l) {
@@ -270,14 +285,12 @@ __global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
                 acc[q][t] = mfma16(w1, act[q][1], c);
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            mask_store(acc[q], a.act + (size_t)(l - 1) * lay, a.g + (size_t)(l - 1) * lay, a.ld, row0 + q * 16 + i16, g);
-            c_to_act(acc[q], act[q]);
-        }
+        mask_layer(l - 1);
     }
-    // dL/dx = W_0^T g_1, only the requested input columns (the grid features)
+    // dL/dx = W_0^T g_1, only the requested input columns (the grid features),
+    // 16 columns at a time through LDS ([column][row] fp32, row stride 68)
     if (a.dx) {
+        float* ft = (float*)tile;
         const _Float16* pk = a.pack + ns.bwd_off[0];
         const int t_lo = a.dx_begin / 16;
         const int t_hi = (a.dx_begin + a.dx_n + 15) / 16;
@@ -288,13 +301,19 @@ __global__ __launch_bounds__(64) void train_bwd_kernel(const TrainBwdArgs a) {
             for (int q = 0; q < 4; ++q) {
                 f4 c = mfma16(w0, act[q][0], f4{0.f, 0.f, 0.f, 0.f});
                 c = mfma16(w1, act[q][1], c);
-                const int64_t row = row0 + q * 16 + i16;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * t + 4 * g + r - a.dx_begin;
-                    if (col >= 0 && col < a.dx_n) a.dx[(size_t)col * a.ld + row] = c[r];
-                }
+                for (int r = 0; r < 4; ++r) ft[(4 * g + r) * 68 + q * 16 + i16] = c[r];
             }
+            __syncthreads();
+            // 16 columns x 64 rows: 16 lanes per column, 4 rows each
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int cc = (lane >> 4) + 4 * k;
+                const int col = 16 * t + cc - a.dx_begin;
+                if (col >= 0 && col < a.dx_n)
+                    *(float4*)(a.dx + (size_t)col * a.ld + row0 + 4 * i16) = *(const float4*)(ft + cc * 68 + 4 * i16);
+            }
+            __syncthreads();
         }
     }
 }

@@ -1050,7 +1050,8 @@ static void net_backward_update(dfg_ctx* c, NetState& s, const float* feat, int 
 
 // Adam + EMA (EmaOptimizer::step, ema.h:102-136) on the gradient in s.grad,
 // then the fragment packs of the new weights.
-static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st) {
+// Adam + EMA arguments of net s for its next step (advances its step count).
+static AdamArgs adam_args(dfg_ctx* c, NetState& s, float loss_scale) {
     const NetSpec& ns = s.spec;
     s.step += 1;
     AdamArgs o;
@@ -1082,9 +1083,48 @@ static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st
     o.m = s.m.p;
     o.v = s.v.p;
     o.steps = s.steps.p;
-    launch_adam_ema(o, st);
-    launch_pack(s.d_spec.p, ns, s.w16.p, s.pack_train.p, 3, st);
-    launch_pack(s.d_spec.p, ns, s.ema16.p, s.pack_infer.p, 1, st);
+    return o;
+}
+
+static void net_pack(NetState& s, hipStream_t st) {
+    launch_pack(s.d_spec.p, s.spec, s.w16.p, s.pack_train.p, 3, st);
+    launch_pack(s.d_spec.p, s.spec, s.ema16.p, s.pack_infer.p, 1, st);
+}
+
+static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st) {
+    launch_adam_ema(adam_args(c, s, loss_scale), st);
+    net_pack(s, st);
+}
+
+// Adam + EMA of both DF nets: each net's MLP parameters on their own, the
+// grid parameters of the two nets together (whole interleaved-table entries).
+static void nets_update_pair(dfg_ctx* c, float loss_scale, hipStream_t st) {
+    NetState& th = c->net[0];
+    NetState& ph = c->net[1];
+    const int64_t entries = (int64_t)th.spec.grid.offset[th.spec.grid.n_levels];
+    // the grid parameters: the whole tail of each net's parameters, past the
+    // matrix parameters, starting at an even index (8-byte vector accesses)
+    auto tail = [&](const NetState& s) {
+        const int64_t b = s.spec.grid.param_begin;
+        return b + 2 * entries == s.spec.n_params && b >= s.spec.n_matrix_params && b % 2 == 0;
+    };
+    const bool whole = c->pair_ok && tail(th) && tail(ph);
+    if (!whole) {
+        net_update(c, th, loss_scale, st);
+        net_update(c, ph, loss_scale, st);
+        return;
+    }
+    const AdamArgs a = adam_args(c, th, loss_scale);
+    const AdamArgs b = adam_args(c, ph, loss_scale);
+    for (const AdamArgs* o : {&a, &b}) {
+        AdamArgs mlp = *o;
+        mlp.n_params = o->grid_begin;
+        mlp.pair_w16 = mlp.pair_ema16 = nullptr;
+        launch_adam_ema(mlp, st);
+    }
+    launch_adam_ema_pair(a, b, entries, st);
+    net_pack(th, st);
+    net_pack(ph, st);
 }
 
 static const int kChunk = 1024;
@@ -1133,8 +1173,7 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
             net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
             net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
         }
-        net_update(ctx, th, 1024.0f, st);
-        net_update(ctx, ph, 1024.0f, st);
+        nets_update_pair(ctx, 1024.0f, st);
     } else {
         net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);
         net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);

@@ -258,6 +258,9 @@ struct AdamArgs {
     uint32_t* steps;
 };
 void launch_adam_ema(const AdamArgs& a, hipStream_t s);
+// Grid parameters [grid_begin, grid_begin + 2 n_entries) of two nets whose
+// grids share the interleaved tables (a: theta, b: phi), whole entries per thread.
+void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entries, hipStream_t s);
 
 // Pack fragments from fp16 parameters. which: 1 forward, 2 backward, 3 both.
 void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which, hipStream_t s);

@@ -961,12 +961,12 @@ void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const GridBucketArgs
 // ---------------------------------------------------------------------------
 // Adam (adam.h:47-120) fused with the half-precision EMA (ema.h:62-77).
 // ---------------------------------------------------------------------------
-__global__ void adam_ema_kernel(const AdamArgs a) {
+// Adam + EMA of parameter i; returns its new fp16 weight and EMA.
+__device__ __forceinline__ void adam_ema_param(const AdamArgs& a, int64_t i, _Float16& w16_out, _Float16& ema_out) {
 #pragma clang fp contract(off)
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_params) return;
     float gr = a.grad[i] / a.loss_scale;
     const bool matrix = i < a.n_matrix;
+    _Float16 w16;
     if (matrix || gr != 0.0f) {
         const float wf = a.master[i];
         if (matrix) gr += a.l2_reg * wf;
@@ -984,13 +984,24 @@ __global__ void adam_ema_kernel(const AdamArgs a) {
         const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
         const float nw = wf - elr * m;
         a.master[i] = nw;
-        a.w16[i] = (_Float16)nw;
+        w16 = (_Float16)nw;
+        a.w16[i] = w16;
+    } else {
+        w16 = a.w16[i];
     }
     const float e = (float)a.ema16[i];
-    const _Float16 w16 = a.w16[i];
     const float w = (float)w16;
     const _Float16 ne = (_Float16)((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
     a.ema16[i] = ne;
+    w16_out = w16;
+    ema_out = ne;
+}
+
+__global__ void adam_ema_kernel(const AdamArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_params) return;
+    _Float16 w16, ne;
+    adam_ema_param(a, i, w16, ne);
     if (a.pair_w16 && i >= a.grid_begin) {
         // the interleaved theta/phi tables: entry {theta f0, f1, phi f0, f1}
         const int64_t l = i - a.grid_begin;
@@ -1000,9 +1011,92 @@ __global__ void adam_ema_kernel(const AdamArgs a) {
     }
 }
 
+// Two consecutive grid parameters (i, i + 1; i even) of one net: every input
+// loaded as one vector before anything is stored, so the updates of both nets
+// in one thread overlap their memory round trips.  Grid parameters only (no
+// l2 term; a zero gradient leaves the parameter and its moments unchanged).
+struct AdamPair {
+    float2 g, w, m, v;
+    uint2 st;
+    h2 w16, ema;
+};
+
+__device__ __forceinline__ AdamPair adam_pair_load(const AdamArgs& a, int64_t i) {
+    AdamPair p;
+    p.g = *(const float2*)(a.grad + i);
+    p.w = *(const float2*)(a.master + i);
+    p.m = *(const float2*)(a.m + i);
+    p.v = *(const float2*)(a.v + i);
+    p.st = *(const uint2*)(a.steps + i);
+    p.w16 = *(const h2*)(a.w16 + i);
+    p.ema = *(const h2*)(a.ema16 + i);
+    return p;
+}
+
+// One grid parameter of adam_ema_param, from loaded values.
+__device__ __forceinline__ void adam_grid_update(const AdamArgs& a, float g, float& w, float& m, float& v,
+                                                 uint32_t& st, _Float16& w16, _Float16& ema) {
+#pragma clang fp contract(off)
+    const float gr = g / a.loss_scale;
+    if (gr != 0.0f) {
+        const float gsq = gr * gr;
+        m = a.beta1 * m + (1.0f - a.beta1) * gr;
+        v = a.beta2 * v + (1.0f - a.beta2) * gsq;
+        st += 1;
+        float lr = a.lr;
+        lr *= sqrtf(1.0f - exp2f((float)st * a.log2_beta2)) / (1.0f - exp2f((float)st * a.log2_beta1));
+        const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
+        w = w - elr * m;
+        w16 = (_Float16)w;
+    }
+    const float e = (float)ema;
+    const float wv = (float)w16;
+    ema = (_Float16)((e * a.ema_decay * a.ema_debias_old + wv * (1.0f - a.ema_decay)) * a.ema_debias_new);
+}
+
+__device__ __forceinline__ void adam_pair_step(const AdamArgs& a, AdamPair& p) {
+    _Float16 w0 = p.w16[0], w1 = p.w16[1], e0 = p.ema[0], e1 = p.ema[1];
+    adam_grid_update(a, p.g.x, p.w.x, p.m.x, p.v.x, p.st.x, w0, e0);
+    adam_grid_update(a, p.g.y, p.w.y, p.m.y, p.v.y, p.st.y, w1, e1);
+    p.w16 = h2{w0, w1};
+    p.ema = h2{e0, e1};
+}
+
+__device__ __forceinline__ void adam_pair_store(const AdamArgs& a, int64_t i, const AdamPair& p) {
+    *(float2*)(a.master + i) = p.w;
+    *(float2*)(a.m + i) = p.m;
+    *(float2*)(a.v + i) = p.v;
+    *(uint2*)(a.steps + i) = p.st;
+    *(h2*)(a.w16 + i) = p.w16;
+    *(h2*)(a.ema16 + i) = p.ema;
+}
+
+// Grid parameters of the theta (a) and phi (b) nets together, one table
+// entry (F = 2 parameters of each) per thread: both interleaved tables get
+// whole 8-byte entries instead of one half from each net's pass.
+__global__ void adam_ema_pair_kernel(const AdamArgs a, const AdamArgs b, int64_t n_entries) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_entries) return;
+    const int64_t ia = a.grid_begin + 2 * e, ib = b.grid_begin + 2 * e;
+    AdamPair pa = adam_pair_load(a, ia);
+    AdamPair pb = adam_pair_load(b, ib);
+    adam_pair_step(a, pa);
+    adam_pair_step(b, pb);
+    adam_pair_store(a, ia, pa);
+    adam_pair_store(b, ib, pb);
+    *(h4*)(a.pair_w16 + 4 * e) = h4{pa.w16[0], pa.w16[1], pb.w16[0], pb.w16[1]};
+    *(h4*)(a.pair_ema16 + 4 * e) = h4{pa.ema[0], pa.ema[1], pb.ema[0], pb.ema[1]};
+}
+
 void launch_adam_ema(const AdamArgs& a, hipStream_t s) {
     if (a.n_params <= 0) return;
     hipLaunchKernelGGL(adam_ema_kernel, dim3((unsigned)((a.n_params + 255) / 256)), dim3(256), 0, s, a);
+}
+
+void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entries, hipStream_t s) {
+    if (n_entries <= 0) return;
+    hipLaunchKernelGGL(adam_ema_pair_kernel, dim3((unsigned)((n_entries + 255) / 256)), dim3(256), 0, s, a, b,
+                       n_entries);
 }
 
 // ---------------------------------------------------------------------------

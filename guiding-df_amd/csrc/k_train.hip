@@ -328,11 +328,15 @@ void launch_train_bwd(const TrainBwdArgs& a, const NetSpec& hs, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Weight gradients dW_l = G_{l+1} A_l^T summed over the batch, on MFMA with the
-// batch as the K dimension.  Block (chunk c, layer l): wave w owns output rows
-// 16w..16w+15 and every column tile; partial sums go to partial[c] and are
-// reduced in chunk order, so the result does not depend on scheduling.
+// batch as the K dimension.  Block (chunk c, layer l): each of the 4 waves
+// takes a quarter of the chunk's K-steps and every output tile (all M rows, N
+// columns 64 at a time), so each fragment is loaded by one wave only; the
+// waves' sums are added in LDS in a fixed tree ((w0 + w2) + (w1 + w3)) and go
+// to partial[c], which is reduced in chunk order: the result does not depend
+// on scheduling.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void wgrad_kernel(const WgradArgs a) {
+__global__ __launch_bounds__(256, 2) void wgrad_kernel(const WgradArgs a) {
+    __shared__ __align__(16) float red[2][64 * 64];
     const NetSpec& ns = *a.ns;
     const int l = blockIdx.y;
     const int c = blockIdx.x;
@@ -342,50 +346,84 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const WgradArgs a) {
     const int H = ns.n_hidden;
     const int M = l == H ? ns.n_out_pad : 64;
     const int N = l == 0 ? ns.enc_width : 64;
-    if (wave * 16 >= M) return;
+    const int mt = M / 16, nt_n = N / 16;
     const size_t lay = (size_t)64 * a.ld;
     const _Float16* G = l == H ? a.gout_fm : a.g + (size_t)l * lay;        // g_{l+1}
     const _Float16* A = l == 0 ? a.x : a.act + (size_t)(l - 1) * lay;      // a_l
-    const int nt_n = N / 16;
-    f4 acc[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     const int64_t k0 = (int64_t)c * a.chunk;
     int64_t k1 = k0 + a.chunk;
     if (k1 > a.ld) k1 = a.ld;
-    const _Float16* gr = G + (size_t)(wave * 16 + i16) * a.ld;
-    // U K-steps per iteration, all of their fragments loaded before the first
-    // product; the products run in K order.  k1 - k0 is a multiple of 64.
-    auto steps = [&](int64_t kb, auto U) {
-        constexpr int NU = decltype(U)::value;
-        h8 af[NU], bf[NU][8];
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-            af[u] = *(const h8*)(gr + kb + 32 * u + g * 8);
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (t < nt_n) bf[u][t] = *(const h8*)(A + (size_t)(t * 16 + i16) * a.ld + kb + 32 * u + g * 8);
-        }
-#pragma unroll
-        for (int u = 0; u < NU; ++u)
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (t < nt_n) acc[t] = mfma16(af[u], bf[u][t], acc[t]);
-    };
-    int64_t kb = k0;
-    for (; kb + 128 <= k1; kb += 128) steps(kb, std::integral_constant<int, 4>{});
-    if (kb < k1) steps(kb, std::integral_constant<int, 2>{});
+    const int nsteps = (int)((k1 - k0) / 32);
+    const int s_lo = wave * nsteps / 4, s_hi = (wave + 1) * nsteps / 4;
     float* out = a.partial + (size_t)c * ns.n_matrix_params + ns.layer_off[l];
+    for (int n0 = 0; n0 < nt_n; n0 += 4) {
+        const int ntg = nt_n - n0 < 4 ? nt_n - n0 : 4;
+        f4 acc[4][4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        if (t < nt_n) {
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = wave * 16 + 4 * g + r;
-                const int n = t * 16 + i16;
-                out[(size_t)m * N + n] = acc[t][r];
+            for (int t = 0; t < 4; ++t) acc[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+        // U K-steps at a time, all of their fragments loaded before the first product
+        auto steps = [&](int s, auto U) {
+            constexpr int NU = decltype(U)::value;
+            h8 af[NU][4], bf[NU][4];
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int64_t kb = k0 + (int64_t)(s + u) * 32 + g * 8;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    if (m < mt) af[u][m] = *(const h8*)(G + (size_t)(m * 16 + i16) * a.ld + kb);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (t < ntg) bf[u][t] = *(const h8*)(A + (size_t)((n0 + t) * 16 + i16) * a.ld + kb);
             }
+#pragma unroll
+            for (int u = 0; u < NU; ++u)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (m < mt && t < ntg) acc[m][t] = mfma16(af[u][m], bf[u][t], acc[m][t]);
+        };
+        int s = s_lo;
+        for (; s + 2 <= s_hi; s += 2) steps(s, std::integral_constant<int, 2>{});
+        if (s < s_hi) steps(s, std::integral_constant<int, 1>{});
+        // acc[m][t][r]: row m*16 + 4g + r, column (n0 + t)*16 + i16 -> red[.][row*64 + t*16 + i16]
+        auto put = [&](float* dst) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dst[(m * 16 + 4 * g + r) * 64 + t * 16 + i16] = acc[m][t][r];
+        };
+        auto add = [&](const float* src) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[m][t][r] += src[(m * 16 + 4 * g + r) * 64 + t * 16 + i16];
+        };
+        if (wave >= 2) put(red[wave - 2]);
+        __syncthreads();
+        if (wave < 2) add(red[wave]);
+        __syncthreads();
+        if (wave == 1) put(red[0]);
+        __syncthreads();
+        if (wave == 0) {
+            add(red[0]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (m < mt && t < ntg) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            out[(size_t)(m * 16 + 4 * g + r) * N + (n0 + t) * 16 + i16] = acc[m][t][r];
+                    }
         }
+        __syncthreads();
     }
 }
 

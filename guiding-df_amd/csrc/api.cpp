@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <memory>
 #include <numeric>
 #include <sstream>
@@ -1086,10 +1087,19 @@ static AdamArgs adam_args(dfg_ctx* c, NetState& s, float loss_scale) {
     return o;
 }
 
-static void net_pack(NetState& s, hipStream_t st) {
-    launch_pack(s.d_spec.p, s.spec, s.w16.p, s.pack_train.p, 3, st);
-    launch_pack(s.d_spec.p, s.spec, s.ema16.p, s.pack_infer.p, 1, st);
+// The training (forward + backward) and inference (forward, EMA) packs of the
+// given nets, in one launch.
+static void nets_pack(std::initializer_list<NetState*> nets, hipStream_t st) {
+    PackJobs jobs;
+    std::memset(&jobs, 0, sizeof(jobs));
+    for (NetState* s : nets) {
+        jobs.job[jobs.n++] = PackJob{s->d_spec.p, s->w16.p, s->pack_train.p, 3, pack_slots(s->spec, 3)};
+        jobs.job[jobs.n++] = PackJob{s->d_spec.p, s->ema16.p, s->pack_infer.p, 1, pack_slots(s->spec, 1)};
+    }
+    launch_pack_jobs(jobs, st);
 }
+
+static void net_pack(NetState& s, hipStream_t st) { nets_pack({&s}, st); }
 
 static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st) {
     launch_adam_ema(adam_args(c, s, loss_scale), st);
@@ -1116,15 +1126,8 @@ static void nets_update_pair(dfg_ctx* c, float loss_scale, hipStream_t st) {
     }
     const AdamArgs a = adam_args(c, th, loss_scale);
     const AdamArgs b = adam_args(c, ph, loss_scale);
-    for (const AdamArgs* o : {&a, &b}) {
-        AdamArgs mlp = *o;
-        mlp.n_params = o->grid_begin;
-        mlp.pair_w16 = mlp.pair_ema16 = nullptr;
-        launch_adam_ema(mlp, st);
-    }
     launch_adam_ema_pair(a, b, entries, st);
-    net_pack(th, st);
-    net_pack(ph, st);
+    nets_pack({&th, &ph}, st);
 }
 
 static const int kChunk = 1024;

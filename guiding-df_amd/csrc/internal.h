@@ -259,11 +259,26 @@ struct AdamArgs {
 };
 void launch_adam_ema(const AdamArgs& a, hipStream_t s);
 // Grid parameters [grid_begin, grid_begin + 2 n_entries) of two nets whose
-// grids share the interleaved tables (a: theta, b: phi), whole entries per thread.
+// grids share the interleaved tables (a: theta, b: phi), whole entries per
+// thread, and in the same launch their MLP parameters [0, grid_begin).
 void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entries, hipStream_t s);
 
 // Pack fragments from fp16 parameters. which: 1 forward, 2 backward, 3 both.
 void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which, hipStream_t s);
+int64_t pack_slots(const NetSpec& hs, int which);
+// Up to four packs in one launch.
+struct PackJob {
+    const NetSpec* dns;
+    const _Float16* w16;
+    _Float16* pack;
+    int which;
+    int64_t slots;  // pack_slots(host spec, which)
+};
+struct PackJobs {
+    PackJob job[4];
+    int n;
+};
+void launch_pack_jobs(const PackJobs& jobs, hipStream_t s);
 void launch_f32_to_f16(const float* src, _Float16* dst, int64_t n, hipStream_t s);
 void launch_f16_to_f32(const _Float16* src, float* dst, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);

@@ -1112,8 +1112,18 @@ __device__ __forceinline__ void adam_pair_store(const AdamArgs& a, int64_t i, co
 // Grid parameters of the theta (a) and phi (b) nets together, one table
 // entry (F = 2 parameters of each) per thread: both interleaved tables get
 // whole 8-byte entries instead of one half from each net's pass.
+// Blocks past the entries update the MLP parameters [0, grid_begin) of a,
+// then of b (per parameter, as adam_ema_kernel).
 __global__ void adam_ema_pair_kernel(const AdamArgs a, const AdamArgs b, int64_t n_entries) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t eb = (n_entries + blockDim.x - 1) / blockDim.x * blockDim.x;
+    if (e >= eb) {
+        int64_t i = e - eb;
+        _Float16 w16, ne;
+        if (i < a.grid_begin) adam_ema_param(a, i, w16, ne);
+        else if ((i -= a.grid_begin) < b.grid_begin) adam_ema_param(b, i, w16, ne);
+        return;
+    }
     if (e >= n_entries) return;
     const int64_t ia = a.grid_begin + 2 * e, ib = b.grid_begin + 2 * e;
     AdamPair pa = adam_pair_load(a, ia);
@@ -1132,9 +1142,9 @@ void launch_adam_ema(const AdamArgs& a, hipStream_t s) {
 }
 
 void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entries, hipStream_t s) {
-    if (n_entries <= 0) return;
-    hipLaunchKernelGGL(adam_ema_pair_kernel, dim3((unsigned)((n_entries + 255) / 256)), dim3(256), 0, s, a, b,
-                       n_entries);
+    const int64_t blocks = (n_entries + 255) / 256 + (a.grid_begin + b.grid_begin + 255) / 256;
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(adam_ema_pair_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n_entries);
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,10 +1154,9 @@ void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entrie
 // (k = 16(2s + j/4) + 4g + j%4) when it comes from the previous layer's
 // accumulators.
 // ---------------------------------------------------------------------------
-__global__ void pack_kernel(const NetSpec* __restrict__ dns, const _Float16* __restrict__ w16,
-                            _Float16* __restrict__ pack, int which) {
-    const NetSpec& ns = *dns;
-    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one h8 per thread
+// One h8 slot of a pack (which: 1 forward, 2 backward, 3 both).
+__device__ __forceinline__ void pack_slot(const NetSpec& ns, const _Float16* __restrict__ w16,
+                                          _Float16* __restrict__ pack, int which, int64_t slot) {
     const int L = ns.n_hidden + 1;
     const uint32_t h = (uint32_t)slot * 8;
     const uint32_t end = (which & 2) ? ns.bwd_off[L] : ns.fwd_off[L];
@@ -1192,14 +1201,38 @@ __global__ void pack_kernel(const NetSpec* __restrict__ dns, const _Float16* __r
     *(h8*)(pack + pos) = v;
 }
 
-void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which,
-                 hipStream_t s) {
+__global__ void pack_kernel(const NetSpec* __restrict__ dns, const _Float16* __restrict__ w16,
+                            _Float16* __restrict__ pack, int which) {
+    pack_slot(*dns, w16, pack, which, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Several packs in one launch: job blockIdx.y.
+__global__ void pack_jobs_kernel(const PackJobs jobs) {
+    const PackJob& j = jobs.job[blockIdx.y];
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= j.slots) return;
+    pack_slot(*j.dns, j.w16, j.pack, j.which, slot);
+}
+
+int64_t pack_slots(const NetSpec& hs, int which) {
     const int L = hs.n_hidden + 1;
     const uint32_t end = (which & 2) ? hs.bwd_off[L] : hs.fwd_off[L];
     const uint32_t begin = (which & 1) ? 0u : hs.bwd_off[0];
-    const int64_t slots = (int64_t)(end - begin) / 8;
+    return (int64_t)(end - begin) / 8;
+}
+
+void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which,
+                 hipStream_t s) {
+    const int64_t slots = pack_slots(hs, which);
     if (slots <= 0) return;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, dns, w16, pack, which);
+}
+
+void launch_pack_jobs(const PackJobs& jobs, hipStream_t s) {
+    int64_t mx = 0;
+    for (int k = 0; k < jobs.n; ++k) mx = jobs.job[k].slots > mx ? jobs.job[k].slots : mx;
+    if (mx <= 0 || jobs.n <= 0) return;
+    hipLaunchKernelGGL(pack_jobs_kernel, dim3((unsigned)((mx + 255) / 256), (unsigned)jobs.n), dim3(256), 0, s, jobs);
 }
 
 // ---------------------------------------------------------------------------

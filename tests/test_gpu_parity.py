@@ -507,6 +507,34 @@ def test_train_steps_are_deterministic(torch_cuda):
         f.close()
 
 
+def test_wgrad_chunk_rows_only_reorders_sums(torch_cuda):
+    """Another weight-gradient chunk size changes only the order of the fp32
+    sums: the matrix gradients agree to 1e-5 (relative norm) and the grid
+    gradients, which do not depend on it, are bit-identical."""
+    torch = torch_cuda
+    import copy
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=15)
+    cfg_b = copy.deepcopy(cfg)
+    cfg_b["dfg"]["wgrad_chunk_rows"] = 192
+    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg_b)
+    B = 5000  # the last 192-row chunk is ragged
+    feat = dev(torch, df_features(B, 70, nrc=True))
+    tgt = dev(torch, train_targets(B, 71))
+    rgb = dev(torch, np.exp(np.random.default_rng(72).normal(0, 2, (B, 3))).astype(np.float32))
+    for f in (fa, fb):
+        f.train_step(feat, tgt)
+        f.nrc_train_step(feat, rgb)
+    torch.cuda.synchronize()
+    th, ph, nr = O.config_layouts(cfg)
+    for k, L in ((0, th), (1, ph), (2, nr)):
+        ga, gb = fa.get_grad(k).astype(np.float64), fb.get_grad(k).astype(np.float64)
+        nm = L["n_matrix"]
+        assert _grad_rel(gb[:nm], ga[:nm]) <= 1e-5, "net %d matrix grads differ" % k
+        assert np.array_equal(ga[nm:], gb[nm:]), "net %d grid grads differ" % k
+    fa.close()
+    fb.close()
+
+
 def test_plain_nrc_train_step_vs_fp64(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
@@ -564,6 +592,12 @@ def test_errors(torch_cuda):
         bad["dfg"]["n_bins_theta"] = 24
         G.GuidingField(bad)
     assert e.value.code == 3
+    for rows in (0, 96, 1 << 17):
+        with pytest.raises(G.DfgError) as e:
+            bad = G.hash_config()
+            bad["dfg"]["wgrad_chunk_rows"] = rows
+            G.GuidingField(bad)
+        assert e.value.code == 3 and "wgrad_chunk_rows" in str(e.value)
     f = G.GuidingField(G.hash_config(with_nrc=False))
     feat = dev(torch, df_features(10, 1))
     with pytest.raises(G.DfgError) as e:

@@ -1031,14 +1031,19 @@ static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, int64_
     NetState& th = c->net[0];
     NetState& ph = c->net[1];
     if (!c->pair_ok || !c->grid_bwd_bucketed || th.spec.grid.interp == INTERP_NEAREST) return false;
-    const GridBucketPlan plan = grid_bucket_plan(th.spec, n);
+    // one record per corner for both nets: entry + float4 values, 2,048-entry slices
+    const GridBucketPlan plan = grid_bucket_plan(th.spec, n, 4);
     if (!plan.ok) return false;
     ensure_bucket_ws(c, plan, 2, 2);
-    c->gb_val2.ensure((size_t)plan.records * 3);
+    c->gb_idx.ensure((size_t)plan.records);
+    c->gb_val.ensure((size_t)plan.records * 4);
     uint32_t* mx = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
-    const GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
-    const GridBucketArgs b = bucket_args(c, ph, feat, stride, n, ld, plan, c->gb_val2.p, mx + DFG_MAX_LEVELS);
-    launch_grid_bwd_bucketed_pair(a, b, th.spec, plan, st);
+    GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
+    a.dx2 = ph.dx.p;
+    a.grad2 = ph.grad.p;
+    a.param_begin2 = ph.spec.grid.param_begin;
+    a.lvl_max2 = mx + DFG_MAX_LEVELS;
+    launch_grid_bwd_bucketed_pair(a, th.spec, plan, st);
     return true;
 }
 

@@ -123,7 +123,7 @@ struct dfg_ctx {
     // bucketed grid-backward workspace (shared by the nets, used in stream order)
     DevBuf<int64_t> gb_offs, gb_cursor;
     DevBuf<uint32_t> gb_idx, gb_cnt;
-    DevBuf<float> gb_val, gb_val2;  // records (gb_val2: phi's, paired backward)
+    DevBuf<float> gb_val;  // record values (paired backward: float4 per record)
     DevBuf<int4> gb_items;
     DevBuf<int> gb_n_items;
     DevBuf<unsigned long long> gb_part;  // fixed-point partial sums of multi-part slices
@@ -134,7 +134,7 @@ struct dfg_ctx {
         if (side) (void)hipStreamDestroy(side);
         tab_pair.release();
         tab_pair_train.release();
-        gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release(); gb_val2.release();
+        gb_offs.release(); gb_cursor.release(); gb_idx.release(); gb_cnt.release(); gb_val.release();
         gb_items.release(); gb_n_items.release(); gb_part.release(); gb_multi.release();
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();

@@ -206,7 +206,8 @@ struct GridBucketPlan {
     int64_t max_partials;    // bound on the parts of slices cut into several parts
     int64_t max_multi;       // bound on the slices cut into several parts
 };
-GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n);
+// values_per_entry: the grid's F, or 4 for the paired theta/phi records.
+GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n, int values_per_entry = 0);
 struct GridBucketArgs {
     const NetSpec* ns;
     const float* feat;
@@ -228,15 +229,19 @@ struct GridBucketArgs {
     uint32_t* lvl_max;       // [levels] bits of max |update| per level (zeroed with offs)
     unsigned long long* partial;  // [max_partials][kGridSliceFloats] fixed-point sums of
                                   // the parts of slices cut into several parts
-    // second net of a paired scatter (same grid structure): its dL/dy, records
-    // and level maxima (null otherwise)
+    // second net of the paired backward (same grid structure): its dL/dy,
+    // gradient, first grid parameter and level maxima; the records are then
+    // rec_idx (entry in slice) + rec_val as float4 {g0, g1, g0', g1'}
     const float* dx2;
-    float* rec_val2;
+    float* grad2;
+    int64_t param_begin2;
     uint32_t* lvl_max2;
 };
 void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s);
-void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const GridBucketArgs& b, const NetSpec& hs,
-                                   const GridBucketPlan& p, hipStream_t s);
+// Theta and phi together (F = 2 grids of identical structure, not nearest):
+// a.*2 describe phi; p from grid_bucket_plan(hs, n, 4).
+void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,
+                                   hipStream_t s);
 
 struct AdamArgs {
     int64_t n_params, n_matrix;

@@ -526,8 +526,9 @@ __device__ __forceinline__ float unfix64(unsigned long long q, int S) {
 
 // Scale exponent of slice s (level `level`); finite = false when the level's
 // largest update is inf or NaN.
-__device__ __forceinline__ int slice_scale(const GridBucketArgs& a, int level, int s, bool& finite) {
-    const float mx = __uint_as_float(a.lvl_max[level]);
+__device__ __forceinline__ int slice_scale(const GridBucketArgs& a, const uint32_t* lvl_max, int level, int s,
+                                           bool& finite) {
+    const float mx = __uint_as_float(lvl_max[level]);
     finite = mx <= 3.402823466e+38f;
     if (!finite || !(mx > 0.0f)) return 0;
     int e;
@@ -710,9 +711,9 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 // fixed-point reduce does not see).
 //
 // PAIR: the theta and phi grids of identical structure in one pass (same
-// positions, so the same corners, slices and record positions): the second
-// net's records {entry, w * dL/dy2[0], w * dL/dy2[1]} go to rec_val2 at the
-// positions of the first net's, and its level maxima to lvl_max2.
+// positions, so the same corners, slices and record positions): one record
+// per corner, the entry in rec_idx and {w dL/dy[0], w dL/dy[1], w dL/dy2[0],
+// w dL/dy2[1]} as a float4 in rec_val, and each net's level maxima.
 constexpr int kStageRows = 256;
 template <bool PAIR>
 __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
@@ -806,10 +807,12 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         for (int i = t; i < nrec; i += blockDim.x) {
             const uint4 r = stage[i];
             const int64_t pos = (int64_t)next[r.w] + (int64_t)(i - (int)loff[r.w]);
-            ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
             if constexpr (PAIR) {
                 const uint2 q = stage2[i];
-                ((uint3*)a.rec_val2)[pos] = make_uint3(r.x, q.x, q.y);
+                a.rec_idx[pos] = r.x;
+                ((uint4*)a.rec_val)[pos] = make_uint4(r.y, r.z, q.x, q.y);
+            } else {
+                ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
             }
         }
         __syncthreads();
@@ -844,18 +847,20 @@ __device__ __forceinline__ void slice_extent(const GridBucketArgs& a, int s, int
 // as fixed-point integers (8 records in flight per thread), then write the
 // slice (single part) or the part's partial sums (several parts).
 __global__ __launch_bounds__(1024) void grid_bwd_reduce_kernel(const GridBucketArgs a) {
-    extern __shared__ unsigned long long acc[];  // [entries of the slice][F]
+    // [F][NE]: feature-major, so random entries of a wave spread over the banks
+    extern __shared__ unsigned long long acc[];
     if ((int)blockIdx.x >= *a.n_items) return;
     const int4 item = a.items[blockIdx.x];
     const GridSpec& gs = a.ns->grid;
     const int F = gs.n_feat;
+    const int NE = kGridSliceFloats / F;
     const int s = item.x;
     int level, nf;
     uint32_t e0;
     slice_extent(a, s, level, e0, nf);
     bool finite;
-    const int S = slice_scale(a, level, s, finite);
-    for (int i = threadIdx.x; i < nf; i += blockDim.x) acc[i] = 0ull;
+    const int S = slice_scale(a, a.lvl_max, level, s, finite);
+    for (int i = threadIdx.x; i < kGridSliceFloats; i += blockDim.x) acc[i] = 0ull;
     __syncthreads();
     const int64_t se = a.offs[s + 1];
     const int64_t pb = a.offs[s] + (int64_t)item.y * kGridPartRecords;
@@ -874,25 +879,26 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_kernel(const GridBucketA
 #pragma unroll
                 for (int j = 0; j < U; ++j) {
                     if (p0 + (int64_t)j * blockDim.x < pe) {
-                        atomicAdd(&acc[2 * r[j].x], (unsigned long long)fix64(__uint_as_float(r[j].y), S));
-                        atomicAdd(&acc[2 * r[j].x + 1], (unsigned long long)fix64(__uint_as_float(r[j].z), S));
+                        atomicAdd(&acc[r[j].x], (unsigned long long)fix64(__uint_as_float(r[j].y), S));
+                        atomicAdd(&acc[NE + r[j].x], (unsigned long long)fix64(__uint_as_float(r[j].z), S));
                     }
                 }
             }
         } else {
             for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
                 const uint32_t e = a.rec_idx[p];
-                for (int f = 0; f < F; ++f) atomicAdd(&acc[e * F + f], (unsigned long long)fix64(a.rec_val[p * F + f], S));
+                for (int f = 0; f < F; ++f) atomicAdd(&acc[f * NE + e], (unsigned long long)fix64(a.rec_val[p * F + f], S));
             }
         }
     }
     __syncthreads();
     if (item.z == 1) {
         float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
-        for (int i = threadIdx.x; i < nf; i += blockDim.x) g[i] = finite ? unfix64(acc[i], S) : __builtin_nanf("");
+        for (int i = threadIdx.x; i < nf; i += blockDim.x)
+            g[i] = finite ? unfix64(acc[(i % F) * NE + i / F], S) : __builtin_nanf("");
     } else {
         unsigned long long* pp = a.partial + (size_t)item.w * kGridSliceFloats;
-        for (int i = threadIdx.x; i < nf; i += blockDim.x) pp[i] = acc[i];
+        for (int i = threadIdx.x; i < kGridSliceFloats; i += blockDim.x) pp[i] = acc[i];
     }
 }
 
@@ -908,34 +914,142 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_kernel(const GridBucketA
     uint32_t e0;
     slice_extent(a, m.x, level, e0, nf);
     bool finite;
-    const int S = slice_scale(a, level, m.x, finite);
+    const int S = slice_scale(a, a.lvl_max, level, m.x, finite);
     float* g = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * F;
     const unsigned long long* pp = a.partial + (size_t)m.y * kGridSliceFloats;
+    const int NE = kGridSliceFloats / F;  // partial sums are [F][NE]
     const int stripe = (nf + gridDim.y - 1) / gridDim.y;
     const int i0 = blockIdx.y * stripe;
     const int i1 = min(nf, i0 + stripe);
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int j = (i % F) * NE + i / F;
         unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, q3 = 0ull;
         int p = 0;
         for (; p + 4 <= m.z; p += 4) {
-            q0 += pp[(size_t)p * kGridSliceFloats + i];
-            q1 += pp[(size_t)(p + 1) * kGridSliceFloats + i];
-            q2 += pp[(size_t)(p + 2) * kGridSliceFloats + i];
-            q3 += pp[(size_t)(p + 3) * kGridSliceFloats + i];
+            q0 += pp[(size_t)p * kGridSliceFloats + j];
+            q1 += pp[(size_t)(p + 1) * kGridSliceFloats + j];
+            q2 += pp[(size_t)(p + 2) * kGridSliceFloats + j];
+            q3 += pp[(size_t)(p + 3) * kGridSliceFloats + j];
         }
-        for (; p < m.z; ++p) q0 += pp[(size_t)p * kGridSliceFloats + i];
+        for (; p < m.z; ++p) q0 += pp[(size_t)p * kGridSliceFloats + j];
         g[i] = finite ? unfix64(q0 + q1 + q2 + q3, S) : __builtin_nanf("");
     }
 }
 
-GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n) {
+// Paired theta/phi reduce: one block per (slice, part) of the 20-byte records
+// (rec_idx entry, rec_val float4); LDS sums [entry][4] (theta f0, f1, phi f0,
+// f1), each net at its own fixed-point scale; writes both nets' slices (single
+// part) or the part's partial sums.
+__global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBucketArgs a) {
+    extern __shared__ unsigned long long acc[];  // [4][NE] (value-major: no bank conflicts)
+    constexpr int NE = kGridSliceFloats / 4;
+    if ((int)blockIdx.x >= *a.n_items) return;
+    const int4 item = a.items[blockIdx.x];
+    const GridSpec& gs = a.ns->grid;
+    const int s = item.x;
+    int level, nf2;
+    uint32_t e0;
+    slice_extent(a, s, level, e0, nf2);
+    const int ne = nf2 / 2, nv = ne * 4;
+    bool fin_a, fin_b;
+    const int Sa = slice_scale(a, a.lvl_max, level, s, fin_a);
+    const int Sb = slice_scale(a, a.lvl_max2, level, s, fin_b);
+    (void)nv;
+    for (int i = threadIdx.x; i < kGridSliceFloats; i += blockDim.x) acc[i] = 0ull;
+    __syncthreads();
+    const int64_t se = a.offs[s + 1];
+    const int64_t pb = a.offs[s] + (int64_t)item.y * kGridPartRecords;
+    const int64_t pe = pb + kGridPartRecords < se ? pb + kGridPartRecords : se;
+    const float4* r4 = (const float4*)a.rec_val;
+    constexpr int U = 4;
+    for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
+        uint32_t e[U];
+        float4 v[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int64_t p = p0 + (int64_t)j * blockDim.x;
+            const bool in = p < pe;
+            e[j] = in ? a.rec_idx[p] : 0u;
+            v[j] = in ? r4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            if (p0 + (int64_t)j * blockDim.x < pe) {
+                unsigned long long* q = acc + e[j];
+                if (fin_a) {
+                    atomicAdd(q, (unsigned long long)fix64(v[j].x, Sa));
+                    atomicAdd(q + NE, (unsigned long long)fix64(v[j].y, Sa));
+                }
+                if (fin_b) {
+                    atomicAdd(q + 2 * NE, (unsigned long long)fix64(v[j].z, Sb));
+                    atomicAdd(q + 3 * NE, (unsigned long long)fix64(v[j].w, Sb));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (item.z == 1) {
+        float* ga = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * 2;
+        float* gb = a.grad2 + a.param_begin2 + ((size_t)gs.offset[level] + e0) * 2;
+        for (int i = threadIdx.x; i < nf2; i += blockDim.x) {
+            const int en = i >> 1, f = i & 1;
+            ga[i] = fin_a ? unfix64(acc[f * NE + en], Sa) : __builtin_nanf("");
+            gb[i] = fin_b ? unfix64(acc[(2 + f) * NE + en], Sb) : __builtin_nanf("");
+        }
+    } else {
+        unsigned long long* pp = a.partial + (size_t)item.w * kGridSliceFloats;
+        for (int i = threadIdx.x; i < kGridSliceFloats; i += blockDim.x) pp[i] = acc[i];
+    }
+}
+
+// Multi-part slices of the paired records: block (k, y) adds, for slice k and
+// the y-th stripe of its values, the parts' partial sums in part order.
+__global__ __launch_bounds__(256) void grid_bwd_combine_pair_kernel(const GridBucketArgs a) {
+    if ((int)blockIdx.x >= a.n_items[1]) return;
+    const int4 m = a.multi[blockIdx.x];
+    const GridSpec& gs = a.ns->grid;
+    int level, nf2;
+    uint32_t e0;
+    slice_extent(a, m.x, level, e0, nf2);
+    constexpr int NE = kGridSliceFloats / 4;  // partial sums are [4][NE]
+    const int nv = nf2 * 2;  // output values: theta's nf2, then phi's nf2
+    bool fin_a, fin_b;
+    const int Sa = slice_scale(a, a.lvl_max, level, m.x, fin_a);
+    const int Sb = slice_scale(a, a.lvl_max2, level, m.x, fin_b);
+    float* ga = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * 2;
+    float* gb = a.grad2 + a.param_begin2 + ((size_t)gs.offset[level] + e0) * 2;
+    const unsigned long long* pp = a.partial + (size_t)m.y * kGridSliceFloats;
+    const int stripe = (nv + gridDim.y - 1) / gridDim.y;
+    const int i0 = blockIdx.y * stripe;
+    const int i1 = min(nv, i0 + stripe);
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const bool second = i >= nf2;
+        const int o = second ? i - nf2 : i;  // output index in the net's slice: 2 en + f
+        const int j = ((second ? 2 : 0) + (o & 1)) * NE + (o >> 1);
+        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, q3 = 0ull;
+        int p = 0;
+        for (; p + 4 <= m.z; p += 4) {
+            q0 += pp[(size_t)p * kGridSliceFloats + j];
+            q1 += pp[(size_t)(p + 1) * kGridSliceFloats + j];
+            q2 += pp[(size_t)(p + 2) * kGridSliceFloats + j];
+            q3 += pp[(size_t)(p + 3) * kGridSliceFloats + j];
+        }
+        for (; p < m.z; ++p) q0 += pp[(size_t)p * kGridSliceFloats + j];
+        const unsigned long long q = q0 + q1 + q2 + q3;
+        if (!second) ga[o] = fin_a ? unfix64(q, Sa) : __builtin_nanf("");
+        else gb[o] = fin_b ? unfix64(q, Sb) : __builtin_nanf("");
+    }
+}
+
+GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n, int values_per_entry) {
     GridBucketPlan p{};
     const GridSpec& g = hs.grid;
     const int F = g.n_feat;
     p.ok = hs.grid.present && (F == 1 || F == 2 || F == 4 || F == 8);
     if (!p.ok) return p;
+    const int V = values_per_entry > 0 ? values_per_entry : F;
     int shift = 0;
-    while ((1 << (shift + 1)) * F <= kGridSliceFloats) ++shift;
+    while ((1 << (shift + 1)) * V <= kGridSliceFloats) ++shift;
     p.slice_shift = shift;
     int maxl = 0;
     for (int l = 0; l < g.n_levels; ++l) {
@@ -971,29 +1085,22 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
 }
 
-// theta (a) and phi (b) grids of identical structure (F = 2, not nearest): one
-// count, scan and staged scatter serve both (their corners, slices and record
-// positions coincide); then each net's reduce and combine.  b shares a's slice
-// offsets, cursors, work list and partial-sum buffer, and has its own records,
-// level maxima, dL/dy and gradient.
-void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const GridBucketArgs& b, const NetSpec& hs,
-                                   const GridBucketPlan& p, hipStream_t s) {
+// Theta and phi grids of identical structure (F = 2, not nearest) in one
+// bucketed pass: count, scan, the paired staged scatter (one 20-byte record
+// per corner for both nets), then one reduce and combine writing both
+// gradients.
+void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,
+                                   hipStream_t s) {
     if (a.n <= 0 || !p.ok) return;
     const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
     // int64 slice totals, then both nets' per-level maxima right after them
     launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + 2 * DFG_MAX_LEVELS, s);
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
-    GridBucketArgs ab = a;
-    ab.dx2 = b.dx;
-    ab.rec_val2 = b.rec_val;
-    ab.lvl_max2 = b.lvl_max;
-    hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024), 0, s, ab);
+    hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
-    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
-    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(grid_bwd_reduce_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, b);
-    hipLaunchKernelGGL(grid_bwd_combine_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(grid_bwd_reduce_pair_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
+    hipLaunchKernelGGL(grid_bwd_combine_pair_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

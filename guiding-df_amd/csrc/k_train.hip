@@ -739,6 +739,26 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
     const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
     const uint32_t emask = (1u << a.slice_shift) - 1u;
     uint32_t umax = 0u, umax2 = 0u;
+    // this thread's row inputs, loaded one sub-chunk ahead (the loads of the
+    // next sub-chunk are in flight while this one is written out)
+    struct RowIn { float d0, d1, e0, e1, px, py, pz; };
+    auto load_row = [&](int64_t row) {
+        RowIn v{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (row < r1) {
+            v.d0 = a.dx[(size_t)(level * 2) * a.ld + row];
+            v.d1 = a.dx[(size_t)(level * 2 + 1) * a.ld + row];
+            if constexpr (PAIR) {
+                v.e0 = a.dx2[(size_t)(level * 2) * a.ld + row];
+                v.e1 = a.dx2[(size_t)(level * 2 + 1) * a.ld + row];
+            }
+            const float* fr = a.feat + row * a.stride + a.pos_dim;
+            v.px = fr[0];
+            v.py = fr[1];
+            v.pz = fr[2];
+        }
+        return v;
+    };
+    RowIn cur = load_row(r0 + (t >> 2));
     for (int64_t s0 = r0; s0 < r1; s0 += kStageRows) {
         for (int i = t; i < nsl; i += blockDim.x) lcnt[i] = 0u;
         __syncthreads();
@@ -748,17 +768,10 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         uint4 rec[2];
         uint2 rec2[2];
         if (live) {
-            const float d0 = a.dx[(size_t)(level * 2) * a.ld + row];
-            const float d1 = a.dx[(size_t)(level * 2 + 1) * a.ld + row];
-            float e0 = 0.0f, e1 = 0.0f;
-            if constexpr (PAIR) {
-                e0 = a.dx2[(size_t)(level * 2) * a.ld + row];
-                e1 = a.dx2[(size_t)(level * 2 + 1) * a.ld + row];
-            }
-            const float* fr = a.feat + row * a.stride + a.pos_dim;
+            const float d0 = cur.d0, d1 = cur.d1, e0 = cur.e0, e1 = cur.e1;
             uint32_t idx[2];
             float w[2];
-            grid_corner_pair(gs, level, fr[0], fr[1], fr[2], t & 3, idx, w);
+            grid_corner_pair(gs, level, cur.px, cur.py, cur.pz, t & 3, idx, w);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 sl[j] = idx[j] >> a.slice_shift;
@@ -803,6 +816,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
             }
         }
         __syncthreads();
+        cur = load_row(s0 + kStageRows + (t >> 2));
         const int nrec = (int)((r1 - s0 < kStageRows ? r1 - s0 : kStageRows) * 8);
         for (int i = t; i < nrec; i += blockDim.x) {
             const uint4 r = stage[i];

@@ -983,8 +983,9 @@ static GridBucketArgs bucket_args(dfg_ctx* c, NetState& s, const float* feat, in
     return gb;
 }
 
-// Shared bucket buffers for a plan (offs carries `maxima` per-level maxima
-// arrays after the slice offsets).
+// Shared bucket buffers for a plan with F values per record (offs carries
+// `maxima` per-level maxima arrays after the slice offsets).  F == 2: 12-byte
+// records in gb_val; otherwise an entry array and F values per record.
 static void ensure_bucket_ws(dfg_ctx* c, const GridBucketPlan& plan, int F, int maxima) {
     c->gb_offs.ensure((size_t)plan.n_slices + 1 + (size_t)maxima * DFG_MAX_LEVELS / 2);
     c->gb_part.ensure((size_t)plan.max_partials * kGridSliceFloats);
@@ -1034,9 +1035,7 @@ static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, int64_
     // one record per corner for both nets: entry + float4 values, 2,048-entry slices
     const GridBucketPlan plan = grid_bucket_plan(th.spec, n, 4);
     if (!plan.ok) return false;
-    ensure_bucket_ws(c, plan, 2, 2);
-    c->gb_idx.ensure((size_t)plan.records);
-    c->gb_val.ensure((size_t)plan.records * 4);
+    ensure_bucket_ws(c, plan, 4, 2);  // entry array + 4 values per record
     uint32_t* mx = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
     GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
     a.dx2 = ph.dx.p;

@@ -394,6 +394,75 @@ __device__ __forceinline__ void sh_encode(int degree, float ix, float iy, float 
     out(15, 0.59004358992664352f * x * (-x2 + 3.0f * y2));
 }
 
+// Non-grid parts of encode_row.  NC > 0: the part's bin / frequency count
+// as a compile-time constant (its loop unrolled, constant output offsets);
+// NC == 0: the count from the part.  Both evaluate the same operations.
+// (Unrolling the dimension loops as well measured slower: more registers.)
+
+// kernel_one_blob_soa, tcnn oneblob.h:98-127: bin k gets S(b_{k+1}) - S(b_k),
+// S(b) = sum over shifts s in {0,-1,+1} of Q(b - x + s).  Q is monotone and
+// saturates (0 / 1) outside |t| < 1/n_bins, so for n_bins >= 2 only the shift
+// nearest zero is unsaturated: S(b) = (1 + r) + Q(t - r), t = b - x,
+// r = rint(t) clamped to [-1, 1] (t - r is exact, so Q sees the same argument
+// as in the three-term sum).
+template <int NC, typename In, typename Out>
+__device__ __forceinline__ void oneblob_part(const EncPart& P, In& in, Out& out, int ob) {
+#pragma clang fp contract(off)
+    const int nb = NC > 0 ? NC : P.param;
+    const float fnb = (float)nb;
+    for (int d = 0; d < P.n_dims; ++d) {
+        const float x = in(P.dim_begin + d);
+        auto S = [&](float b) {
+            const float t = b - x;
+            if (nb < 2) return quartic_cdf(t, fnb) + quartic_cdf(t - 1.0f, fnb) + quartic_cdf(t + 1.0f, fnb);
+            const float r = __builtin_amdgcn_fmed3f(__builtin_rintf(t), -1.0f, 1.0f);
+            return (1.0f + r) + quartic_cdf(t - r, fnb);
+        };
+        float left = S(0.0f);
+#pragma unroll
+        for (int k = 0; k < (NC > 0 ? NC : nb); ++k) {
+            const float right = S((float)(k + 1) / fnb);
+            out(ob + d * nb + k, right - left);
+            left = right;
+        }
+    }
+}
+
+// triangle_wave_encoding, tcnn triangle_wave.h:46-82
+template <int NC, typename In, typename Out>
+__device__ __forceinline__ void triwave_part(const EncPart& P, In& in, Out& out, int ob) {
+#pragma clang fp contract(off)
+    const int nf = NC > 0 ? NC : P.param;
+    for (int d = 0; d < P.n_dims; ++d) {
+        const float xin = in(P.dim_begin + d);
+#pragma unroll
+        for (int f = 0; f < (NC > 0 ? NC : nf); ++f) {
+            const float x = ldexpf(xin, f - 1);
+            const float val = x + (float)f * 0.25f;
+            out(ob + d * nf + f, fabsf(val - floorf(val) - 0.5f) * 4.0f - 1.0f);
+        }
+    }
+}
+
+// frequency_encoding, tcnn frequency.h:46-94:
+// sin(2^f pi x + (j%2) pi/2) = sin(2 pi r), r = frac(x 2^(f-1) + (j%2)/4);
+// the reduction to one turn is exact, then v_sin_f32 (whose argument is in
+// turns) evaluates the reduced angle.
+template <int NC, typename In, typename Out>
+__device__ __forceinline__ void frequency_part(const EncPart& P, In& in, Out& out, int ob) {
+#pragma clang fp contract(off)
+    const int nf = NC > 0 ? NC : P.param;
+    for (int d = 0; d < P.n_dims; ++d) {
+        const float xin = in(P.dim_begin + d);
+#pragma unroll
+        for (int j = 0; j < 2 * (NC > 0 ? NC : nf); ++j) {
+            const float v = ldexpf(xin, (j >> 1) - 1) + (float)(j & 1) * 0.25f;
+            const float fr = v - floorf(v);
+            out(ob + d * 2 * nf + j, __builtin_amdgcn_sinf(fr));
+        }
+    }
+}
+
 // Whole composite encoding of one row.  In: callable in(d) -> float.
 // Out: callable out(col, float).  Writes enc_width columns and zero-fills up to
 // 32 * k_steps (the MFMA K padding, which has zero weights).
@@ -418,66 +487,23 @@ __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __
             case ENC_IDENTITY:
                 for (int d = 0; d < P.n_dims; ++d) out(ob + d, in(P.dim_begin + d) * P.id_scale + P.id_offset);
                 break;
-            case ENC_ONEBLOB: {
-                // kernel_one_blob_soa, tcnn oneblob.h:98-127: bin k gets
-                // S(b_{k+1}) - S(b_k), S(b) = sum over shifts s in {0,-1,+1} of
-                // Q(b - x + s).  Q is monotone and saturates (0 / 1) outside
-                // |t| < 1/n_bins, so for n_bins >= 2 only the shift nearest zero
-                // is unsaturated: S(b) = (1 + r) + Q(t - r), t = b - x,
-                // r = rint(t) clamped to [-1, 1] (t - r is exact, so Q sees the
-                // same argument as in the three-term sum).
-                const int nb = P.param;
-                const float fnb = (float)nb;
-                for (int d = 0; d < P.n_dims; ++d) {
-                    const float x = in(P.dim_begin + d);
-                    auto S = [&](float b) {
-                        const float t = b - x;
-                        if (nb < 2) return quartic_cdf(t, fnb) + quartic_cdf(t - 1.0f, fnb) + quartic_cdf(t + 1.0f, fnb);
-                        const float r = __builtin_amdgcn_fmed3f(__builtin_rintf(t), -1.0f, 1.0f);
-                        return (1.0f + r) + quartic_cdf(t - r, fnb);
-                    };
-                    float left = S(0.0f);
-                    for (int k = 0; k < nb; ++k) {
-                        const float right = S((float)(k + 1) / fnb);
-                        out(ob + d * nb + k, right - left);
-                        left = right;
-                    }
-                }
+            case ENC_ONEBLOB:
+                // the common 4 bins with the bin loop unrolled
+                if (P.param == 4) oneblob_part<4>(P, in, out, ob);
+                else oneblob_part<0>(P, in, out, ob);
                 break;
-            }
             case ENC_SH:
                 sh_encode(P.param, in(P.dim_begin), in(P.dim_begin + 1), in(P.dim_begin + 2),
                           [&](int j, float v) { out(ob + j, v); });
                 break;
-            case ENC_TRIWAVE: {
-                // triangle_wave_encoding, tcnn triangle_wave.h:46-82
-                const int nf = P.param;
-                for (int d = 0; d < P.n_dims; ++d) {
-                    const float xin = in(P.dim_begin + d);
-                    for (int f = 0; f < nf; ++f) {
-                        const float x = ldexpf(xin, f - 1);
-                        const float val = x + (float)f * 0.25f;
-                        out(ob + d * nf + f, fabsf(val - floorf(val) - 0.5f) * 4.0f - 1.0f);
-                    }
-                }
+            case ENC_TRIWAVE:
+                if (P.param == 12) triwave_part<12>(P, in, out, ob);
+                else triwave_part<0>(P, in, out, ob);
                 break;
-            }
-            case ENC_FREQUENCY: {
-                // frequency_encoding, tcnn frequency.h:46-94:
-                // sin(2^f pi x + (j%2) pi/2) = sin(2 pi r), r = frac(x 2^(f-1) + (j%2)/4);
-                // the reduction to one turn is exact, then v_sin_f32 (whose
-                // argument is in turns) evaluates the reduced angle.
-                const int nf = P.param;
-                for (int d = 0; d < P.n_dims; ++d) {
-                    const float xin = in(P.dim_begin + d);
-                    for (int j = 0; j < 2 * nf; ++j) {
-                        const float v = ldexpf(xin, (j >> 1) - 1) + (float)(j & 1) * 0.25f;
-                        const float fr = v - floorf(v);
-                        out(ob + d * 2 * nf + j, __builtin_amdgcn_sinf(fr));
-                    }
-                }
+            case ENC_FREQUENCY:
+                if (P.param == 12) frequency_part<12>(P, in, out, ob);
+                else frequency_part<0>(P, in, out, ob);
                 break;
-            }
             case ENC_GRID: {
                 const GridSpec& g = ns.grid;
                 const int F = g.n_feat;

@@ -221,8 +221,9 @@ struct GridBucketArgs {
     uint32_t* cnt;           // [n_slices][n_chunks] updates per slice and row chunk
     int64_t* offs;           // [n_slices + 1]: slice totals, then exclusive offsets
     int64_t* cursor;         // [n_slices] next free record of each slice
-    uint32_t* rec_idx;       // [records] entry within the slice (F != 2)
-    float* rec_val;          // F == 2: [records] 12-byte {entry, g0, g1}; else [records][F]
+    uint32_t* rec_idx;       // [records] entry within the slice (F != 2, and the paired records)
+    float* rec_val;          // F == 2: [records] 12-byte {entry, g0, g1}; paired: [records] float4;
+                             // else [records][F]
     int4* items;             // [max_items] (slice, part, parts of the slice, partial slot)
     int* n_items;            // [2]: work items, slices cut into several parts
     int4* multi;             // [max_multi] (slice, first partial slot, parts, -)

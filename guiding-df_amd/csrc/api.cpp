@@ -1028,7 +1028,10 @@ static void net_grid_backward(dfg_ctx* c, NetState& s, const float* feat, int st
 
 // Grid backward of both DF nets in one bucketed pass (interleaved-table
 // grids: F = 2, identical structure); false when that does not apply.
-static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, int64_t n, int64_t ld, hipStream_t st) {
+// pos: the batch's compact grid positions [n][3] (from the step's paired
+// pre-pass), read 16 times (once per level) instead of the strided features.
+static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, const float* pos, int64_t n, int64_t ld,
+                               hipStream_t st) {
     NetState& th = c->net[0];
     NetState& ph = c->net[1];
     if (!c->pair_ok || !c->grid_bwd_bucketed || th.spec.grid.interp == INTERP_NEAREST) return false;
@@ -1038,6 +1041,11 @@ static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, int64_
     ensure_bucket_ws(c, plan, 4, 2);  // entry array + 4 values per record
     uint32_t* mx = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
     GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
+    if (pos) {
+        a.feat = pos;
+        a.stride = 3;
+        a.pos_dim = 0;
+    }
     a.dx2 = ph.dx.p;
     a.grad2 = ph.grad.p;
     a.param_begin2 = ph.spec.grid.param_begin;
@@ -1176,7 +1184,8 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
         // both MLP backwards, then one grid backward for the two grids
         net_backward_mlp(th, B, ld, n_chunks, chunk, st);
         net_backward_mlp(ph, B, ld, n_chunks, chunk, st);
-        if (!grid_backward_pair(ctx, feat, feat_stride, B, ld, st)) {
+        // the compact positions of train_grid_pair are still in ctx->pos
+        if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
             net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
             net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
         }

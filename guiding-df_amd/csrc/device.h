@@ -1069,13 +1069,14 @@ __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float
 }
 
 // ---------------------------------------------------------------------------
-// XCD-aware grid encoding.  Workgroups are dispatched round-robin over the
-// XCDs, so block b runs on XCD b % n_groups.  Blocks are ordered level-major in
-// phases: in phase p, block b encodes level p * n_groups + b % n_groups for a
-// chunk of rows, so each XCD works on one level (<= 2 MB of fp16 table) at a
-// time and keeps it resident in its own L2, instead of streaming all 16 levels
-// of a 28.5 MB table from the Infinity Cache.  One row per thread; positions
-// from a compact [n][3] array; output feature-major fp32 [L*F][ld].
+// XCD-aware grid encoding.  Workgroups are dispatched round-robin over the 8
+// XCDs, so block b runs on XCD b % 8, and with n_groups dividing 8 the XCDs
+// with b % n_groups == k serve level group k.  Blocks are ordered level-major
+// in phases: in phase p, block b encodes level p * n_groups + b % n_groups for
+// a chunk of rows, so each XCD works on one level (<= 4 MB of interleaved
+// table) at a time and keeps it resident in its own L2, instead of streaming
+// all 16 levels from the Infinity Cache.  One row per thread; positions from a
+// compact [n][3] array; output feature-major fp32 [L*F][ld].
 // ---------------------------------------------------------------------------
 template <int F>
 __device__ __forceinline__ void grid_encode_block(const NetSpec& ns, const _Float16* __restrict__ table,

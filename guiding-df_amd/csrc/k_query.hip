@@ -388,7 +388,7 @@ void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2*
                              int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev,
                              int max_blocks) {
     if (n <= 0 || !hs.grid.present) return;
-    const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
+    const int groups = hs.grid.n_levels < kPrepassLevelGroups ? hs.grid.n_levels : kPrepassLevelGroups;
     const int phases = (hs.grid.n_levels + groups - 1) / groups;
     const int chunks = (int)((n + 255) / 256);
     const int64_t items = (int64_t)chunks * groups * phases;  // <= 2^26 for n <= 2^30
@@ -402,7 +402,7 @@ void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2*
 void launch_grid_encode(const NetSpec* dns, const NetSpec& hs, const _Float16* table, const float* pos, int64_t n,
                         int64_t ld, float* out, hipStream_t s, const int64_t* n_dev) {
     if (n <= 0 || !hs.grid.present) return;
-    const int groups = hs.grid.n_levels < kXcdGroups ? hs.grid.n_levels : kXcdGroups;
+    const int groups = hs.grid.n_levels < kPrepassLevelGroups ? hs.grid.n_levels : kPrepassLevelGroups;
     const int phases = (hs.grid.n_levels + groups - 1) / groups;
     const int chunks = (int)((n + 255) / 256);
     const unsigned blocks = (unsigned)((int64_t)chunks * groups * phases);

@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: 3 fused queries of the batch, then 3 copies of a 256 MiB buffer "
                          "(the calibration kernel of known traffic), and exit")
+    ap.add_argument("--pmc-train-pass", action="store_true",
+                    help="for counter runs: 3 training steps (DF + NRC) of the training batch, then 3 copies of a "
+                         "256 MiB buffer (the calibration kernel), and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -139,15 +142,25 @@ def main():
     def q_plain():
         field.query_plain(feat, u, wi, nrc_rgb=rgb, stream=stream)
 
-    if args.pmc_pass:
+    if args.pmc_train_pass:
+        B = args.train_batch
+        tf = torch.from_numpy(df_features(B, 3, nrc=True)).cuda()
+        tt = torch.from_numpy(train_targets(B, 4)).cuda()
+        trgb = torch.from_numpy(np.exp(np.random.default_rng(5).normal(0, 2, (B, 3))).astype(np.float32)).cuda()
+        tl = torch.zeros(2, dtype=torch.float32, device="cuda")
         for _ in range(3):
+            field.train_step(tf, tt, tl[0:1], stream=stream)
+            field.nrc_train_step(tf, trgb, tl[1:2], stream=stream)
+    if args.pmc_pass or args.pmc_train_pass:
+        for _ in range(3 if args.pmc_pass else 0):
             q_fused()
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"pmc_pass": True, "queries": n, "copy_bytes_read": src.numel() * 4,
+        print(json.dumps({"pmc_pass": True, "queries": n if args.pmc_pass else 0,
+                          "train_steps": 3 if args.pmc_train_pass else 0, "copy_bytes_read": src.numel() * 4,
                           "copy_bytes_written": dst.numel() * 4}))
         return
 

@@ -342,12 +342,25 @@ __device__ __forceinline__ void grid_level_pair(const GridSpec& g, const uint2* 
             }
             continue;
         }
+        // adjacent entries (always on dense levels, and on hashed levels when
+        // the x pair shares an aligned pair) come from one 16-byte load at
+        // the lower index (8-byte aligned); otherwise the aligned pair holding
+        // corner k0 and a separate load of corner k1
         const uint32_t i0 = c.idx[k0], i1 = c.idx[k1];
-        const uint4 pr = *(const uint4*)(lt + (i0 & ~1u));
+        const bool up = i1 == i0 + 1u, down = i1 + 1u == i0;
+        const uint32_t base = up ? i0 : (down ? i1 : (i0 & ~1u));
+        uint4 pr;
+        __builtin_memcpy(&pr, lt + base, sizeof(pr));
         const uint2 lo = make_uint2(pr.x, pr.y), hi = make_uint2(pr.z, pr.w);
-        const uint2 e0 = (i0 & 1u) ? hi : lo;
-        uint2 e1 = (i0 & 1u) ? lo : hi;
-        if (i1 != (i0 ^ 1u)) e1 = lt[i1];
+        uint2 e0, e1;
+        if (up) {
+            e0 = lo; e1 = hi;
+        } else if (down) {
+            e0 = hi; e1 = lo;
+        } else {
+            e0 = (i0 & 1u) ? hi : lo;
+            e1 = lt[i1];
+        }
         const h2 t0 = __builtin_bit_cast(h2, e0.x), p0 = __builtin_bit_cast(h2, e0.y);
         const h2 t1 = __builtin_bit_cast(h2, e1.x), p1 = __builtin_bit_cast(h2, e1.y);
         vt[k0][0] = (float)t0[0]; vt[k0][1] = (float)t0[1];

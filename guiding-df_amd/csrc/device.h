@@ -277,11 +277,11 @@ __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __
     grid_corners(g, level, px, py, pz, c);
     float v[8][F];
     if constexpr (F == 2) {
-        // Fewer distinct cache lines per lane: the x / x+1 corners usually sit
-        // in one aligned 8-byte pair (hash: (x^h, x^h^1) for even x; dense:
-        // idx, idx+1 for even idx), so load the pair holding corner x and only
-        // load corner x+1 separately when it lies elsewhere.  On levels whose
-        // dense stride for z wrapped to 0 the z+1 corners are the z corners.
+        // Fewer loads per lane: the x / x+1 corners usually sit in adjacent
+        // entries (hash: x^h and x^h^1 for even x; dense: idx and idx+1), so
+        // one 8-byte load serves both, and corner x+1 is loaded separately
+        // only when it lies elsewhere.  On levels whose dense stride for z
+        // wrapped to 0 the z+1 corners are the z corners.
         const uint32_t* lt32 = (const uint32_t*)lt;
         const bool zdup = !(g.hashed[level] & 1u) && g.s2[level] == 0u;
 #pragma unroll
@@ -292,11 +292,22 @@ __device__ __forceinline__ void grid_level(const GridSpec& g, const _Float16* __
                 v[k1][0] = v[k1 - 4][0]; v[k1][1] = v[k1 - 4][1];
                 continue;
             }
+            // adjacent entries (either order) in one 8-byte load at the lower
+            // index (4-byte aligned), else the aligned pair holding corner k0
+            // and a separate load of corner k1
             const uint32_t i0 = c.idx[k0], i1 = c.idx[k1];
-            const uint2 pr = *(const uint2*)(lt32 + (i0 & ~1u));
-            const uint32_t e0 = (i0 & 1u) ? pr.y : pr.x;
-            uint32_t e1 = (i0 & 1u) ? pr.x : pr.y;
-            if (i1 != (i0 ^ 1u)) e1 = lt32[i1];
+            const bool up = i1 == i0 + 1u, down = i1 + 1u == i0;
+            uint2 pr;
+            __builtin_memcpy(&pr, lt32 + (up ? i0 : (down ? i1 : (i0 & ~1u))), sizeof(pr));
+            uint32_t e0, e1;
+            if (up) {
+                e0 = pr.x; e1 = pr.y;
+            } else if (down) {
+                e0 = pr.y; e1 = pr.x;
+            } else {
+                e0 = (i0 & 1u) ? pr.y : pr.x;
+                e1 = lt32[i1];
+            }
             const h2 h0 = __builtin_bit_cast(h2, e0), h1 = __builtin_bit_cast(h2, e1);
             v[k0][0] = (float)h0[0]; v[k0][1] = (float)h0[1];
             v[k1][0] = (float)h1[0]; v[k1][1] = (float)h1[1];

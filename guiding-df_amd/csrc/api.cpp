@@ -1211,6 +1211,7 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     if (feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below 15");
     if (B > 0 && (!feat || !target_rgb)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
     if (B == 0) return DFG_OK;
+    const NrcScratch scope(ctx);  // its own step scratch: may run beside a DF step on another stream
     hipStream_t st = (hipStream_t)stream;
     const int64_t ld = round_up(B, 64);
     const int chunk = ctx->wgrad_chunk_rows;
@@ -1352,6 +1353,7 @@ int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float*
     if (feat_stride < 15) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below 15");
     if (B > 0 && (!feat || !target_rgb)) throw Error(DFG_ERR_INVALID_ARG, "null argument");
     if (B == 0) return DFG_OK;
+    const NrcScratch scope(ctx);  // its own step scratch: may run beside a DF step on another stream
     hipStream_t st = (hipStream_t)stream;
     HIPX(hipStreamSynchronize(st));
     const int64_t ld = round_up(B, 64);

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <utility>
 
 #include "../../include/dfg.h"
 #include "internal.h"
@@ -128,6 +129,22 @@ struct dfg_ctx {
     DevBuf<int> gb_n_items;
     DevBuf<unsigned long long> gb_part;  // fixed-point partial sums of multi-part slices
     DevBuf<int4> gb_multi;               // the multi-part slices
+    // The same step scratch (loss, its mean, compact positions, the grid
+    // backward's buckets) for the radiance-cache training calls, swapped in
+    // for their duration (NrcScratch), so that a DF step and an NRC step
+    // enqueued on two streams share no device memory.
+    DevBuf<float> n_loss, n_pos, n_mean_ws, n_gb_val;
+    DevBuf<int64_t> n_gb_offs, n_gb_cursor;
+    DevBuf<uint32_t> n_gb_idx, n_gb_cnt;
+    DevBuf<int4> n_gb_items, n_gb_multi;
+    DevBuf<int> n_gb_n_items;
+    DevBuf<unsigned long long> n_gb_part;
+    void swap_nrc_scratch() {
+        std::swap(loss, n_loss); std::swap(pos, n_pos); std::swap(mean_ws, n_mean_ws);
+        std::swap(gb_val, n_gb_val); std::swap(gb_offs, n_gb_offs); std::swap(gb_cursor, n_gb_cursor);
+        std::swap(gb_idx, n_gb_idx); std::swap(gb_cnt, n_gb_cnt); std::swap(gb_items, n_gb_items);
+        std::swap(gb_multi, n_gb_multi); std::swap(gb_n_items, n_gb_n_items); std::swap(gb_part, n_gb_part);
+    }
     ~dfg_ctx() {
         for (auto e : pipe_ev) (void)hipEventDestroy(e);
         if (pipe_start) (void)hipEventDestroy(pipe_start);
@@ -141,7 +158,20 @@ struct dfg_ctx {
         pos.release(); mean_ws.release();
         rec_feat.release(); rec_tgt.release(); rec_loss.release(); rec_local.release(); rec_blk.release();
         rec_count.release(); rec_vidx.release(); img_rows.release(); img_wi2.release();
+        n_loss.release(); n_pos.release(); n_mean_ws.release(); n_gb_val.release(); n_gb_offs.release();
+        n_gb_cursor.release(); n_gb_idx.release(); n_gb_cnt.release(); n_gb_items.release(); n_gb_multi.release();
+        n_gb_n_items.release(); n_gb_part.release();
     }
+};
+
+// Scope of a radiance-cache training call: its step scratch in place of the
+// DF step's (host-side pointer swap; the calls are made from one host thread).
+struct NrcScratch {
+    dfg_ctx* c;
+    explicit NrcScratch(dfg_ctx* ctx) : c(ctx) { c->swap_nrc_scratch(); }
+    ~NrcScratch() { c->swap_nrc_scratch(); }
+    NrcScratch(const NrcScratch&) = delete;
+    NrcScratch& operator=(const NrcScratch&) = delete;
 };
 
 namespace dfg {

@@ -148,7 +148,11 @@ int dfg_nrc_infer(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int 
                   float* rgb);
 /* NRC training step with RelativeL2Luminance (nrc.cpp:114-170):
  * feat [B][15], target_rgb [B][3]; loss_out (device, may be NULL) receives the
- * summed loss divided by B, as the reference reports it. */
+ * summed loss divided by B, as the reference reports it.  It has step scratch
+ * of its own and touches only the NRC's parameters, so it may be enqueued on
+ * another stream beside dfg_train_step (both called from one host thread; the
+ * caller orders each stream after the inputs it reads).  The same holds for
+ * the two plain steps below. */
 int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int feat_stride,
                        const float* target_rgb, float* loss_out);
 

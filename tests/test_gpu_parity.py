@@ -507,6 +507,50 @@ def test_train_steps_are_deterministic(torch_cuda):
         f.close()
 
 
+@pytest.mark.parametrize("plain", [False, True])
+def test_df_and_nrc_steps_on_two_streams_match_one_stream(torch_cuda, plain):
+    """The NRC training step has step scratch of its own, so a DF step and an
+    NRC step enqueued on two streams (fork/join every step) leave weights,
+    gradients and losses bit-identical to the same steps on one stream.  The
+    plain DF step scatters its grid gradient with fp32 atomics (order-
+    dependent), so for the plain pair only the NRC net is compared bit for bit
+    and the DF losses are checked finite."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=16)
+    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg)
+    B = 20000
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    la = torch.zeros((3, 2), dtype=torch.float32, device="cuda")
+    lb = torch.zeros((3, 2), dtype=torch.float32, device="cuda")
+    for step in range(3):
+        feat = dev(torch, df_features(B, 110 + step, nrc=True))
+        tgt = dev(torch, train_targets(B, 120 + step))
+        rgb = dev(torch, np.exp(np.random.default_rng(130 + step).normal(0, 2, (B, 3))).astype(np.float32))
+        df_step = fa.train_step_plain if plain else fa.train_step
+        nrc_step = fa.nrc_train_step_plain if plain else fa.nrc_train_step
+        df_step(feat, tgt, la[step, 0:1], stream=main)
+        nrc_step(feat, rgb, la[step, 1:2], stream=main)
+        df_step = fb.train_step_plain if plain else fb.train_step
+        nrc_step = fb.nrc_train_step_plain if plain else fb.nrc_train_step
+        side.wait_stream(main)
+        df_step(feat, tgt, lb[step, 0:1], stream=main)
+        nrc_step(feat, rgb, lb[step, 1:2], stream=side)
+        main.wait_stream(side)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(la).all()) and bool(torch.isfinite(lb).all())
+    assert torch.equal(la[:, 1], lb[:, 1]), "NRC losses differ"
+    if not plain:
+        assert torch.equal(la[:, 0], lb[:, 0]), "DF losses differ"
+    for k in ((2,) if plain else (0, 1, 2)):
+        assert np.array_equal(fa.get_grad(k), fb.get_grad(k)), "net %d gradients differ" % k
+        for which in (0, 1, 2):
+            assert np.array_equal(fa.get_params(k, which), fb.get_params(k, which)), \
+                "net %d params (%d) differ" % (k, which)
+    fa.close()
+    fb.close()
+
+
 def test_wgrad_chunk_rows_only_reorders_sums(torch_cuda):
     """Another weight-gradient chunk size changes only the order of the fp32
     sums: the matrix gradients agree to 1e-5 (relative norm) and the grid

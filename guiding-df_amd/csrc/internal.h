@@ -239,7 +239,7 @@ struct GridBucketArgs {
                                   // the parts of slices cut into several parts
     // second net of the paired backward (same grid structure): its dL/dy,
     // gradient, first grid parameter and level maxima; the records are then
-    // rec_idx (entry in slice) + rec_val as float4 {g0, g1, g0', g1'}
+    // rec_idx as 16-bit entries in the slice + rec_val as float4 {g0, g1, g0', g1'}
     const float* dx2;
     float* grad2;
     int64_t param_begin2;

@@ -712,7 +712,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 //
 // PAIR: the theta and phi grids of identical structure in one pass (same
 // positions, so the same corners, slices and record positions): one record
-// per corner, the entry in rec_idx and {w dL/dy[0], w dL/dy[1], w dL/dy2[0],
+// per corner, the 16-bit entry in rec_idx and {w dL/dy[0], w dL/dy[1], w dL/dy2[0],
 // w dL/dy2[1]} as a float4 in rec_val, and each net's level maxima.
 constexpr int kStageRows = 256;
 template <bool PAIR>
@@ -823,7 +823,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
             const int64_t pos = (int64_t)next[r.w] + (int64_t)(i - (int)loff[r.w]);
             if constexpr (PAIR) {
                 const uint2 q = stage2[i];
-                a.rec_idx[pos] = r.x;
+                ((uint16_t*)a.rec_idx)[pos] = (uint16_t)r.x;  // entry in a slice of kGridSliceFloats / 4
                 ((uint4*)a.rec_val)[pos] = make_uint4(r.y, r.z, q.x, q.y);
             } else {
                 ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
@@ -950,13 +950,15 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_kernel(const GridBucketA
     }
 }
 
-// Paired theta/phi reduce: one block per (slice, part) of the 20-byte records
-// (rec_idx entry, rec_val float4); LDS sums [entry][4] (theta f0, f1, phi f0,
+// Paired theta/phi reduce: one block per (slice, part) of the 18-byte records
+// (16-bit entry in rec_idx, rec_val float4); LDS sums [entry][4] (theta f0, f1, phi f0,
 // f1), each net at its own fixed-point scale; writes both nets' slices (single
 // part) or the part's partial sums.
 __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBucketArgs a) {
     extern __shared__ unsigned long long acc[];  // [4][NE] (value-major: no bank conflicts)
     constexpr int NE = kGridSliceFloats / 4;
+    static_assert(NE <= 65536, "paired record entries are 16-bit");
+    const uint16_t* rec_e = (const uint16_t*)a.rec_idx;
     if ((int)blockIdx.x >= *a.n_items) return;
     const int4 item = a.items[blockIdx.x];
     const GridSpec& gs = a.ns->grid;
@@ -983,7 +985,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
         for (int j = 0; j < U; ++j) {
             const int64_t p = p0 + (int64_t)j * blockDim.x;
             const bool in = p < pe;
-            e[j] = in ? a.rec_idx[p] : 0u;
+            e[j] = in ? (uint32_t)rec_e[p] : 0u;
             v[j] = in ? r4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
@@ -1100,7 +1102,7 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
 }
 
 // Theta and phi grids of identical structure (F = 2, not nearest) in one
-// bucketed pass: count, scan, the paired staged scatter (one 20-byte record
+// bucketed pass: count, scan, the paired staged scatter (one 18-byte record
 // per corner for both nets), then one reduce and combine writing both
 // gradients.
 void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,

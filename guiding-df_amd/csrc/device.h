@@ -149,13 +149,28 @@ __device__ __forceinline__ uint32_t grid_index3(int type, uint32_t hsize, uint32
     return index % hsize;
 }
 
-// pos_fract, tcnn common_device.h:418-431 (identity or smoothstep)
+// pos_fract, tcnn common_device.h:418-431 (identity or smoothstep), with the
+// position p = in * scale + 0.5 taken exactly instead of rounded to fp32.
+// Rounded, p carries up to half an ulp of error (2^-6 at scale 2^19), which
+// is a percent of the fine levels' interpolation weights.  Here:
+//   t = floor(fma(in, scale, 0.5))   (the fma rounds once; round-to-nearest
+//                                     never lands below an integer <= p, so
+//                                     t is floor(p) or, when p lies within
+//                                     half an ulp below t, floor(p) + 1)
+//   f = fma(in, scale, 0.5 - t)      (0.5 - t is exact below 2^24, so f is
+//                                     p - t rounded once; f < 0 flags the
+//                                     second case, which takes t - 1)
+// so the cell is exactly floor(p) and the fraction p - cell within 2^-25,
+// for |in * scale| < 2^23.
 __device__ __forceinline__ void pos_fract(float in, float scale, int interp, float& frac, uint32_t& cell) {
 #pragma clang fp contract(off)
-    float p = in * scale + 0.5f;
-    const float t = floorf(p);
+    float t = floorf(__builtin_fmaf(in, scale, 0.5f));
+    float p = __builtin_fmaf(in, scale, 0.5f - t);
+    if (p < 0.0f) {
+        t = t - 1.0f;
+        p = __builtin_fmaf(in, scale, 0.5f - t);
+    }
     cell = (uint32_t)(int)t;
-    p = p - t;
     if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0f - 2.0f * p);
     frac = p;
 }
@@ -694,15 +709,26 @@ __device__ __forceinline__ float sample_linear(const Segment& sg, float u, float
         const float w = cd > DFG_EPS ? vd / cd : u_extra;
         x = (v + w * ecm) * invN;
     } else {
+        // the branch is decided on the reference's global-coordinate
+        // discriminant (train.h:494-497) ...
         const float C = -(m * 0.5f) * (v * v * invN * invN) - (B * v * invN) + sg.c_lo - u;
         const float delta = B * B - 4.0f * A * C;
         if (delta < DFG_EPS) {
             x = (v + u_extra * ecm) * invN;
         } else {
-            const float sd = __builtin_sqrtf(delta);
-            const float s0 = (-B + sd) / (2.0f * A);
-            const float s1 = (-B - sd) / (2.0f * A);
-            x = A >= 0.0f ? fmaxf(s0, s1) : fminf(s0, s1);
+            // ... and the root taken in the segment's own coordinate t = x - x0,
+            // x0 = v / N the left bin centre.  There the cdf is
+            // c_lo + p0 t + A t^2 with p0 = P(vim1) (= m x0 + B), the
+            // discriminant is p0^2 + 4 A d (= delta) with d = u - c_lo, and the
+            // root the reference selects (the larger one for A >= 0, the
+            // smaller for A < 0) is, for either sign, 2 d / (p0 + sqrt(p0^2 +
+            // 4 A d)): free of the cancellation of the global form, whose
+            // terms reach A x0^2 ~ N while the root moves x by ~1/N.
+            const float d = u - sg.c_lo;
+            const float disc = fmaxf(pb * pb + 4.0f * A * d, 0.0f);
+            const float den = pb + __builtin_sqrtf(disc);
+            const float t = den > 0.0f ? (2.0f * d) / den : 0.0f;
+            x = v * invN + t;
         }
     }
     return fminf(x, 1.0f - DFG_EPS);

@@ -10,7 +10,9 @@
 //   *_f32 : the reference's fp32 operation order, with the deterministic
 //           exp/log restated below, compiled with -ffp-contract=off — the
 //           "reference-order" mode used for bit-exact bin/segment checks and
-//           as the CPU baseline (OpenMP over rows).
+//           as the CPU baseline (OpenMP over rows).  One deliberate departure,
+//           shared with the library: grid positions x * scale + 0.5 are
+//           evaluated exactly (two fmaf), not rounded to fp32 (pos_fract).
 //
 // Parity pins (see tests/test_oracle.py): closed forms, invariants, finite
 // differences, brute force on tiny grids, numpy matmul chains; the reference
@@ -183,13 +185,37 @@ uint32_t grid_index3(int type, uint32_t hsize, uint32_t res, uint32_t x, uint32_
     return index % hsize;
 }
 
+// common_device.h:418-431.  The position p = in * scale + 0.5 is taken
+// exactly in both modes (the reference rounds it to fp32, which costs up to
+// half an ulp, 2^-6 at scale 2^19, of the fine levels' weights):
+//   double: the product of two floats and the + 0.5 are exact in double
+//           wherever p is near an integer, so floor and p - floor are exact;
+//   float:  t = floor(fmaf(in, scale, 0.5)), f = fmaf(in, scale, 0.5 - t), and
+//           t - 1 when f < 0 (the first fmaf rounded up onto an integer): the
+//           cell is floor(p) and f is p - cell rounded once.
 template <typename R>
-void pos_fract(R in, R scale, int interp, R& frac, uint32_t& cell) {  // common_device.h:418-431
-    R p = in * scale + (R)0.5;
-    const R t = std::floor(p);
+void pos_fract(R in, R scale, int interp, R& frac, uint32_t& cell);
+
+template <>
+void pos_fract<double>(double in, double scale, int interp, double& frac, uint32_t& cell) {
+    double p = in * scale + 0.5;
+    const double t = std::floor(p);
     cell = (uint32_t)(int)t;
     p = p - t;
-    if (interp == INTERP_SMOOTHSTEP) p = p * p * ((R)3 - (R)2 * p);
+    if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0 - 2.0 * p);
+    frac = p;
+}
+
+template <>
+void pos_fract<float>(float in, float scale, int interp, float& frac, uint32_t& cell) {
+    float t = std::floor(fmaf(in, scale, 0.5f));
+    float p = fmaf(in, scale, 0.5f - t);
+    if (p < 0.0f) {
+        t = t - 1.0f;
+        p = fmaf(in, scale, 0.5f - t);
+    }
+    cell = (uint32_t)(int)t;
+    if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0f - 2.0f * p);
     frac = p;
 }
 
@@ -422,14 +448,22 @@ R sample_linear(const R* p, int N, const Seg<R>& sg, R u, R u_extra) {  // train
         const R w = cd > eps ? vd / cd : u_extra;
         x = (v + w * ecm) * invN;
     } else {
+        // branch on the reference's global discriminant (train.h:494-497);
+        // root in the segment's coordinate t = x - v/N, where the cdf is
+        // lo + p0 t + A t^2 (p0 = p[vim1]) and the selected root is
+        // 2 d / (p0 + sqrt(p0^2 + 4 A d)), d = u - lo, for either sign of A
+        // (the same root; no cancellation of terms of size A (v/N)^2)
         const R C = -(m * (R)0.5) * (v * v * invN * invN) - (B * v * invN) + sg.lo - u;
         const R delta = B * B - (R)4 * A * C;
         if (delta < eps) {
             x = (v + u_extra * ecm) * invN;
         } else {
-            const R sd = std::sqrt(delta);
-            const R s0 = (-B + sd) / ((R)2 * A), s1 = (-B - sd) / ((R)2 * A);
-            x = A >= 0 ? std::fmax(s0, s1) : std::fmin(s0, s1);
+            const R p0 = p[vim1];
+            const R d = u - sg.lo;
+            const R disc = std::fmax(p0 * p0 + (R)4 * A * d, (R)0);
+            const R den = p0 + std::sqrt(disc);
+            const R t = den > 0 ? ((R)2 * d) / den : (R)0;
+            x = v * invN + t;
         }
     }
     return std::fmin(x, (R)1 - eps);

@@ -94,3 +94,22 @@ def random_records(n_pixels, max_depth=5, seed=0, aabb=(-2.0, -1.0, 0.5, 3.0, 2.
     rec["thp"][rng.random((D, N, 3)) < 0.03] = 1e-6
     rec["L"][rng.random((D, N, 3)) < 0.01] = np.nan
     return rec, np.array(aabb, np.float32)
+
+
+def trained_scale_params(L, rng, gain=2.0, grid_amp=0.5):
+    """fp16-representable parameters of one net at the scale training reaches
+    (theta/phi logits of sigma ~1.5, grid entries far above the U[-1e-4, 1e-4]
+    initialisation): Xavier-uniform matrices times `gain`, grid entries
+    U[-grid_amp, grid_amp].  L is an oracle layout."""
+    p = np.zeros(L["n_params"], np.float32)
+    offs = list(L["layer_off"]) + [L["n_matrix"]]
+    for i in range(len(offs) - 1):
+        a, b = offs[i], offs[i + 1]
+        fan_in = L["enc_width"] if i == 0 else 64
+        fan_out = (b - a) // fan_in
+        lim = gain * math.sqrt(6.0 / (fan_in + fan_out))
+        p[a:b] = rng.uniform(-lim, lim, b - a)
+    if L["grid"]:
+        g0 = L["grid"]["param_begin"]
+        p[g0:] = rng.uniform(-grid_amp, grid_amp, L["n_params"] - g0)
+    return p.astype(np.float16).astype(np.float32)

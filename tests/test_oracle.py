@@ -8,6 +8,7 @@ import pytest
 
 import oracle as O
 import dfguide as G
+from gpu_util import df_features, uniforms, wi_init, trained_scale_params
 
 
 # ----------------------------------------------------------------------------
@@ -120,6 +121,74 @@ def test_grid_index_uint32_wrap_levels():
     assert lib.orc_grid_index3(1, T, res, x, y, z) == (x ^ ((y * 2654435761) & 0xffffffff) ^ ((z * 805459861) & 0xffffffff)) % T
 
 
+def _near_vertex_positions(scales, n_per_level, rng):
+    """float32 inputs x within a few ulps of (k - 0.5) / scale for random
+    integers k, i.e. positions x * scale + 0.5 within an ulp or two of a cell
+    boundary, on every level's scale."""
+    out = []
+    for s in scales:
+        k = rng.integers(1, int(s), n_per_level)
+        x = ((k - 0.5) / s).astype(np.float32)
+        for step in range(-3, 4):
+            y = x.copy()
+            for _ in range(abs(step)):
+                y = np.nextafter(y, np.float32(np.inf if step > 0 else -np.inf))
+            out.append(y)
+    return np.concatenate(out)
+
+
+def test_grid_positions_are_exact():
+    """The cell of every level is floor(x * scale + 0.5) of the exact real
+    product, also within an ulp of a boundary, where the reference's fp32
+    rounding of the position lands in the neighbouring cell: the FP32 mode's
+    corner indices equal those composed in float64 (exact for float inputs)
+    through the brute-force-pinned index function."""
+    cfg = G.hash_config(with_nrc=False)
+    th, _, _ = O.config_layouts(cfg)
+    g = th["grid"]
+    rng = np.random.default_rng(11)
+    scales = [float(s) for s in g["scale"]]
+    xs = _near_vertex_positions(scales, 40, rng)
+    pos = np.stack([xs, rng.permutation(xs), rng.permutation(xs)], 1).astype(np.float32)
+    feat = np.zeros((pos.shape[0], 9), np.float32)
+    feat[:, :3] = pos
+    got = O.grid_index(th, feat)
+    lib = O.lib()
+    rounded_differs = 0
+    for l in range(g["levels"]):
+        s = np.float64(np.float32(scales[l]))
+        exact = np.floor(pos.astype(np.float64) * s + 0.5).astype(np.int64)
+        rounded = np.floor(pos * np.float32(s) + np.float32(0.5)).astype(np.int64)
+        rounded_differs += int(np.any(exact != rounded, axis=1).sum())
+        hsize = g["offset"][l + 1] - g["offset"][l]
+        for i in range(0, pos.shape[0], 7):
+            for k in range(8):
+                c = [int(exact[i, d] + ((k >> d) & 1)) & 0xffffffff for d in range(3)]
+                assert got[i, l, k] == lib.orc_grid_index3(1, hsize, g["res"][l], *c), (i, l, k)
+    # the probe set really contains positions that fp32 rounding puts in another cell
+    assert rounded_differs > 100
+
+
+def test_fp32_query_tracks_fp64_on_trained_scale_weights():
+    """With exact grid positions the FP32 restatement stays within 1e-5
+    (median) of FP64 on weights of trained scale (logits of sigma ~1.5,
+    grid entries U[-0.5, 0.5]), where fp32-rounded positions give ~1e-3."""
+    cfg = G.hash_config(with_nrc=False)
+    th, ph, _ = O.config_layouts(cfg)
+    rng = np.random.default_rng(5)
+    pt, pp = trained_scale_params(th, rng), trained_scale_params(ph, rng)
+    n = 2048
+    feat, u = df_features(n, 1), uniforms(n, 2)
+    o32, _ = O.query_f32_omp(th, ph, pt, pp, feat, 9, u, wi_init(n), 1)
+    at = wi_init(n)
+    at[:, :2] = o32[:, :2]
+    o64 = O.query_f64(th, ph, pt, pp, feat, 9, u, at, 1, guided=np.zeros(n, np.uint8))
+    ref = o64[:, 2].astype(np.float64) * o64[:, 3]
+    rel = np.abs(o32[:, 2].astype(np.float64) * o32[:, 3] - ref) / ref
+    assert np.median(rel) <= 1e-5, np.median(rel)
+    assert np.percentile(rel, 99) <= 1e-3, np.percentile(rel, 99)
+
+
 def _grid_cfg(interp="Linear", gtype="HashGrid", levels=4, F=2, log2T=10):
     return {"otype": "Composite", "nested": [
         {"otype": gtype, "n_levels": levels, "n_features_per_level": F, "log2_hashmap_size": log2T,
@@ -143,13 +212,21 @@ def test_grid_interpolation_properties(interp):
     ea, eb, eab = O.encode_f64(L, a, x), O.encode_f64(L, b, x), O.encode_f64(L, 2 * a - 3 * b, x)
     assert np.allclose(eab[:, :8], 2 * ea[:, :8] - 3 * eb[:, :8], atol=1e-10)
     # at a vertex (frac == 0) the value is the table entry of that vertex
+    # (float inputs whose exact position x * scale + 0.5 lies at the vertex or
+    # at most a few float ulps above it, so that the vertex is corner 0)
     lvl0_scale = float(g["scale"][0])
-    cell = np.array([[1, 2, 0]])
-    xv = (cell - 0.5) / lvl0_scale
-    idx = O.grid_index(L, xv.astype(np.float32))[0, 0, 0]
-    ev = O.encode_f64(L, params, xv)
+    cell = np.array([1, 2, 0])
+    xv = ((cell - 0.5) / lvl0_scale).astype(np.float32)
+    for d in range(3):
+        while float(xv[d]) * lvl0_scale + 0.5 < cell[d]:
+            xv[d] = np.nextafter(xv[d], np.float32(np.inf))
+    frac = xv.astype(np.float64) * lvl0_scale + 0.5 - cell
+    assert np.all((frac >= 0) & (frac < 1e-5))
+    xv = xv[None, :]
+    idx = O.grid_index(L, xv)[0, 0, 0]
+    ev = O.encode_f64(L, params, xv.astype(np.float64))
     t0 = params[g["param_begin"]:]
-    assert np.allclose(ev[0, :2], t0[idx * 2: idx * 2 + 2], atol=1e-9)
+    assert np.allclose(ev[0, :2], t0[idx * 2: idx * 2 + 2], atol=1e-4)
 
 
 def test_smoothstep_closed_form():
@@ -317,6 +394,66 @@ def test_fp32_head_tracks_fp64():
     x32, p32, _, s32 = O.head(z.astype(np.float32), 1, u.astype(np.float32), ue.astype(np.float32), fp32=True)
     assert np.mean(s64 == s32) > 0.999
     assert np.median(np.abs(x32 - x64)) < 1e-6 and np.median(np.abs(p32 / p64 - 1)) < 1e-5
+
+
+def _global_form_root(bins, N, seg, u):
+    """The reference's quadratic in global coordinates (train.h:453-506) in
+    float64: A x^2 + B x + C = 0 with the root selected by the sign of A; NaN
+    where the reference takes its linear or extra-uniform fallback."""
+    i = seg
+    vim1 = i - 1
+    v = vim1 + 0.5
+    if vim1 < 0 or i > N - 1:
+        return np.nan
+    m = (bins[i] - bins[vim1]) * N
+    B = bins[vim1] * (1 + v) - bins[i] * v
+    A = m * 0.5
+    if abs(A) < 1e-5:
+        return np.nan
+    lo = bins[0] * 0.5 / N + sum((bins[k - 1] + bins[k]) / (2 * N) for k in range(1, i))
+    C = -A * (v / N) ** 2 - B * v / N + lo - u
+    delta = B * B - 4 * A * C
+    if delta < 1e-5:
+        return np.nan
+    s0, s1 = (-B + math.sqrt(delta)) / (2 * A), (-B - math.sqrt(delta)) / (2 * A)
+    return max(s0, s1) if A >= 0 else min(s0, s1)
+
+
+@pytest.mark.parametrize("N", [16, 32, 64])
+def test_sampler_root_equals_the_reference_global_form(N):
+    """The sampler takes the root in the segment's own coordinate; in float64
+    that is the reference's global-coordinate root to rounding, on peaked
+    bins where both roots of the quadratic lie near the segment."""
+    rng = np.random.default_rng(30 + N)
+    n = 3000
+    z = rng.normal(0, 2.0, (n, N))
+    u = rng.random(n)
+    x, _, bins, seg = O.head(z, 1, u, rng.random(n))
+    checked = 0
+    for i in range(n):
+        r = _global_form_root(bins[i], N, seg[i], u[i])
+        if np.isnan(r):
+            continue
+        checked += 1
+        assert abs(x[i] - min(r, 1 - 1e-5)) < 1e-10, (i, x[i], r)
+    assert checked > n // 2
+
+
+def test_fp32_sample_tracks_fp64_on_peaked_bins():
+    """With the local-coordinate root the FP32 sample stays within an ulp or
+    so of FP64 (median) on the logit spread of trained nets (sigma 2, 64
+    bins); the global-coordinate form loses 25x more."""
+    rng = np.random.default_rng(17)
+    n, N = 5000, 64
+    z = rng.normal(0, 2.0, (n, N))
+    u, ue = rng.random(n), rng.random(n)
+    x64, _, _, s64 = O.head(z, 1, u, ue)
+    x32, _, _, s32 = O.head(z.astype(np.float32), 1, u.astype(np.float32), ue.astype(np.float32), fp32=True)
+    same = s64 == s32
+    assert np.mean(same) > 0.999
+    dx = np.abs(x32 - x64)[same]
+    # global form: median 3.6e-7, p99 7.7e-6; local form: 1.5e-8, 8.7e-7
+    assert np.median(dx) < 5e-8 and np.percentile(dx, 99) < 3e-6, (np.median(dx), np.percentile(dx, 99))
 
 
 def test_theta_input():

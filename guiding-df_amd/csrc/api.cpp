@@ -400,6 +400,8 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
             HIPX(hipGetDevice(&dev));
             HIPX(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             if (cus > 0) c->n_cus = cus;
+            c->device = dev;
+            c->noalloc_scratch = noalloc_scratch_setup(dev);
         }
         const std::string gbw = lower(d.string_or("grid_backward", "bucketed"));
         if (gbw != "bucketed" && gbw != "atomic") throw Error(DFG_ERR_CONFIG, "grid_backward must be bucketed or atomic");
@@ -627,37 +629,51 @@ void dfg::refresh_pair_table(dfg_ctx* c, hipStream_t st) {
     launch_interleave_tables(c->net[0].tab_train(), c->net[1].tab_train(), c->tab_pair_train.p, entries, st);
 }
 
-void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
-    if (n < c->prepass_min_rows) return;
+dfg::PrepassSizes dfg::prepass_sizes(const dfg_ctx* c, int64_t n) {
+    PrepassSizes z;
+    z.ld = round_up(n, 64);
+    z.pos = (size_t)n * 3;
+    for (int i = 0; i < 2; ++i) {
+        const NetSpec& ns = c->net[i].spec;
+        z.gfeat[i] = ns.grid.present ? (size_t)ns.grid.n_levels * (c->pair_ok ? 2 : ns.grid.n_feat) * z.ld : 0;
+    }
+    return z;
+}
+
+void dfg::grid_prepass_into(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st,
+                            float* pos_buf, float* gfeat_th, float* gfeat_ph) {
     const int64_t ld = round_up(n, 64);
+    float* gf[2] = {gfeat_th, gfeat_ph};
     if (c->pair_ok) {
-        const size_t cols = (size_t)c->net[0].spec.grid.n_levels * 2;
-        c->net[0].gfeat.ensure(cols * ld);
-        c->net[1].gfeat.ensure(cols * ld);
-        const float* pos = compact_pos(c, feat, stride, grid_pos_dim(c->net[0].spec), n, st);
-        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, pos, n, ld, c->net[0].gfeat.p,
-                                c->net[1].gfeat.p, st);
-        a.gfeat_th = c->net[0].gfeat.p;
-        a.gfeat_ph = c->net[1].gfeat.p;
+        launch_extract_pos(feat, stride, grid_pos_dim(c->net[0].spec), n, pos_buf, st);
+        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, pos_buf, n, ld, gf[0], gf[1], st);
+        a.gfeat_th = gf[0];
+        a.gfeat_ph = gf[1];
         a.gld = ld;
         return;
     }
-    const float* pos = nullptr;
     int pos_dim = -1;
     for (int i = 0; i < 2; ++i) {
         NetState& s = c->net[i];
         if (!s.spec.grid.present) continue;
         const int pd = grid_pos_dim(s.spec);
-        if (pd != pos_dim) {
-            pos = compact_pos(c, feat, stride, pd, n, st);
+        if (pd != pos_dim) {  // stream order: the previous net's pre-pass has read the buffer
+            launch_extract_pos(feat, stride, pd, n, pos_buf, st);
             pos_dim = pd;
         }
-        const size_t cols = (size_t)s.spec.grid.n_levels * s.spec.grid.n_feat;
-        s.gfeat.ensure(cols * ld);
-        launch_grid_encode(s.d_spec.p, s.spec, s.tab_infer(), pos, n, ld, s.gfeat.p, st);
-        (i == 0 ? a.gfeat_th : a.gfeat_ph) = s.gfeat.p;
+        launch_grid_encode(s.d_spec.p, s.spec, s.tab_infer(), pos_buf, n, ld, gf[i], st);
+        (i == 0 ? a.gfeat_th : a.gfeat_ph) = gf[i];
     }
     a.gld = ld;
+}
+
+void dfg::grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st) {
+    if (n < c->prepass_min_rows) return;
+    const PrepassSizes z = prepass_sizes(c, n);
+    c->pos.ensure(z.pos);
+    for (int i = 0; i < 2; ++i)
+        if (z.gfeat[i]) c->net[i].gfeat.ensure(z.gfeat[i]);
+    grid_prepass_into(c, a, n, feat, stride, st, c->pos.p, c->net[0].gfeat.p, c->net[1].gfeat.p);
 }
 
 // Chunked query of a large batch: the paired grid pre-pass of every chunk is

@@ -103,6 +103,8 @@ struct dfg_ctx {
     int64_t pipeline_min_rows = 262144;
     int wgrad_chunk_rows = 1024;       // batch rows per weight-gradient partial sum (multiple of 64)
     int n_cus = 256;
+    int device = 0;                    // the device current at dfg_create (weights and workspace live there)
+    void* noalloc_scratch = nullptr;   // that device's address of the allocation-free query's scratch array
     hipStream_t side = nullptr;
     hipEvent_t pipe_start = nullptr;
     std::vector<hipEvent_t> pipe_ev;
@@ -179,6 +181,20 @@ int fail(dfg_ctx* c, int code, const std::string& msg);
 QueryArgs base_args(dfg_ctx* c);
 void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df);
 void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st);
+// Element counts (floats) of the grid pre-pass buffers of an n-row query:
+// compact positions [n][3] and each DF net's features [columns][ld].
+struct PrepassSizes {
+    int64_t ld = 0;
+    size_t pos = 0;
+    size_t gfeat[2] = {0, 0};
+};
+PrepassSizes prepass_sizes(const dfg_ctx* c, int64_t n);
+// The grid pre-pass into caller-provided buffers of prepass_sizes(c, n).
+void grid_prepass_into(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st,
+                       float* pos_buf, float* gfeat_th, float* gfeat_ph);
+// Resolves, once per device, the device address of the allocation-free
+// query's file-scope scratch array on `device` (which must be current).
+void* noalloc_scratch_setup(int device);
 void check_rows(const dfg_ctx* c, int64_t n, const float* feat, int stride);
 void refresh_pair_table(dfg_ctx* c, hipStream_t st);
 }  // namespace dfg

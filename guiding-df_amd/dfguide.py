@@ -33,7 +33,7 @@ STATUS = {
 # Every symbol include/dfg.h declares.
 EXPORTS = [
     "dfg_create", "dfg_destroy", "dfg_last_error", "dfg_get_info", "dfg_get_params", "dfg_get_grad",
-    "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_plain", "dfg_pdf",
+    "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_noalloc", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
     "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
     "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_pdf_image", "dfg_directions", "dfg_scatter", "dfg_prepare_features",
@@ -130,6 +130,7 @@ def load_library(path=LIB_PATH):
         "dfg_save": ([P, ctypes.c_char_p], I),
         "dfg_load": ([P, ctypes.c_char_p], I),
         "dfg_query": ([P, P, I64, P, I, P, P, P, P], I),
+        "dfg_query_noalloc": ([P, P, I64, P, I, P, P, P, P], I),
         "dfg_query_plain": ([P, P, I64, P, I, P, P, P, P], I),
         "dfg_pdf": ([P, P, I64, P, I, P, P], I),
         "dfg_train_step": ([P, P, I64, P, I, P, P], I),
@@ -351,6 +352,13 @@ class GuidingField:
         n = feat.shape[0]
         self._c(self.lib.dfg_query(self.ctx, _stream(stream), n, _ptr(feat), feat.shape[1], _ptr(u),
                                    _ptr(guided), _ptr(wi), _ptr(nrc_rgb)))
+
+    def query_noalloc(self, feat, u, wi, guided=None, nrc_rgb=None, stream=None):
+        """dfg_query without device allocation (rows limited by the library's
+        fixed scratch array)."""
+        n = feat.shape[0]
+        self._c(self.lib.dfg_query_noalloc(self.ctx, _stream(stream), n, _ptr(feat), feat.shape[1], _ptr(u),
+                                           _ptr(guided), _ptr(wi), _ptr(nrc_rgb)))
 
     def query_plain(self, feat, u, wi, guided=None, nrc_rgb=None, stream=None):
         n = feat.shape[0]

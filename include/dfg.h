@@ -130,6 +130,21 @@ int dfg_query(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat
 int dfg_query_plain(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
                     const float* u, const uint8_t* guided, float* wi, float* nrc_rgb);
 
+/* dfg_query without any device allocation during the call.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * contract and bit-identical results as dfg_query, except:
+ *   - the grid pre-pass buffers live in one 16.75 MiB device array of the
+ *     library (per GPU, held until the process ends), so n is limited to what
+ *     fits: 65,536 rows for two 16-level F = 2 grids (or two 8-level F = 4
+ *     grids); larger n returns DFG_ERR_BAD_SIZE with the maximum for the
+ *     context's configuration;
+ *   - the array is shared by every call on a GPU: calls that overlap on one
+ *     GPU (two streams, contexts or threads) race; order them on one stream;
+ *   - no chunked side-stream pipelining (query_chunks is ignored). */
+int dfg_query_noalloc(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,
+                      const float* u, const uint8_t* guided, float* wi, float* nrc_rgb);
+
 /* Solid-angle pdf of given directions (AutoRegressive::pdf, AR.h:41-43):
  * pdf[i] = p_theta(wi2[i][0]) * p_phi(wi2[i][1] | theta) / (4 pi). */
 int dfg_pdf(dfg_ctx* ctx, void* stream, int64_t n, const float* feat, int feat_stride,

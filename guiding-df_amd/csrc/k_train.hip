@@ -516,12 +516,34 @@ void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s) {
 // grid gradient is deterministic.  A level whose maximum is not finite gets
 // NaN gradients.
 // ---------------------------------------------------------------------------
+// rint(v * 2^S) as an integer, from v's bits: v = +-m 2^(E - 150) (m with
+// its implicit bit; subnormals as E = 1 without it), so v 2^S = +-m 2^sh with
+// sh = E - 150 + S.  sh >= 0 is a left shift (the scale bounds the result
+// below 2^62); sh < 0 a right shift rounded half to even (0 for sh <= -25,
+// where m 2^sh < 1/2).
 __device__ __forceinline__ long long fix64(float v, int S) {
-    return (long long)__builtin_rint(__builtin_ldexp((double)v, S));
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t E = (b >> 23) & 0xffu;
+    const uint64_t m = (b & 0x7fffffu) | (E ? 0x800000u : 0u);
+    const int sh = (int)(E ? E : 1u) - 150 + S;
+    uint64_t q;
+    if (sh >= 0) {
+        q = sh < 64 ? m << sh : 0ull;  // >= 64 only for inf / NaN, whose level is flagged non-finite
+    } else if (sh <= -25) {
+        q = 0;
+    } else {
+        const int k = -sh;
+        q = m >> k;
+        const uint64_t r = m & ((1ull << k) - 1ull), half = 1ull << (k - 1);
+        q += (r > half || (r == half && (q & 1ull))) ? 1ull : 0ull;
+    }
+    return (b >> 31) ? -(long long)q : (long long)q;
 }
 
+// q 2^-S: the int64 -> float conversion rounds once to nearest, the scaling
+// by a power of two is exact (above the subnormal range).
 __device__ __forceinline__ float unfix64(unsigned long long q, int S) {
-    return (float)__builtin_ldexp((double)(long long)q, -S);
+    return __builtin_ldexpf((float)(long long)q, -S);
 }
 
 // Scale exponent of slice s (level `level`); finite = false when the level's

@@ -395,12 +395,24 @@ def test_train_step_vs_fp64(torch_cuda, cfgname):
     loss_h, _, _ = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1, grads=False, half_bins=True)
     assert abs(loss_g - loss_h) <= 2e-3 * abs(loss_h) + 1e-6, (loss_g, loss_h, loss_o)
     assert abs(loss_g - loss_o) <= 2e-2 * abs(loss_o), (loss_g, loss_o)
-    for k, go, L in ((0, gt_o, th), (1, gp_o, ph)):
+    # gradients against FP64 and against the fp16-bin emulation (the bins the
+    # reference differentiates through are fp16, train.h:874-893)
+    _, gt_h, gp_h = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1, half_bins=True)
+    errs = {}
+    for k, go, gh, L in ((0, gt_o, gt_h, th), (1, gp_o, gp_h, ph)):
         gg = f.get_grad(k).astype(np.float64) / 1024.0
         nm = L["n_matrix"]
-        assert _grad_rel(gg[:nm], go[:nm]) <= 3e-2, "net %d matrix grad rel err %.3g" % (k, _grad_rel(gg[:nm], go[:nm]))
+        errs[(k, "matrix")] = (_grad_rel(gg[:nm], go[:nm]), _grad_rel(gg[:nm], gh[:nm]))
         if L["grid"]:
-            assert _grad_rel(gg[nm:], go[nm:]) <= 5e-2, "net %d grid grad rel err %.3g" % (k, _grad_rel(gg[nm:], go[nm:]))
+            errs[(k, "grid")] = (_grad_rel(gg[nm:], go[nm:]), _grad_rel(gg[nm:], gh[nm:]))
+    for key, (e64, eh) in errs.items():
+        print("net %d %s grad rel err: %.3g vs FP64, %.3g vs fp16-bin FP64" % (key[0], key[1], e64, eh))
+    # measured on MI355X: 1.9e-3 .. 6.0e-3 for every net and part, the same
+    # with and without the fp16 bins: what remains is the fp16 MFMA operands
+    # of the backward (activations and data gradients, the reference's own
+    # storage precision).  Held at twice the largest.
+    for key, (e64, eh) in errs.items():
+        assert e64 <= 1.2e-2 and eh <= 1.2e-2, "net %d %s grad rel err %.3g / %.3g" % (key[0], key[1], e64, eh)
         # Adam + EMA applied to the GPU's own gradient
         opt = cfg["nn"]["optimizer"]
         ad = opt["nested"]
@@ -413,6 +425,56 @@ def test_train_step_vs_fp64(torch_cuda, cfgname):
         ema = f.get_params(k, G.DFG_PARAMS_INFERENCE)
         assert np.array_equal(ema, f.get_params(k, G.DFG_PARAMS_TRAIN))
         assert not np.array_equal(ema, ema_before[k])
+    f.close()
+
+
+def test_adam_and_ema_over_steps_vs_fp64(torch_cuda):
+    """Four training steps on different batches.  Master weights against the
+    FP64 Adam (adam.h:47-120) fed the GPU's own gradient of each step, with
+    its moments and per-parameter step counters carried across steps (grid
+    entries no sample touches keep their counter); the EMA (ema.h:62-136)
+    against the debiased recursion at global step t applied to the GPU's
+    previous EMA and new fp16 weights, to fp16 rounding."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    opt = cfg["nn"]["optimizer"]
+    ad = opt["nested"]
+    state = {}
+    for k, L in ((0, th), (1, ph)):
+        n = L["n_params"]
+        state[k] = [f.get_params(k, G.DFG_PARAMS_MASTER).astype(np.float64), np.zeros(n), np.zeros(n),
+                    np.zeros(n, np.uint32), f.get_params(k, G.DFG_PARAMS_INFERENCE).astype(np.float64)]
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    B = 2000
+    for t in range(1, 5):
+        feat, tgt = df_features(B, 300 + t), train_targets(B, 400 + t)
+        f.train_step(dev(torch, feat), dev(torch, tgt), d_loss)
+        torch.cuda.synchronize()
+        for k, L in ((0, th), (1, ph)):
+            master, m, v, steps, ema_prev = state[k]
+            gg = f.get_grad(k).astype(np.float64) / 1024.0
+            master, m, v, steps, _ = O.adam_ema_f64(L["n_matrix"], ad["learning_rate"], ad["beta1"], ad["beta2"],
+                                                    ad["epsilon"], ad["l2_reg"], opt["decay"], t, gg, master, m, v,
+                                                    steps)
+            got = f.get_params(k, G.DFG_PARAMS_MASTER).astype(np.float64)
+            assert np.allclose(got, master, rtol=1e-5, atol=1e-7), "step %d net %d: master differs (max %.3g)" % (
+                t, k, np.abs(got - master).max())
+            w16 = f.get_params(k, G.DFG_PARAMS_TRAIN).astype(np.float64)
+            _, _, _, _, ema = O.adam_ema_f64(0, 0.0, 0.9, 0.99, 1e-8, 0.0, opt["decay"], t, np.zeros(w16.size), w16,
+                                             np.zeros(w16.size), np.zeros(w16.size), np.zeros(w16.size, np.uint32),
+                                             w_lowp=w16, ema=ema_prev)
+            ema_g = f.get_params(k, G.DFG_PARAMS_INFERENCE).astype(np.float64)
+            tol = 2.0 ** -10 * np.abs(ema) + 2.0 ** -24   # one fp16 ulp of the stored EMA
+            bad = np.abs(ema_g - ema) > tol
+            assert not bad.any(), "step %d net %d: EMA differs at %d params (max %.3g)" % (
+                t, k, bad.sum(), np.abs(ema_g - ema).max())
+            if t >= 2:   # past step 1 the EMA is a blend, not the new weights
+                assert not np.array_equal(ema_g, w16)
+            state[k] = [got, m, v, steps, ema_g]
+        # some grid entries were skipped on some step: the counters differ
+        assert len(np.unique(state[0][3][th["n_matrix"]:])) > 1
     f.close()
 
 

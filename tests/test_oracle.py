@@ -730,3 +730,144 @@ def test_fixed_point_grid_gradient_scale_cannot_overflow():
         S = _slice_scale(float(mx), 1)
         assert int(np.rint(np.ldexp(np.float64(mx), S))) < (1 << 63)
     assert _slice_scale(0.0, 5) == 0
+
+
+# ----------------------------------------------------------------------------
+# Hand-worked pins: NRC substitution on hand-built records, the DF loss at
+# uniform logits, Adam and EMA under a constant gradient.
+# ----------------------------------------------------------------------------
+def _hand_records():
+    """Three pixels, four depths, every value chosen so that no record is
+    filtered (finite radiance, throughput above eps):
+      pixel 0: cur_depth 4, no delta or miss vertex;
+      pixel 1: cur_depth 4, vertex 1 delta, vertex 2 a miss;
+      pixel 2: cur_depth 3, vertex 1 delta."""
+    from gpu_util import random_records
+    rec, aabb = random_records(3, max_depth=4, seed=90)
+    D, N = 4, 3
+    rec["cur_depth"] = np.array([4, 4, 3], np.uint32)
+    rec["delta"] = np.zeros((D, N), np.uint8)
+    rec["miss"] = np.zeros((D, N), np.uint8)
+    rec["delta"][1, 1] = 1
+    rec["miss"][2, 1] = 1
+    rec["delta"][1, 2] = 1
+    rec["L"] = (1.0 + np.arange(D * N * 3, dtype=np.float32).reshape(D, N, 3)) * np.float32(0.25)
+    rec["thp"] = np.full((D, N, 3), 0.5, np.float32) + np.arange(D, dtype=np.float32)[:, None, None] * np.float32(0.125)
+    rec["Le"] = np.full((D, N, 3), 0.0625, np.float32)
+    rec["wi_pdf"] = np.full((D, N), 0.5, np.float32)
+    return rec, aabb
+
+
+def test_nrc_substitution_hand_worked():
+    """OPTIMIZE_ON_NRC = 4 (train.cu:156-213, 265-308; workqueue.h:113-138) on
+    hand-built records, the answers worked out by hand:
+      inputs: every vertex below cur_depth that is neither delta nor a miss,
+              pixel-major;  cur_depth drops by one;
+      apply (replaceIndirectRadianceSkipDelta(p, k - 1, R_k)): the first
+              non-delta vertex d <= k - 1 still below the new cur_depth gets
+              L = thp[k-1] * (R_k + Le[k-1]);
+      divide (divideRadiance(p, k, R_k)): L[k] /= R_k per channel where
+              R_k > eps, for k below the new cur_depth."""
+    rec, aabb = _hand_records()
+    _, verts, _, depth = O.nrc_substitute_f64(rec, 3, 0, 1, aabb, None)
+    assert [tuple(v) for v in verts] == [(0, 0), (1, 0), (2, 0), (3, 0), (0, 1), (3, 1), (0, 2), (2, 2)]
+    assert list(depth) == [3, 3, 2]
+    R = np.array([[2.0, 4.0, 1e-6]] * len(verts)) * (1 + np.arange(len(verts)))[:, None]
+    R[:, 2] = [1e-6, 3.0, 1e-6, 5.0, 2.0, 1e-6, 7.0, 1.5]   # some channels at or below eps: not divided
+    _, _, L, depth = O.nrc_substitute_f64(rec, 3, 0, 1, aabb, R)
+    L0 = rec["L"].astype(np.float64)
+    thp, Le = rec["thp"].astype(np.float64), rec["Le"].astype(np.float64)
+
+    def div(v, r):
+        return np.where(r > 1e-5, v / np.where(r > 1e-5, r, 1), v)
+
+    exp = L0.copy()
+    # pixel 0: R for vertices 0..3 are rows 0..3
+    exp[0, 0] = div(thp[0, 0] * (R[1] + Le[0, 0]), R[0])
+    exp[1, 0] = div(thp[1, 0] * (R[2] + Le[1, 0]), R[1])
+    exp[2, 0] = div(thp[2, 0] * (R[3] + Le[2, 0]), R[2])
+    # vertex 3 of pixel 0 is at the new cur_depth: untouched
+    # pixel 1: inputs are vertices 0 (row 4) and 3 (row 5); vertex 3 writes
+    # depth 2 (a miss, not a delta) from its own predecessor's thp and Le
+    exp[2, 1] = thp[2, 1] * (R[5] + Le[2, 1])
+    exp[0, 1] = div(L0[0, 1], R[4])
+    # pixel 2: inputs are vertices 0 (row 6) and 2 (row 7); vertex 2's
+    # predecessor 1 is a delta, so depth 0 gets thp[1] * (R + Le[1])
+    exp[0, 2] = div(thp[1, 2] * (R[7] + Le[1, 2]), R[6])
+    assert np.allclose(L, exp, rtol=1e-12, atol=0)
+    assert list(depth) == [3, 3, 2]
+
+
+def test_df_loss_closed_form_at_uniform_logits():
+    """With the output layers zeroed every logit is 0, the bins are all 1 and
+    q = p_theta p_phi + 1e-5 = 1 + 1e-5 for any direction, so
+    loss = mean_i -Li / (wiPdf_i + 1e-5) * ln(1 + 1e-5), Li = mean of L_rgb
+    (train.h:931-965) — with and without the fp16 bin storage."""
+    cfg = G.hash_config(with_nrc=False, n_levels=4, log2_hashmap_size=8)
+    th, ph, _ = O.config_layouts(cfg)
+    rng = np.random.default_rng(95)
+    pt, pp = rng.normal(0, 0.3, th["n_params"]), rng.normal(0, 0.3, ph["n_params"])
+    for L, p in ((th, pt), (ph, pp)):
+        p[L["layer_off"][L["n_hidden"]]:L["n_matrix"]] = 0.0
+    from gpu_util import df_features, train_targets
+    B = 64
+    feat, tgt = df_features(B, 96), train_targets(B, 97)
+    tgt[:, 5] = rng.uniform(0.05, 3.0, B)   # wiPdf per row
+    eps = float(np.float32(1e-5))
+    Li = tgt[:, 2:5].astype(np.float64).mean(1)
+    expect = np.mean(-Li / (tgt[:, 5].astype(np.float64) + eps) * math.log(1 + eps))
+    for half in (False, True):
+        loss, _, _ = O.df_train_f64(th, ph, pt, pp, feat, 9, tgt, 1, grads=False, half_bins=half)
+        assert abs(loss - expect) <= 1e-12 * abs(expect), (half, loss, expect)
+
+
+def test_adam_constant_gradient_closed_form():
+    """Under a constant gradient g the bias-corrected Adam step is
+    lr * sqrt(1-b2^t)/(1-b1^t) * g(1-b1^t) / (|g| sqrt(1-b2^t) + eps), i.e.
+    lr * sign(g) to eps / |g|: after t steps w = w0 - t lr sign(g).  A grid
+    (non-matrix) parameter whose gradient is 0 on a step is skipped entirely
+    (its own step counter stays), so (g, 0, g) moves it by 2 lr; a matrix
+    parameter is updated on the zero step from its decayed moments."""
+    lr, b1, b2, eps = 1e-2, 0.9, 0.99, 1e-15
+    n, n_matrix = 6, 3
+    w0 = np.array([0.5, -0.25, 1.0, 0.75, -0.5, 2.0])
+    g = np.array([0.3, -2.0, 1e-3, 0.7, -0.05, 5.0])
+    m, v, st, w = np.zeros(n), np.zeros(n), np.zeros(n, np.uint32), w0.copy()
+    for t in range(1, 5):
+        w, m, v, st, _ = O.adam_ema_f64(n_matrix, lr, b1, b2, eps, 0.0, 0.95, t, g, w, m, v, st)
+        assert np.allclose(w, w0 - t * lr * np.sign(g), rtol=0, atol=1e-12), t
+        assert np.allclose(m, g * (1 - b1 ** t), rtol=1e-12) and np.allclose(v, g * g * (1 - b2 ** t), rtol=1e-12)
+    # a zero gradient on step 2 of 3
+    m, v, st, w = np.zeros(n), np.zeros(n), np.zeros(n, np.uint32), w0.copy()
+    for t, gg in enumerate((g, np.zeros(n), g), 1):
+        w, m, v, st, _ = O.adam_ema_f64(n_matrix, lr, b1, b2, eps, 0.0, 0.95, t, gg, w, m, v, st)
+    assert list(st) == [3, 3, 3, 2, 2, 2]
+    assert np.allclose(w[3:], w0[3:] - 2 * lr * np.sign(g[3:]), rtol=0, atol=1e-12)
+    # matrix params moved on all three steps: lr on step 1; on step 2 by the
+    # decayed moments m = b1 (1-b1) g, v = b2 (1-b2) g^2 at t = 2; on step 3
+    # by m = (b1^2 (1-b1) + 1-b1) g, v = (b2^2 (1-b2) + 1-b2) g^2 at t = 3
+    s2 = lr * math.sqrt(1 - b2 ** 2) / (1 - b1 ** 2) * b1 * (1 - b1) / math.sqrt(b2 * (1 - b2))
+    s3 = lr * math.sqrt(1 - b2 ** 3) / (1 - b1 ** 3) * (b1 * b1 * (1 - b1) + 1 - b1) / math.sqrt(
+        b2 * b2 * (1 - b2) + 1 - b2)
+    assert np.allclose(w[:3], w0[:3] - (lr + s2 + s3) * np.sign(g[:3]), rtol=0, atol=1e-10)
+    # l2 only on matrix params: with l2 > 0 and g = 0 a grid entry stays put
+    w2, _, _, st2, _ = O.adam_ema_f64(n_matrix, lr, b1, b2, eps, 1e-2, 0.95, 1, np.zeros(n), w0, np.zeros(n),
+                                      np.zeros(n), np.zeros(n, np.uint32))
+    assert np.array_equal(w2[3:], w0[3:]) and list(st2) == [1, 1, 1, 0, 0, 0]
+    assert np.allclose(w2[:3], w0[:3] - lr * np.sign(w0[:3]), rtol=0, atol=1e-12)
+
+
+def test_ema_is_the_debiased_exponential_average():
+    """EmaOptimizer (ema.h:62-136): after t steps of weights w_1..w_t the EMA
+    is sum_s d^(t-s) (1-d) w_s / (1 - d^t), the normalised exponential average;
+    the recursion's debias terms must reproduce that sum at every t."""
+    d = 0.95
+    rng = np.random.default_rng(98)
+    n = 5
+    ws = [rng.normal(0, 1, n) for _ in range(6)]
+    ema = rng.normal(0, 1, n)   # the initial value has zero weight after debiasing
+    for t in range(1, 7):
+        _, _, _, _, ema = O.adam_ema_f64(0, 0.0, 0.9, 0.99, 1e-8, 0.0, d, t, np.zeros(n), ws[t - 1], np.zeros(n),
+                                         np.zeros(n), np.zeros(n, np.uint32), w_lowp=ws[t - 1], ema=ema)
+        expect = sum(d ** (t - s) * (1 - d) * ws[s - 1] for s in range(1, t + 1)) / (1 - d ** t)
+        assert np.allclose(ema, expect, rtol=1e-12, atol=1e-12), t

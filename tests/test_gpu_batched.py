@@ -175,3 +175,110 @@ def test_server_training_interleave_is_ordered(torch_cuda):
             assert np.array_equal(host(res[i][0]), host(w))
     srv.close()
     f.close()
+
+
+# ----------------------------------------------------------------------------
+# The two batched workloads at full size
+# ----------------------------------------------------------------------------
+def test_wavefront_full_frame_matches_per_bounce_and_fp64(torch_cuda):
+    """The 6-bounce frame from 2,097,152 rows shrinking by 0.7 per bounce
+    (6,168,076 queries), 64/64 bins, in one captured persistent launch with
+    the grid pre-pass: bit-identical to dfg_query per bounce, and a 4,096-row
+    slice (first rows of the first and last bounces) within the query's
+    tolerance of the FP64 oracle."""
+    import oracle as O
+    torch = torch_cuda
+    cfg = G.hash_config(n_bins_theta=64, n_bins_phi=64, with_nrc=False)
+    f = G.GuidingField(cfg)
+    sizes = _bounces(2097152, 6)
+    assert sizes[-1] == 352467 and sum(sizes) == 6168076
+    cap = sum(sizes)
+    feat, u = df_features(cap, 180), uniforms(cap, 181)
+    wi0 = wi_init(cap)
+    d_feat, d_u = dev(torch, feat), dev(torch, u)
+    ref = torch.empty((cap, 4), dtype=torch.float32, device="cuda")
+    ref.copy_(dev(torch, wi0))
+    row = 0
+    for n in sizes:
+        f.query(d_feat[row:row + n], d_u[row:row + n], ref[row:row + n])
+        row += n
+    wf = G.Wavefront(f, 6, cap)
+    wf.set_tuning(8, 2, 1)
+    d_sizes = torch.tensor(sizes, dtype=torch.int64, device="cuda")
+    d_wi = dev(torch, wi0)
+    wf.run(d_sizes, d_feat, d_u, d_wi)
+    torch.cuda.synchronize()
+    g, r = host(d_wi), host(ref)
+    assert np.array_equal(g, r), "frame differs from per-bounce queries in %d rows" % (g != r).any(1).sum()
+    th, ph, _ = O.config_layouts(cfg)
+    rows = np.concatenate([np.arange(2048), cap - sizes[-1] + np.arange(2048)])
+    o = O.query_f64(th, ph, f.get_params(0, G.DFG_PARAMS_INFERENCE), f.get_params(1, G.DFG_PARAMS_INFERENCE),
+                    feat[rows], 9, u[rows], wi0[rows], 1)
+    rel = np.abs(g[rows, 2] * g[rows, 3] - o[:, 2].astype(np.float64) * o[:, 3]) / np.abs(o[:, 2] * o[:, 3])
+    assert np.median(rel) <= 1e-3 and np.percentile(rel, 99) <= 3e-2, (np.median(rel), np.percentile(rel, 99))
+    wf.close()
+    f.close()
+
+
+def test_server_full_size_mix_alone_equals_mix(torch_cuda):
+    """The served workload: 24 requests of log-uniform size in [1,024,
+    1,048,576] rows, a 262,144-sample DF + NRC training step after every 8,
+    a 16 GiB admission budget, two streams.  Every request equals the same
+    request queried alone on a second field (same seed) that has taken the
+    same training steps as the weights version the server reports for it;
+    the pre-training requests also match the FP64 oracle on a slice."""
+    import oracle as O
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True)
+    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg)
+    rng = np.random.default_rng(4)
+    n_req = 24
+    sizes = np.exp(rng.uniform(np.log(1024), np.log(1 << 20), n_req)).astype(np.int64)
+    sizes[3] = 1 << 20            # the largest size of the range is in the mix
+    reqs = [(dev(torch, df_features(int(n), 2000 + i)), dev(torch, uniforms(int(n), 3000 + i)))
+            for i, n in enumerate(sizes)]
+    B = 262144
+    tf = dev(torch, df_features(B, 8, nrc=True))
+    tt = dev(torch, train_targets(B, 9))
+    trgb = dev(torch, np.exp(np.random.default_rng(10).normal(0.0, 1.0, (B, 3))).astype(np.float32))
+    srv = G.Server(fa, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8)
+    for _ in range(n_req // 8 + 1):
+        srv.submit_train(tf, tt, nrc_feat=tf, nrc_target=trgb)
+    outs = []
+    for i, (fe, uu) in enumerate(reqs):
+        w = dev(torch, wi_init(fe.shape[0]))
+        outs.append(w)
+        srv.submit(i, fe, uu, w)
+        srv.progress()
+    srv.progress(wait=True)
+    ids, lat, ver = srv.drain()
+    st = srv.stats()
+    assert sorted(ids.tolist()) == list(range(n_req)) and st["train_steps"] >= 2
+    version = dict(zip(ids.tolist(), ver.tolist()))
+    final = max(version.values())
+    assert final >= 1
+    checked = 0
+    for v in range(final + 1):
+        for i in sorted(k for k in version if version[k] == v):
+            fe, uu = reqs[i]
+            w = dev(torch, wi_init(fe.shape[0]))
+            fb.query(fe, uu, w)
+            torch.cuda.synchronize()
+            assert np.array_equal(host(outs[i]), host(w)), "request %d (version %d) differs alone vs mixed" % (i, v)
+            checked += 1
+            if v == 0 and i == min(k for k in version if version[k] == 0):
+                th, ph, _ = O.config_layouts(cfg)
+                k = min(4096, fe.shape[0])
+                o = O.query_f64(th, ph, fb.get_params(0, G.DFG_PARAMS_INFERENCE),
+                                fb.get_params(1, G.DFG_PARAMS_INFERENCE), host(fe[:k]), 9, host(uu[:k]),
+                                wi_init(k), 1)
+                g = host(outs[i])[:k]
+                rel = np.abs(g[:, 2] * g[:, 3] - o[:, 2].astype(np.float64) * o[:, 3]) / np.abs(o[:, 2] * o[:, 3])
+                assert np.median(rel) <= 1e-3, np.median(rel)
+        if v < final:
+            fb.train_step(tf, tt)
+            fb.nrc_train_step(tf, trgb)
+    assert checked == n_req
+    srv.close()
+    fa.close()
+    fb.close()

@@ -280,6 +280,54 @@ void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entrie
 void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which, hipStream_t s);
 int64_t pack_slots(const NetSpec& hs, int which);
 // Up to four packs in one launch.
+// sampleWi (integrator.cpp:355-401): the caller's uniforms, BSDF sample and
+// smooth flags in; the RR / guide / mixture flags and wi slots out.
+struct SampleWiArgs {
+    float rr_prob, bsdf_fraction;
+    int guiding_enabled, max_guided_depth;
+    const int32_t* depth;
+    const uint8_t* smooth;
+    const float* u_rr;
+    const float* u_mix;
+    const float* bsdf_wi;
+    const float* bsdf_pdf;
+    const float* bsdf_f;
+    uint8_t* rr_killed;
+    uint8_t* guided;
+    uint8_t* mix;
+    float* wi;
+    float* thp;
+};
+
+// The training-record step of handleIntersections (integrator.cpp:277-296).
+struct RecordScatterArgs {
+    int train_enabled, max_train_depth, mis_aware, product_with_cosine;
+    uint32_t pixel_offset, pixel_stride;
+    const int32_t* pixel;
+    const int32_t* depth;
+    const uint8_t* rr_killed;
+    const uint8_t* alive;
+    const float* wi_dir;
+    const float* thp;
+    const float* scatter_pdf;
+    const float* mis;
+    const float* bsdf_val;
+    const float* cos_theta;
+    const uint8_t* delta;
+};
+
+// recordRadiance / overwriteLe of handleEmissiveHit and handleMiss
+// (integrator.cpp:144-191).
+struct RecordRadianceArgs {
+    int train_enabled, max_train_depth;
+    uint32_t pixel_offset, pixel_stride;
+    const int32_t* pixel;
+    const int32_t* depth;
+    const float* contrib;
+    const float* Le;
+    const uint8_t* miss;
+};
+
 struct PackJob {
     const NetSpec* dns;
     const _Float16* w16;

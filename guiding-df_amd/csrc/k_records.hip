@@ -282,7 +282,194 @@ __global__ void pdf_image_rows_kernel(const float* __restrict__ feat, int D, int
     wi2[2 * i + 1] = ((float)b + 0.5f) / (float)rp;
 }
 
+// ---------------------------------------------------------------------------
+// The consumer stage around the query (GuidedPathTracer, integrator.cpp):
+// the Russian-roulette and guide-or-BSDF decision of sampleWi, and the
+// training-record producers of GuidedPixelStateBuffer (workqueue.h:26-92).
+// One thread per queue row; rows whose pixel is outside the records are
+// ignored; a pixel appears at most once per launch (one ray per pixel and
+// bounce), so the per-pixel updates do not race.
+// ---------------------------------------------------------------------------
+
+// v.normalized() as Eigen computes it: v / |v|, v itself when |v| = 0
+__device__ __forceinline__ void normalized3(const float* __restrict__ v, float (&o)[3]) {
+    const float n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const float n = sqrtf(n2);
+    for (int c = 0; c < 3; ++c) o[c] = n2 > 0.0f ? v[c] / n : v[c];
+}
+
+// cartesianToSpherical, math_utils.h:183-190: (acos z, atan2(y, x) in [0, 2pi))
+__device__ __forceinline__ void cart_to_sph(const float* __restrict__ v, float* __restrict__ out) {
+    float vn[3];
+    normalized3(v, vn);
+    const float th = acosf(fminf(fmaxf(vn[2], -1.0f), 1.0f));  // safe_acos
+    float ph = atan2f(vn[1], vn[0]);
+    if (ph < 0.0f) ph += kTwoPi;
+    out[0] = th;
+    out[1] = ph;
+}
+
+// uniformSphereToSquare, math_utils.h:233-240
+__device__ __forceinline__ void sphere_to_square(const float* __restrict__ v, float& u0, float& u1) {
+    float vn[3];
+    normalized3(v, vn);
+    const float ph = atan2f(vn[1], vn[0]) * (1.0f / kTwoPi);
+    u0 = (1.0f - vn[2]) * 0.5f;
+    u1 = ph < 0.0f ? ph + 1.0f : ph;
+}
+
+// isTrainingPixel, guided.h:26-28 (uint32 arithmetic as the reference's)
+__device__ __forceinline__ bool training_pixel(uint32_t p, uint32_t offset, uint32_t stride) {
+    return (p - offset) % stride == 0u;
+}
+
+// sampleWi (integrator.cpp:355-401) and the w.thp /= probRR of
+// handleIntersections (integrator.cpp:211-212)
+__global__ void sample_wi_kernel(int64_t n, SampleWiArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool killed = a.u_rr[i] >= a.rr_prob;
+    if (a.rr_killed) a.rr_killed[i] = killed ? 1 : 0;
+    if (killed) {
+        if (a.guided) a.guided[i] = 0;
+        if (a.mix) a.mix[i] = 0;
+        return;
+    }
+    if (a.thp)
+        for (int c = 0; c < 3; ++c) a.thp[3 * i + c] = a.thp[3 * i + c] / a.rr_prob;
+    const bool on = a.guiding_enabled && a.depth[i] < a.max_guided_depth && a.smooth[i] != 0;
+    const bool guided = on && (a.bsdf_fraction == 0.0f || a.u_mix[i] >= a.bsdf_fraction);
+    if (a.guided) a.guided[i] = guided ? 1 : 0;
+    // a BSDF-sampled direction still gets the guide's pdf for the mixture
+    // when guiding is on at this depth, f < 1 and the BSDF is smooth
+    // (integrator.cpp:256-261)
+    if (a.mix) a.mix[i] = (!guided && on && a.bsdf_fraction < 1.0f) ? 1 : 0;
+    float* w = a.wi + 4 * i;
+    w[2] = 1.0f;
+    w[3] = 1.0f;
+    if (!guided) {
+        const float* bf = a.bsdf_f + 3 * i;
+        if (a.bsdf_pdf[i] != 0.0f && (bf[0] != 0.0f || bf[1] != 0.0f || bf[2] != 0.0f))
+            sphere_to_square(a.bsdf_wi + 3 * i, w[0], w[1]);
+    }
+}
+
+// incrementDepthLastBounceNRC (workqueue.h:60-81) of shading row i (or of an
+// empty ShadingData when sd_row < 0: zero position, reflectances and
+// directions, roughness 1, a diffuse BSDF)
+__device__ void last_bounce_record(const dfg_records& r, const dfg_shading& sd, int64_t sd_row, uint32_t p,
+                                   bool miss) {
+    const uint32_t d = r.cur_depth[p];
+    if (d >= (uint32_t)r.max_depth) return;
+    const int64_t k = (int64_t)d * r.n_pixels + p;
+    r.cur_depth[p] = d + 1;
+    if (sd_row >= 0) {
+        for (int c = 0; c < 3; ++c) {
+            r.pos[3 * k + c] = sd.pos[3 * sd_row + c];
+            r.diffuse[3 * k + c] = sd.diffuse[3 * sd_row + c];
+            r.specular[3 * k + c] = sd.specular[3 * sd_row + c];
+        }
+        cart_to_sph(sd.normal + 3 * sd_row, r.normal + 2 * k);
+        cart_to_sph(sd.wo + 3 * sd_row, r.wo + 2 * k);
+        r.roughness[k] = sd.roughness[sd_row];
+        r.delta[k] = sd.smooth[sd_row] ? 0 : 1;
+    } else {
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < 3; ++c) r.pos[3 * k + c] = r.diffuse[3 * k + c] = r.specular[3 * k + c] = 0.0f;
+        cart_to_sph(zero, r.normal + 2 * k);
+        cart_to_sph(zero, r.wo + 2 * k);
+        r.roughness[k] = 1.0f;
+        r.delta[k] = 0;
+    }
+    r.miss[k] = miss ? 1 : 0;
+    for (int c = 0; c < 3; ++c) {
+        r.L[3 * k + c] = 0.0f;
+        r.thp[3 * k + c] = 0.0f;
+        if (r.Le) r.Le[3 * k + c] = 0.0f;
+    }
+}
+
+// The record step of handleIntersections (integrator.cpp:277-296) and
+// handleNRCLastBounce (integrator.cpp:193-201)
+__global__ void record_scatter_kernel(int64_t n, const dfg_records r, const dfg_shading sd, RecordScatterArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (a.rr_killed && a.rr_killed[i]) return;
+    const int32_t pix = a.pixel[i];
+    if (pix < 0 || (int64_t)pix >= r.n_pixels) return;
+    const uint32_t p = (uint32_t)pix;
+    if (!(a.train_enabled && a.depth[i] < a.max_train_depth && training_pixel(p, a.pixel_offset, a.pixel_stride)))
+        return;
+    if (!(a.alive && a.alive[i])) {
+        last_bounce_record(r, sd, i, p, false);
+        return;
+    }
+    // incrementDepth (workqueue.h:26-58)
+    const uint32_t d = r.cur_depth[p];
+    if (d >= (uint32_t)r.max_depth) return;
+    const int64_t k = (int64_t)d * r.n_pixels + p;
+    const float c = fabsf(a.cos_theta[i]);
+    for (int ch = 0; ch < 3; ++ch) {
+        r.L[3 * k + ch] = 0.0f;
+        r.thp[3 * k + ch] = a.thp[3 * i + ch];
+        r.pos[3 * k + ch] = sd.pos[3 * i + ch];
+        r.diffuse[3 * k + ch] = sd.diffuse[3 * i + ch];
+        r.specular[3 * k + ch] = sd.specular[3 * i + ch];
+        const float b = a.bsdf_val[3 * i + ch];
+        r.bsdf_val[3 * k + ch] = a.product_with_cosine ? b * c : b;
+        if (r.Le) r.Le[3 * k + ch] = 0.0f;
+    }
+    cart_to_sph(a.wi_dir + 3 * i, r.dir + 2 * k);
+    cart_to_sph(sd.normal + 3 * i, r.normal + 2 * k);
+    cart_to_sph(sd.wo + 3 * i, r.wo + 2 * k);
+    r.wi_pdf[k] = a.scatter_pdf[i];
+    r.mis_weight[k] = a.mis_aware ? a.mis[i] : 1.0f;
+    r.delta[k] = a.delta && a.delta[i] ? 1 : 0;
+    r.miss[k] = 0;
+    r.roughness[k] = sd.roughness[i];
+    r.cur_depth[p] = d + 1;
+}
+
+// recordRadiance + overwriteLe (workqueue.h:83-106) of handleEmissiveHit and
+// handleMiss (integrator.cpp:144-191), with the miss's placeholder record
+__global__ void record_radiance_kernel(int64_t n, const dfg_records r, RecordRadianceArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t pix = a.pixel[i];
+    if (pix < 0 || (int64_t)pix >= r.n_pixels) return;
+    const uint32_t p = (uint32_t)pix;
+    if (!(a.train_enabled && training_pixel(p, a.pixel_offset, a.pixel_stride))) return;
+    const uint32_t cd = r.cur_depth[p];
+    const uint32_t dd = cd < (uint32_t)r.max_depth ? cd : (uint32_t)r.max_depth;
+    for (uint32_t d = 0; d < dd; ++d) {
+        const int64_t k = (int64_t)d * r.n_pixels + p;
+        for (int c = 0; c < 3; ++c) r.L[3 * k + c] = r.L[3 * k + c] + a.contrib[3 * i + c];
+    }
+    const int last = (int)cd - 1;
+    if (r.Le && a.Le && last >= 0 && last < r.max_depth) {
+        const int64_t k = (int64_t)last * r.n_pixels + p;
+        for (int c = 0; c < 3; ++c) r.Le[3 * k + c] = a.Le[3 * i + c];
+    }
+    if (a.miss && a.miss[i] && a.depth[i] <= a.max_train_depth) last_bounce_record(r, dfg_shading{}, -1, p, true);
+}
+
 }  // namespace
+
+void launch_sample_wi(int64_t n, const SampleWiArgs& a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sample_wi_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a);
+}
+
+void launch_record_scatter(int64_t n, const dfg_records& r, const dfg_shading& sd, const RecordScatterArgs& a,
+                           hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(record_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, r, sd, a);
+}
+
+void launch_record_radiance(int64_t n, const dfg_records& r, const RecordRadianceArgs& a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(record_radiance_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, r, a);
+}
 
 void launch_pdf_image_rows(const float* feat, int D, int rt, int rp, float* rows, float* wi2, hipStream_t s) {
     const int64_t n = (int64_t)rt * rp;

@@ -20,6 +20,10 @@ void launch_pdf_image_rows(const float* feat, int D, int rt, int rp, float* rows
 void launch_scatter(int64_t n, float f, const float* wi, const uint8_t* guided, const uint8_t* mix,
                     const float* bsdf_pdf, const float* bsdf_val, const float* cos_theta, float* thp, float* spdf,
                     float* mis, uint8_t* alive, hipStream_t s);
+void launch_sample_wi(int64_t n, const SampleWiArgs& a, hipStream_t s);
+void launch_record_scatter(int64_t n, const dfg_records& r, const dfg_shading& sd, const RecordScatterArgs& a,
+                           hipStream_t s);
+void launch_record_radiance(int64_t n, const dfg_records& r, const RecordRadianceArgs& a, hipStream_t s);
 }
 
 namespace {
@@ -241,6 +245,85 @@ int dfg_scatter(void* stream, int64_t n, float bsdf_fraction, const float* wi, c
     if (!wi || !bsdf_pdf || !bsdf_val || !cos_theta || !thp) return DFG_ERR_INVALID_ARG;
     launch_scatter(n, bsdf_fraction, wi, guided, mix, bsdf_pdf, bsdf_val, cos_theta, thp, scatter_pdf, mis, alive,
                    (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+int dfg_sample_wi(void* stream, int64_t n, float rr_prob, float bsdf_fraction, int guiding_enabled,
+                  int max_guided_depth, const int32_t* depth, const uint8_t* smooth, const float* u_rr,
+                  const float* u_mix, const float* bsdf_wi, const float* bsdf_pdf, const float* bsdf_f,
+                  uint8_t* rr_killed, uint8_t* guided, uint8_t* mix, float* wi, float* thp) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (!(rr_prob > 0.0f && rr_prob <= 1.0f) || !(bsdf_fraction >= 0.0f && bsdf_fraction <= 1.0f))
+        return DFG_ERR_INVALID_ARG;
+    if (n == 0) return DFG_OK;
+    if (!depth || !smooth || !u_rr || !u_mix || !bsdf_wi || !bsdf_pdf || !bsdf_f || !wi) return DFG_ERR_INVALID_ARG;
+    SampleWiArgs a;
+    a.rr_prob = rr_prob;
+    a.bsdf_fraction = bsdf_fraction;
+    a.guiding_enabled = guiding_enabled != 0;
+    a.max_guided_depth = max_guided_depth;
+    a.depth = depth; a.smooth = smooth; a.u_rr = u_rr; a.u_mix = u_mix;
+    a.bsdf_wi = bsdf_wi; a.bsdf_pdf = bsdf_pdf; a.bsdf_f = bsdf_f;
+    a.rr_killed = rr_killed; a.guided = guided; a.mix = mix; a.wi = wi; a.thp = thp;
+    launch_sample_wi(n, a, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+static int check_producer_records(const dfg_records* r, bool scatter) {
+    if (!r) return DFG_ERR_INVALID_ARG;
+    if (r->max_depth < 1 || r->n_pixels < 1) return DFG_ERR_BAD_SIZE;
+    const void* need[] = {r->cur_depth, r->pos, r->wo, r->normal, r->roughness, r->L, r->thp, r->diffuse,
+                          r->specular, r->delta, r->miss};
+    for (const void* p : need)
+        if (!p) return DFG_ERR_INVALID_ARG;
+    if (scatter && (!r->dir || !r->wi_pdf || !r->mis_weight || !r->bsdf_val)) return DFG_ERR_INVALID_ARG;
+    return DFG_OK;
+}
+
+int dfg_record_scatter(void* stream, int64_t n, dfg_records* rec, const dfg_shading* sd, int train_enabled,
+                       int max_train_depth, uint32_t pixel_offset, uint32_t pixel_stride, int mis_aware,
+                       int product_with_cosine, const int32_t* pixel, const int32_t* depth, const uint8_t* rr_killed,
+                       const uint8_t* alive, const float* wi_dir, const float* thp, const float* scatter_pdf,
+                       const float* mis, const float* bsdf_val, const float* cos_theta, const uint8_t* delta) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (pixel_stride < 1) return DFG_ERR_BAD_SIZE;
+    if (n == 0) return DFG_OK;
+    const int rc = check_producer_records(rec, alive != nullptr);
+    if (rc != DFG_OK) return rc;
+    if (!sd || !sd->pos || !sd->normal || !sd->wo || !sd->roughness || !sd->diffuse || !sd->specular || !sd->smooth ||
+        !pixel || !depth)
+        return DFG_ERR_INVALID_ARG;
+    if (alive && (!wi_dir || !thp || !scatter_pdf || !bsdf_val || !cos_theta || (mis_aware && !mis)))
+        return DFG_ERR_INVALID_ARG;
+    RecordScatterArgs a;
+    a.train_enabled = train_enabled != 0;
+    a.max_train_depth = max_train_depth;
+    a.mis_aware = mis_aware != 0;
+    a.product_with_cosine = product_with_cosine != 0;
+    a.pixel_offset = pixel_offset;
+    a.pixel_stride = pixel_stride;
+    a.pixel = pixel; a.depth = depth; a.rr_killed = rr_killed; a.alive = alive; a.wi_dir = wi_dir; a.thp = thp;
+    a.scatter_pdf = scatter_pdf; a.mis = mis; a.bsdf_val = bsdf_val; a.cos_theta = cos_theta; a.delta = delta;
+    launch_record_scatter(n, *rec, *sd, a, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
+}
+
+int dfg_record_radiance(void* stream, int64_t n, dfg_records* rec, int train_enabled, int max_train_depth,
+                        uint32_t pixel_offset, uint32_t pixel_stride, const int32_t* pixel, const int32_t* depth,
+                        const float* contrib, const float* Le, const uint8_t* miss) {
+    if (n < 0) return DFG_ERR_BAD_SIZE;
+    if (pixel_stride < 1) return DFG_ERR_BAD_SIZE;
+    if (n == 0) return DFG_OK;
+    const int rc = check_producer_records(rec, false);
+    if (rc != DFG_OK) return rc;
+    if (!pixel || !contrib || (miss && !depth)) return DFG_ERR_INVALID_ARG;
+    RecordRadianceArgs a;
+    a.train_enabled = train_enabled != 0;
+    a.max_train_depth = max_train_depth;
+    a.pixel_offset = pixel_offset;
+    a.pixel_stride = pixel_stride;
+    a.pixel = pixel; a.depth = depth; a.contrib = contrib; a.Le = Le; a.miss = miss;
+    launch_record_radiance(n, *rec, a, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? DFG_OK : DFG_ERR_HIP;
 }
 

@@ -36,7 +36,8 @@ EXPORTS = [
     "dfg_set_params", "dfg_save", "dfg_load", "dfg_query", "dfg_query_noalloc", "dfg_query_plain", "dfg_pdf",
     "dfg_train_step", "dfg_nrc_infer", "dfg_nrc_train_step", "dfg_train_step_plain",
     "dfg_nrc_train_step_plain", "dfg_training_targets", "dfg_nrc_training_targets", "dfg_train_records",
-    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_pdf_image", "dfg_directions", "dfg_scatter", "dfg_prepare_features",
+    "dfg_nrc_train_records", "dfg_nrc_substitute_radiance", "dfg_pdf_image", "dfg_directions", "dfg_scatter", "dfg_sample_wi",
+    "dfg_record_scatter", "dfg_record_radiance", "dfg_prepare_features",
     "dfg_df_head", "dfg_logits", "dfg_encode", "dfg_grid_index",
     "dfg_wavefront_create", "dfg_wavefront_destroy", "dfg_wavefront_set_tuning", "dfg_wavefront_get_tuning",
     "dfg_wavefront_run", "dfg_wavefront_autotune",
@@ -92,6 +93,22 @@ RECORD_FIELDS = ("cur_depth", "pos", "dir", "wo", "normal", "roughness", "L", "t
 class DfgRecords(ctypes.Structure):
     _fields_ = [("max_depth", ctypes.c_int), ("n_pixels", ctypes.c_int64)] + \
                [(f, ctypes.c_void_p) for f in RECORD_FIELDS]
+
+
+SHADING_FIELDS = ("pos", "normal", "wo", "roughness", "diffuse", "specular", "smooth")
+
+
+class DfgShading(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_void_p) for f in SHADING_FIELDS]
+
+
+def shading_struct(sd):
+    """dfg_shading view of a dict of device tensors [n(, k)]."""
+    r = DfgShading()
+    for f in SHADING_FIELDS:
+        t = sd.get(f)
+        setattr(r, f, None if t is None else t.data_ptr())
+    return r
 
 
 def records_struct(rec):
@@ -151,6 +168,9 @@ def load_library(path=LIB_PATH):
         "dfg_pdf_image": ([P, P, P, I, I, P], I),
         "dfg_directions": ([P, I64, P, P, P], I),
         "dfg_scatter": ([P, I64, ctypes.c_float, P, P, P, P, P, P, P, P, P, P], I),
+        "dfg_sample_wi": ([P, I64, ctypes.c_float, ctypes.c_float, I, I] + [P] * 12, I),
+        "dfg_record_scatter": ([P, I64, P, P, I, I, ctypes.c_uint32, ctypes.c_uint32, I, I] + [P] * 11, I),
+        "dfg_record_radiance": ([P, I64, P, I, I, ctypes.c_uint32, ctypes.c_uint32] + [P] * 5, I),
         "dfg_prepare_features": ([P, I64, P, P, P, P, P, P], I),
         "dfg_df_head": ([P, I64, I, I, I, P, P, P, P, P, P, P, P], I),
         "dfg_logits": ([P, P, I64, P, I, P, P, P], I),
@@ -573,6 +593,43 @@ def scatter(wi, bsdf_pdf, bsdf_val, cos_theta, thp, bsdf_fraction=0.5, guided=No
                          _ptr(alive))
     if rc != 0:
         raise DfgError(rc, "dfg_scatter")
+
+
+def sample_wi(depth, smooth, u_rr, u_mix, bsdf_wi, bsdf_pdf, bsdf_f, wi, rr_killed=None, guided=None, mix=None,
+              thp=None, rr_prob=0.8, bsdf_fraction=0.3, guiding_enabled=True, max_guided_depth=6, stream=None):
+    """The RR and guide-or-BSDF decision before the query (sampleWi)."""
+    lib = load_library()
+    rc = lib.dfg_sample_wi(_stream(stream), depth.shape[0], rr_prob, bsdf_fraction, int(guiding_enabled),
+                           max_guided_depth, _ptr(depth), _ptr(smooth), _ptr(u_rr), _ptr(u_mix), _ptr(bsdf_wi),
+                           _ptr(bsdf_pdf), _ptr(bsdf_f), _ptr(rr_killed), _ptr(guided), _ptr(mix), _ptr(wi), _ptr(thp))
+    if rc != 0:
+        raise DfgError(rc, "dfg_sample_wi")
+
+
+def record_scatter(rec, sd, pixel, depth, alive=None, wi_dir=None, thp=None, scatter_pdf=None, mis=None,
+                   bsdf_val=None, cos_theta=None, delta=None, rr_killed=None, train_enabled=True, max_train_depth=5,
+                   pixel_offset=0, pixel_stride=1, mis_aware=True, product_with_cosine=False, stream=None):
+    """Training records of a scattered (alive) or ended path vertex (handleIntersections)."""
+    lib = load_library()
+    r, h = records_struct(rec), shading_struct(sd)
+    rc = lib.dfg_record_scatter(_stream(stream), pixel.shape[0], ctypes.byref(r), ctypes.byref(h), int(train_enabled),
+                                max_train_depth, pixel_offset, pixel_stride, int(mis_aware), int(product_with_cosine),
+                                _ptr(pixel), _ptr(depth), _ptr(rr_killed), _ptr(alive), _ptr(wi_dir), _ptr(thp),
+                                _ptr(scatter_pdf), _ptr(mis), _ptr(bsdf_val), _ptr(cos_theta), _ptr(delta))
+    if rc != 0:
+        raise DfgError(rc, "dfg_record_scatter")
+
+
+def record_radiance(rec, pixel, contrib, Le=None, miss=None, depth=None, train_enabled=True, max_train_depth=5,
+                    pixel_offset=0, pixel_stride=1, stream=None):
+    """Radiance reaching the recorded path vertices (handleEmissiveHit / handleMiss)."""
+    lib = load_library()
+    r = records_struct(rec)
+    rc = lib.dfg_record_radiance(_stream(stream), pixel.shape[0], ctypes.byref(r), int(train_enabled),
+                                 max_train_depth, pixel_offset, pixel_stride, _ptr(pixel), _ptr(depth), _ptr(contrib),
+                                 _ptr(Le), _ptr(miss))
+    if rc != 0:
+        raise DfgError(rc, "dfg_record_radiance")
 
 
 def prepare_features(pos, aabb, wo, normal, roughness, feat, stream=None):

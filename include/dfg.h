@@ -190,25 +190,80 @@ int dfg_nrc_train_step_plain(dfg_ctx* ctx, void* stream, int64_t B, const float*
 typedef struct dfg_records {
     int max_depth;             /* depth slices (MAX_TRAIN_DEPTH, parameters.h:24)        */
     int64_t n_pixels;          /* pixels per slice                                      */
-    uint32_t* cur_depth;       /* [n_pixels] vertices recorded per pixel (written only by
-                                  dfg_nrc_substitute_radiance)                          */
-    const float* pos;          /* [3] world-space vertex                                */
-    const float* dir;          /* [2] spherical (theta, phi) of the scattered direction */
-    const float* wo;           /* [2] spherical outgoing direction                      */
-    const float* normal;       /* [2] spherical shading normal                          */
+    uint32_t* cur_depth;       /* [n_pixels] vertices recorded per pixel                */
+    float* pos;                /* [3] world-space vertex                                */
+    float* dir;                /* [2] spherical (theta, phi) of the scattered direction */
+    float* wo;                 /* [2] spherical outgoing direction                      */
+    float* normal;             /* [2] spherical shading normal                          */
+    float* roughness;          /* [1]                                                   */
+    float* L;                  /* [3] radiance gathered after the vertex                */
+    float* thp;                /* [3] throughput up to the vertex                       */
+    float* bsdf_val;           /* [3]                                                   */
+    float* diffuse;            /* [3]                                                   */
+    float* specular;           /* [3]                                                   */
+    float* wi_pdf;             /* [1] pdf of the scattered direction                    */
+    float* mis_weight;         /* [1]                                                   */
+    uint8_t* delta;            /* [1] sampled from a delta lobe                         */
+    uint8_t* miss;             /* [1] the path left the scene there                     */
+    float* Le;                 /* [3] emission seen at the vertex (may be NULL: zero)   */
+} dfg_records;
+/* Written by: dfg_record_scatter and dfg_record_radiance (the record
+ * producers) and dfg_nrc_substitute_radiance (L, cur_depth); read by the
+ * row generators and training passes below.  The reference's bsdfPdf and
+ * isTwoSided record fields are not kept: nothing on the training path reads
+ * them (train.cu:6-213). */
+
+/* Shading data of the scatter queue's rows (the ShadingData fields the
+ * records keep, shared.h:36-52), device arrays [n][k]. */
+typedef struct dfg_shading {
+    const float* pos;          /* [3] hit point                                         */
+    const float* normal;       /* [3] shading normal (frame.N)                          */
+    const float* wo;           /* [3] outgoing direction                                */
     const float* roughness;    /* [1]                                                   */
-    float* L;                  /* [3] radiance gathered after the vertex (written only by
-                                  dfg_nrc_substitute_radiance)                          */
-    const float* thp;          /* [3] throughput up to the vertex                       */
-    const float* bsdf_val;     /* [3]                                                   */
     const float* diffuse;      /* [3]                                                   */
     const float* specular;     /* [3]                                                   */
-    const float* wi_pdf;       /* [1] pdf of the scattered direction                    */
-    const float* mis_weight;   /* [1]                                                   */
-    const uint8_t* delta;      /* [1] sampled from a delta lobe                         */
-    const uint8_t* miss;       /* [1] the path left the scene there                     */
-    const float* Le;           /* [3] emission seen at the vertex (may be NULL: zero)   */
-} dfg_records;
+    const uint8_t* smooth;     /* [1] getBsdfType() has a diffuse or glossy lobe       */
+} dfg_shading;
+
+/* ---- Producing the records in a wavefront renderer --------------------------
+ * Training pixels are those with (pixel - pixel_offset) % pixel_stride == 0 in
+ * uint32 arithmetic (isTrainingPixel, guided.h:26-28); train_enabled stands
+ * for the reference's enableTraining || oneStep.  Queue rows whose pixel lies
+ * outside [0, rec->n_pixels) are ignored.  A pixel may appear at most once
+ * per call (one ray per pixel and bounce); calls on one stream apply in order.
+ *
+ * The record step of GuidedPathTracer::handleIntersections
+ * (integrator.cpp:277-296) and handleNRCLastBounce (integrator.cpp:193-201),
+ * after dfg_scatter.  For each queue row i of a training pixel with
+ * depth[i] < max_train_depth (and rr_killed[i] == 0 when given):
+ *   alive[i] != 0 -> incrementDepth (workqueue.h:26-58): the vertex at
+ *     cur_depth gets L = 0, thp = thp[i] (the updated throughput), pos,
+ *     dir = spherical(wi_dir[i]), wo, normal (spherical, math_utils.h:183-190),
+ *     roughness, diffuse, specular, wiPdf = scatter_pdf[i], misWeight =
+ *     mis_aware ? mis[i] : 1, delta[i], miss = 0, Le = 0, bsdfVal =
+ *     product_with_cosine ? bsdf_val |cos| : bsdf_val; cur_depth + 1;
+ *   otherwise (alive NULL: every row) -> incrementDepthLastBounceNRC
+ *     (workqueue.h:60-81): pos, normal, wo, roughness, diffuse, specular,
+ *     delta = !smooth, miss = 0, L = thp = Le = 0; cur_depth + 1.
+ * Nothing is written at cur_depth >= rec->max_depth.  rr_killed and delta may
+ * be NULL (all 0). */
+int dfg_record_scatter(void* stream, int64_t n, dfg_records* rec, const dfg_shading* sd, int train_enabled,
+                       int max_train_depth, uint32_t pixel_offset, uint32_t pixel_stride, int mis_aware,
+                       int product_with_cosine, const int32_t* pixel, const int32_t* depth, const uint8_t* rr_killed,
+                       const uint8_t* alive, const float* wi_dir, const float* thp, const float* scatter_pdf,
+                       const float* mis, const float* bsdf_val, const float* cos_theta, const uint8_t* delta);
+/* Radiance reaching the path (handleEmissiveHit / handleMiss,
+ * integrator.cpp:144-191).  For each row i of a training pixel:
+ * recordRadiance (workqueue.h:83-92) adds contrib[i] (the radiance times the
+ * path throughput) to L of every recorded vertex below min(cur_depth,
+ * max_depth); overwriteLe (workqueue.h:98-106) sets Le of the last vertex to
+ * Le[i] (Le may be NULL: skipped); and where miss[i] != 0 and depth[i] <=
+ * max_train_depth, a placeholder vertex (incrementDepthLastBounceNRC with an
+ * empty ShadingData: miss = 1, zero position and reflectances, roughness 1)
+ * keeps the radiance-cache inputs aligned.  miss may be NULL. */
+int dfg_record_radiance(void* stream, int64_t n, dfg_records* rec, int train_enabled, int max_train_depth,
+                        uint32_t pixel_offset, uint32_t pixel_stride, const int32_t* pixel, const int32_t* depth,
+                        const float* contrib, const float* Le, const uint8_t* miss);
 
 /* DF training rows (generate_training_data_AR, train.cu:6-96).  For the
  * training pixels p = pixel_offset + t * pixel_stride, t < n_train, every
@@ -275,6 +330,25 @@ int dfg_pdf_image(dfg_ctx* ctx, void* stream, const float* feat, int res_theta, 
  * (math_utils.h:226-231); pdf[i] = |p_theta| |p_phi| / (4 pi).  dir or pdf may
  * be NULL. */
 int dfg_directions(void* stream, int64_t n, const float* wi, float* dir, float* pdf);
+/* The decision before the query (GuidedPathTracer::sampleWi,
+ * integrator.cpp:355-401), with the caller's uniforms and BSDF sample:
+ *   rr_killed[i] = u_rr[i] >= rr_prob (Russian roulette; such a row is left
+ *     as it is, guided = mix = 0);
+ *   otherwise thp[i] /= rr_prob (the w.thp /= probRR of handleIntersections,
+ *     integrator.cpp:211-212; thp may be NULL), and with on = guiding_enabled
+ *     && depth[i] < max_guided_depth && smooth[i]:
+ *     guided[i] = on && (bsdf_fraction == 0 || u_mix[i] >= bsdf_fraction);
+ *     mix[i] = !guided[i] && on && bsdf_fraction < 1 (the BSDF-sampled
+ *       direction takes the guide's pdf into the mixture, integrator.cpp:256-261);
+ *     wi[i][2..3] = 1, and for !guided rows whose BSDF sample has a non-zero
+ *       pdf and value, wi[i][0..1] = uniformSphereToSquare(bsdf_wi[i])
+ *       (math_utils.h:233-240), the direction dfg_query then evaluates.
+ * The reference draws u_mix only when bsdf_fraction != 0; here the caller
+ * supplies both uniforms.  bsdf_wi [n][3] world direction, bsdf_f [n][3]. */
+int dfg_sample_wi(void* stream, int64_t n, float rr_prob, float bsdf_fraction, int guiding_enabled,
+                  int max_guided_depth, const int32_t* depth, const uint8_t* smooth, const float* u_rr,
+                  const float* u_mix, const float* bsdf_wi, const float* bsdf_pdf, const float* bsdf_f,
+                  uint8_t* rr_killed, uint8_t* guided, uint8_t* mix, float* wi, float* thp);
 /* The scatter step of GuidedPathTracer::handleIntersections
  * (integrator.cpp:203-299) once the BSDF is evaluated: per row, the guide pdf
  * pg = |p_theta| |p_phi| / (4 pi) from wi [n][4], the BSDF pdf pb, value

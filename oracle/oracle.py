@@ -560,3 +560,149 @@ def scatter_f64(f, wi, guided, mix, bsdf_pdf, bsdf_val, cos_theta, thp):
         mis = (1 - f) * pg / p
     thp = np.where(alive[:, None], upd, thp)
     return thp, p, mis, alive
+
+
+# ----------------------------------------------------------------------------
+# The consumer stage around the query (GuidedPathTracer, integrator.cpp):
+# sampleWi's decision and the record producers of GuidedPixelStateBuffer
+# (workqueue.h:26-92), FP64, one row at a time.
+# ----------------------------------------------------------------------------
+def _normalized(v):
+    """Eigen's normalized(): v / |v|, v itself when |v| = 0."""
+    v = np.asarray(v, np.float64)
+    n = math.sqrt(float(v @ v))
+    return v / n if n > 0 else v
+
+
+def cartesian_to_spherical(v):  # math_utils.h:183-190
+    vn = _normalized(v)
+    th = math.acos(min(max(vn[2], -1.0), 1.0))
+    ph = math.atan2(vn[1], vn[0])
+    return np.array([th, ph + 2 * math.pi if ph < 0 else ph])
+
+
+def uniform_sphere_to_square_one(v):  # math_utils.h:233-240
+    vn = _normalized(v)
+    ph = math.atan2(vn[1], vn[0]) / (2 * math.pi)
+    return np.array([(1.0 - vn[2]) * 0.5, ph + 1.0 if ph < 0 else ph])
+
+
+def sample_wi_f64(rr_prob, f, guiding_enabled, max_guided_depth, depth, smooth, u_rr, u_mix, bsdf_wi, bsdf_pdf,
+                  bsdf_f, wi, thp=None):
+    """sampleWi (integrator.cpp:355-401) and the thp /= probRR of
+    handleIntersections (integrator.cpp:211-212).  Returns (rr_killed, guided,
+    mix, wi', thp')."""
+    n = len(depth)
+    killed, guided, mix = np.zeros(n, np.uint8), np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    wi = np.asarray(wi, np.float64).copy()
+    thp = None if thp is None else np.asarray(thp, np.float64).copy()
+    for i in range(n):
+        if u_rr[i] >= rr_prob:
+            killed[i] = 1
+            continue
+        if thp is not None:
+            thp[i] /= rr_prob
+        on = bool(guiding_enabled) and depth[i] < max_guided_depth and smooth[i] != 0
+        g = on and (f == 0 or u_mix[i] >= f)
+        guided[i] = g
+        mix[i] = (not g) and on and f < 1          # integrator.cpp:256-261
+        wi[i, 2] = wi[i, 3] = 1.0
+        if not g and bsdf_pdf[i] != 0 and np.any(np.asarray(bsdf_f[i]) != 0):
+            wi[i, :2] = uniform_sphere_to_square_one(bsdf_wi[i])
+    return killed, guided, mix, wi, thp
+
+
+def _is_training_pixel(p, offset, stride):  # guided.h:26-28, uint32 arithmetic
+    return ((int(p) - int(offset)) & 0xffffffff) % int(stride) == 0
+
+
+def _last_bounce(rec, p, miss, sd=None, i=None):
+    """incrementDepthLastBounceNRC (workqueue.h:60-81); sd None: an empty
+    ShadingData (zero position, reflectances and directions, roughness 1,
+    diffuse BSDF)."""
+    d = int(rec["cur_depth"][p])
+    if d >= rec["pos"].shape[0]:
+        return
+    rec["cur_depth"][p] = d + 1
+    if sd is None:
+        rec["pos"][d, p] = 0.0
+        rec["normal"][d, p] = cartesian_to_spherical(np.zeros(3))
+        rec["wo"][d, p] = cartesian_to_spherical(np.zeros(3))
+        rec["roughness"][d, p] = 1.0
+        rec["diffuse"][d, p] = rec["specular"][d, p] = 0.0
+        rec["delta"][d, p] = 0
+    else:
+        rec["pos"][d, p] = sd["pos"][i]
+        rec["normal"][d, p] = cartesian_to_spherical(sd["normal"][i])
+        rec["wo"][d, p] = cartesian_to_spherical(sd["wo"][i])
+        rec["roughness"][d, p] = sd["roughness"][i]
+        rec["diffuse"][d, p] = sd["diffuse"][i]
+        rec["specular"][d, p] = sd["specular"][i]
+        rec["delta"][d, p] = 0 if sd["smooth"][i] else 1
+    rec["miss"][d, p] = 1 if miss else 0
+    rec["L"][d, p] = rec["thp"][d, p] = 0.0
+    if "Le" in rec:
+        rec["Le"][d, p] = 0.0
+
+
+def record_scatter_f64(rec, sd, pixel, depth, train_enabled, max_train_depth, offset, stride, mis_aware,
+                       product_with_cosine, rr_killed=None, alive=None, wi_dir=None, thp=None, scatter_pdf=None,
+                       mis=None, bsdf_val=None, cos_theta=None, delta=None):
+    """The record step of handleIntersections (integrator.cpp:277-296):
+    incrementDepth (workqueue.h:26-58) for scattered rows,
+    incrementDepthLastBounceNRC for the others (and handleNRCLastBounce,
+    integrator.cpp:193-201, when alive is None).  Updates rec (numpy arrays
+    [max_depth][n_pixels](k), cur_depth [n_pixels]) in place."""
+    D, N = rec["pos"].shape[0], rec["pos"].shape[1]
+    for i in range(len(pixel)):
+        p = int(pixel[i])
+        if rr_killed is not None and rr_killed[i]:
+            continue
+        if p < 0 or p >= N:
+            continue
+        if not (train_enabled and depth[i] < max_train_depth and _is_training_pixel(p, offset, stride)):
+            continue
+        if alive is None or not alive[i]:
+            _last_bounce(rec, p, False, sd, i)
+            continue
+        d = int(rec["cur_depth"][p])
+        if d >= D:
+            continue
+        rec["L"][d, p] = 0.0
+        rec["thp"][d, p] = thp[i]
+        rec["pos"][d, p] = sd["pos"][i]
+        rec["dir"][d, p] = cartesian_to_spherical(wi_dir[i])
+        rec["wi_pdf"][d, p] = scatter_pdf[i]
+        rec["mis_weight"][d, p] = mis[i] if mis_aware else 1.0
+        rec["delta"][d, p] = 1 if (delta is not None and delta[i]) else 0
+        rec["miss"][d, p] = 0
+        rec["normal"][d, p] = cartesian_to_spherical(sd["normal"][i])
+        rec["roughness"][d, p] = sd["roughness"][i]
+        rec["diffuse"][d, p] = sd["diffuse"][i]
+        rec["specular"][d, p] = sd["specular"][i]
+        b = np.asarray(bsdf_val[i], np.float64)
+        rec["bsdf_val"][d, p] = b * abs(float(cos_theta[i])) if product_with_cosine else b
+        rec["wo"][d, p] = cartesian_to_spherical(sd["wo"][i])
+        if "Le" in rec:
+            rec["Le"][d, p] = 0.0
+        rec["cur_depth"][p] = d + 1
+
+
+def record_radiance_f64(rec, pixel, contrib, train_enabled, max_train_depth, offset, stride, Le=None, miss=None,
+                        depth=None):
+    """handleEmissiveHit / handleMiss (integrator.cpp:144-191): recordRadiance
+    (workqueue.h:83-92), overwriteLe (workqueue.h:98-106) and, for misses at
+    depth <= max_train_depth, the placeholder vertex.  In place."""
+    D, N = rec["pos"].shape[0], rec["pos"].shape[1]
+    for i in range(len(pixel)):
+        p = int(pixel[i])
+        if p < 0 or p >= N or not (train_enabled and _is_training_pixel(p, offset, stride)):
+            continue
+        cd = int(rec["cur_depth"][p])
+        for d in range(min(cd, D)):
+            rec["L"][d, p] = rec["L"][d, p] + contrib[i]
+        last = cd - 1
+        if Le is not None and "Le" in rec and 0 <= last < D:
+            rec["Le"][last, p] = Le[i]
+        if miss is not None and miss[i] and depth[i] <= max_train_depth:
+            _last_bounce(rec, p, True)

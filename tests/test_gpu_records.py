@@ -235,3 +235,135 @@ def test_pdf_image_matches_oracle_and_integrates_to_one(torch_cuda):
     with pytest.raises(G.DfgError):
         f.pdf_image(dev(torch, feat), torch.zeros((0, 4), device="cuda"))
     f.close()
+
+
+# ----------------------------------------------------------------------------
+# The consumer stage: sampleWi's decision and the record producers
+# ----------------------------------------------------------------------------
+def _queue(rng, n, n_pixels):
+    pixel = rng.permutation(n_pixels)[:n].astype(np.int32)
+    pixel[:3] = [-1, n_pixels, n_pixels + 7]     # outside the records: ignored
+    v = lambda *s: rng.normal(0, 1, s).astype(np.float32)
+    sd = {"pos": v(n, 3), "normal": v(n, 3), "wo": v(n, 3), "roughness": rng.random(n).astype(np.float32),
+          "diffuse": rng.random((n, 3)).astype(np.float32), "specular": rng.random((n, 3)).astype(np.float32),
+          "smooth": (rng.random(n) < 0.8).astype(np.uint8)}
+    sd["normal"][5] = 0.0                        # a zero vector: normalized() leaves it
+    return pixel, rng.integers(0, 7, n).astype(np.int32), sd
+
+
+def _rec_host(drec):
+    out = {k: host(v) for k, v in drec.items()}
+    out["cur_depth"] = out["cur_depth"].view(np.uint32).astype(np.int64)
+    return out
+
+
+def _rec_oracle(rec):
+    out = {k: np.array(v, dtype=np.float64 if v.dtype.kind == "f" else v.dtype) for k, v in rec.items()}
+    out["cur_depth"] = rec["cur_depth"].astype(np.int64)
+    return out
+
+
+def _assert_records_match(g, o):
+    for k in o:
+        if k in ("dir", "normal", "wo"):
+            # spherical angles: acos of a normalised fp32 z near +-1 loses
+            # digits (d theta ~ dz / sqrt(1 - z^2)); typical rows agree to 1e-6
+            err = np.abs(g[k] - o[k])
+            assert np.nanmax(err) <= 5e-5 and np.nanmedian(err) <= 1e-6, "record field %s differs" % k
+        elif o[k].dtype.kind == "f":
+            assert np.allclose(g[k], o[k], rtol=2e-6, atol=2e-6, equal_nan=True), "record field %s differs" % k
+        else:
+            assert np.array_equal(g[k].astype(np.int64), o[k].astype(np.int64)), "record field %s differs" % k
+
+
+def test_sample_wi_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(70)
+    n = 5000
+    depth = rng.integers(0, 7, n).astype(np.int32)
+    smooth = (rng.random(n) < 0.8).astype(np.uint8)
+    u_rr, u_mix = rng.random(n).astype(np.float32), rng.random(n).astype(np.float32)
+    bwi = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    bwi[7] = 0.0
+    bpdf = rng.random(n).astype(np.float32)
+    bpdf[rng.random(n) < 0.05] = 0.0
+    bf = rng.random((n, 3)).astype(np.float32)
+    bf[rng.random(n) < 0.05] = 0.0
+    wi = rng.random((n, 4)).astype(np.float32)
+    thp = rng.random((n, 3)).astype(np.float32)
+    for f in (0.0, 0.3, 1.0):
+        d = {k: dev(torch, a) for k, a in (("depth", depth), ("smooth", smooth), ("u_rr", u_rr), ("u_mix", u_mix),
+                                             ("bwi", bwi), ("bpdf", bpdf), ("bf", bf), ("wi", wi), ("thp", thp))}
+        k_, g_, m_ = (torch.full((n,), 9, dtype=torch.uint8, device="cuda") for _ in range(3))
+        G.sample_wi(d["depth"], d["smooth"], d["u_rr"], d["u_mix"], d["bwi"], d["bpdf"], d["bf"], d["wi"], rr_killed=k_,
+                    guided=g_, mix=m_, thp=d["thp"], rr_prob=0.8, bsdf_fraction=f, guiding_enabled=True,
+                    max_guided_depth=4)
+        torch.cuda.synchronize()
+        ko, go, mo, wo, to = O.sample_wi_f64(np.float32(0.8), np.float32(f), True, 4, depth, smooth, u_rr, u_mix, bwi,
+                                             bpdf, bf, wi, thp)
+        assert np.array_equal(host(k_), ko) and np.array_equal(host(g_), go) and np.array_equal(host(m_), mo)
+        assert np.allclose(host(d["wi"]), wo, rtol=0, atol=2e-6)
+        assert np.allclose(host(d["thp"]), to, rtol=1e-6, atol=0)
+        assert 0 < ko.sum() and (go.sum() == 0 if f == 1.0 else 0 < go.sum() < n)
+
+
+@pytest.mark.parametrize("offset,stride", [(0, 1), (3, 4)])
+def test_record_producers_match_oracle(torch_cuda, offset, stride):
+    """dfg_record_scatter (scattered and ended rows, RR-killed rows skipped)
+    then dfg_record_radiance (emission and misses with placeholders) on
+    records already holding a few vertices per pixel: every field against
+    the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(71 + stride)
+    n_pix = 3000
+    rec, _ = random_records(n_pix, max_depth=5, seed=72)
+    rec["cur_depth"] = rng.integers(0, 6, n_pix).astype(np.uint32)
+    rec["Le"] = rng.random((5, n_pix, 3)).astype(np.float32)
+    drec = _dev_records(torch, rec)
+    orec = _rec_oracle(rec)
+    n = 2000
+    pixel, depth, sd = _queue(rng, n, n_pix)
+    alive = (rng.random(n) < 0.8).astype(np.uint8)
+    killed = (rng.random(n) < 0.1).astype(np.uint8)
+    wi_dir = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    thp = rng.random((n, 3)).astype(np.float32)
+    spdf, mis = rng.random(n).astype(np.float32), rng.random(n).astype(np.float32)
+    bval, cos = rng.random((n, 3)).astype(np.float32), rng.uniform(-1, 1, n).astype(np.float32)
+    delta = (rng.random(n) < 0.1).astype(np.uint8)
+    dsd = {k: dev(torch, v) for k, v in sd.items()}
+    G.record_scatter(drec, dsd, dev(torch, pixel), dev(torch, depth), alive=dev(torch, alive), wi_dir=dev(torch, wi_dir),
+                     thp=dev(torch, thp), scatter_pdf=dev(torch, spdf), mis=dev(torch, mis), bsdf_val=dev(torch, bval),
+                     cos_theta=dev(torch, cos), delta=dev(torch, delta), rr_killed=dev(torch, killed),
+                     train_enabled=True, max_train_depth=4, pixel_offset=offset, pixel_stride=stride, mis_aware=True,
+                     product_with_cosine=True)
+    O.record_scatter_f64(orec, sd, pixel, depth, True, 4, offset, stride, True, True, rr_killed=killed, alive=alive,
+                         wi_dir=wi_dir, thp=thp, scatter_pdf=spdf, mis=mis, bsdf_val=bval, cos_theta=cos, delta=delta)
+    torch.cuda.synchronize()
+    _assert_records_match(_rec_host(drec), orec)
+    # the last bounce: every row an ended vertex (alive NULL)
+    G.record_scatter(drec, dsd, dev(torch, pixel), dev(torch, depth), train_enabled=True, max_train_depth=6,
+                     pixel_offset=offset, pixel_stride=stride)
+    O.record_scatter_f64(orec, sd, pixel, depth, True, 6, offset, stride, True, False)
+    torch.cuda.synchronize()
+    _assert_records_match(_rec_host(drec), orec)
+    # radiance of emitter hits and misses
+    pixel2, depth2, _ = _queue(rng, n, n_pix)
+    contrib, Le = rng.random((n, 3)).astype(np.float32), rng.random((n, 3)).astype(np.float32)
+    miss = (rng.random(n) < 0.4).astype(np.uint8)
+    G.record_radiance(drec, dev(torch, pixel2), dev(torch, contrib), Le=dev(torch, Le), miss=dev(torch, miss),
+                      depth=dev(torch, depth2), train_enabled=True, max_train_depth=3, pixel_offset=offset,
+                      pixel_stride=stride)
+    O.record_radiance_f64(orec, pixel2, contrib, True, 3, offset, stride, Le=Le, miss=miss, depth=depth2)
+    torch.cuda.synchronize()
+    g = _rec_host(drec)
+    _assert_records_match(g, orec)
+    assert (g["cur_depth"] != rec["cur_depth"]).sum() > 100
+    # training off: nothing changes
+    before = {k: v.clone() for k, v in drec.items()}
+    G.record_radiance(drec, dev(torch, pixel2), dev(torch, contrib), train_enabled=False)
+    torch.cuda.synchronize()
+    def same(a, b):   # the records hold NaN radiance on purpose
+        if a.is_floating_point():
+            return torch.equal(a.isnan(), b.isnan()) and torch.equal(a.nan_to_num(0.0), b.nan_to_num(0.0))
+        return torch.equal(a, b)
+    assert all(same(before[k], drec[k]) for k in drec)

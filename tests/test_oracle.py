@@ -871,3 +871,158 @@ def test_ema_is_the_debiased_exponential_average():
                                          np.zeros(n), np.zeros(n, np.uint32), w_lowp=ws[t - 1], ema=ema)
         expect = sum(d ** (t - s) * (1 - d) * ws[s - 1] for s in range(1, t + 1)) / (1 - d ** t)
         assert np.allclose(ema, expect, rtol=1e-12, atol=1e-12), t
+
+
+# ----------------------------------------------------------------------------
+# The consumer stage: sampleWi's decision and the record producers
+# ----------------------------------------------------------------------------
+def test_sphere_maps_closed_forms():
+    """cartesianToSpherical and uniformSphereToSquare at the axes (closed
+    forms), scale invariance, the zero vector (Eigen's normalized() leaves it)."""
+    c2s, s2q = O.cartesian_to_spherical, O.uniform_sphere_to_square_one
+    assert np.allclose(c2s([0, 0, 1]), [0, 0]) and np.allclose(c2s([0, 0, -2]), [math.pi, 0])
+    assert np.allclose(c2s([1, 0, 0]), [math.pi / 2, 0]) and np.allclose(c2s([0, 1, 0]), [math.pi / 2, math.pi / 2])
+    assert np.allclose(c2s([0, -3, 0]), [math.pi / 2, 1.5 * math.pi]) and np.allclose(c2s([-1, 0, 0]), [math.pi / 2, math.pi])
+    assert np.allclose(c2s([0, 0, 0]), [math.pi / 2, 0])
+    assert np.allclose(s2q([0, 0, 1]), [0, 0]) and np.allclose(s2q([0, 0, -1]), [1, 0])
+    assert np.allclose(s2q([0, 1, 0]), [0.5, 0.25]) and np.allclose(s2q([0, -5, 0]), [0.5, 0.75])
+    rng = np.random.default_rng(60)
+    for v in rng.normal(0, 1, (50, 3)):
+        sq = s2q(v)
+        assert np.allclose(O.square_to_uniform_sphere(sq[None])[0], v / np.linalg.norm(v))   # inverse maps
+        th, ph = c2s(v)
+        assert np.allclose(O.spherical_to_cartesian(th, ph), v / np.linalg.norm(v))
+
+
+def test_sample_wi_decisions_hand_worked():
+    """sampleWi (integrator.cpp:355-401), row by row with rr 0.8, f 0.3,
+    max guided depth 3:
+      0: u_rr 0.9 >= 0.8 -> killed, wi and thp untouched
+      1: smooth, depth 0, u_mix 0.5 >= 0.3 -> guided; thp / 0.8; wi[2:] = 1
+      2: smooth, u_mix 0.1 < 0.3 -> BSDF sampled, mix; wi[:2] = square(+y)
+      3: not smooth -> BSDF, no mix; its sample has f = 0: wi[:2] kept
+      4: depth 3 (guiding off there) -> BSDF, no mix; square(-z) = (1, 0)
+      5: smooth, u_mix 0.2 but bsdf pdf 0 -> BSDF, mix, wi[:2] kept"""
+    depth = np.array([0, 0, 1, 0, 3, 2])
+    smooth = np.array([1, 1, 1, 0, 1, 1], np.uint8)
+    u_rr = np.array([0.9, 0.1, 0.5, 0.79, 0.0, 0.3])
+    u_mix = np.array([0.5, 0.5, 0.1, 0.9, 0.9, 0.2])
+    bwi = np.array([[1, 0, 0], [1, 0, 0], [0, 2, 0], [0, 0, 1], [0, 0, -1], [1, 1, 1]], np.float64)
+    bpdf = np.array([1, 1, 0.5, 2.0, 0.3, 0.0])
+    bf = np.array([[1, 1, 1], [1, 1, 1], [0.2, 0, 0], [0, 0, 0], [0.5, 0.5, 0.5], [1, 1, 1]], np.float64)
+    wi = np.full((6, 4), 0.125)
+    thp = np.ones((6, 3))
+    k, g, m, w, t = O.sample_wi_f64(0.8, 0.3, True, 3, depth, smooth, u_rr, u_mix, bwi, bpdf, bf, wi, thp)
+    assert list(k) == [1, 0, 0, 0, 0, 0] and list(g) == [0, 1, 0, 0, 0, 0] and list(m) == [0, 0, 1, 0, 0, 1]
+    assert np.array_equal(w[0], wi[0]) and np.array_equal(t[0], thp[0])
+    assert np.allclose(t[1:], 1.25)
+    assert np.allclose(w[1:, 2:], 1.0)
+    assert np.allclose(w[1, :2], 0.125) and np.allclose(w[2, :2], [0.5, 0.25])
+    assert np.allclose(w[3, :2], 0.125) and np.allclose(w[4, :2], [1.0, 0.0]) and np.allclose(w[5, :2], 0.125)
+    # f = 0: always guided where guiding is on; f = 1: never, and no mixture
+    _, g0, m0, _, _ = O.sample_wi_f64(0.8, 0.0, True, 3, depth, smooth, u_rr, u_mix, bwi, bpdf, bf, wi)
+    _, g1, m1, _, _ = O.sample_wi_f64(0.8, 1.0, True, 3, depth, smooth, u_rr, u_mix, bwi, bpdf, bf, wi)
+    assert list(g0) == [0, 1, 1, 0, 0, 1] and list(m0) == [0] * 6
+    assert list(g1) == [0] * 6 and list(m1) == [0] * 6
+
+
+def _empty_records(D, N):
+    return {"cur_depth": np.zeros(N, np.int64), "pos": np.zeros((D, N, 3)), "dir": np.zeros((D, N, 2)),
+            "wo": np.zeros((D, N, 2)), "normal": np.zeros((D, N, 2)), "roughness": np.zeros((D, N)),
+            "L": np.zeros((D, N, 3)), "thp": np.zeros((D, N, 3)), "bsdf_val": np.zeros((D, N, 3)),
+            "diffuse": np.zeros((D, N, 3)), "specular": np.zeros((D, N, 3)), "wi_pdf": np.zeros((D, N)),
+            "mis_weight": np.zeros((D, N)), "delta": np.zeros((D, N), np.uint8), "miss": np.zeros((D, N), np.uint8),
+            "Le": np.zeros((D, N, 3))}
+
+
+def test_record_producers_hand_worked():
+    """incrementDepth / incrementDepthLastBounceNRC / recordRadiance /
+    overwriteLe (workqueue.h:26-106) on a 2-deep buffer of 4 pixels, training
+    pixels 1 and 3 (offset 1, stride 2)."""
+    rec = _empty_records(2, 4)
+    sd = {"pos": np.array([[1, 2, 3], [4, 5, 6], [7, 8, 9]], np.float64),
+          "normal": np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]], np.float64),
+          "wo": np.array([[0, 1, 0], [0, 0, -1], [-1, 0, 0]], np.float64),
+          "roughness": np.array([0.1, 0.2, 0.3]), "diffuse": np.full((3, 3), 0.4), "specular": np.full((3, 3), 0.6),
+          "smooth": np.array([1, 0, 1], np.uint8)}
+    pixel, depth = np.array([1, 3, 2]), np.array([0, 0, 0])
+    alive = np.array([1, 0, 1], np.uint8)
+    wi_dir = np.array([[0, 0, 3], [1, 1, 1], [1, 0, 0]], np.float64)
+    thp = np.array([[0.5, 0.25, 1.0], [9, 9, 9], [9, 9, 9]])
+    O.record_scatter_f64(rec, sd, pixel, depth, True, 5, 1, 2, True, True, alive=alive, wi_dir=wi_dir, thp=thp,
+                         scatter_pdf=np.array([0.7, 0, 0.1]), mis=np.array([0.6, 0, 0]),
+                         bsdf_val=np.array([[0.2, 0.4, 0.8], [0, 0, 0], [1, 1, 1]]), cos_theta=np.array([-0.5, 0, 1]),
+                         delta=np.array([1, 0, 0], np.uint8))
+    # pixel 1 scattered: every field of vertex 0
+    assert list(rec["cur_depth"]) == [0, 1, 0, 1]
+    assert np.allclose(rec["thp"][0, 1], [0.5, 0.25, 1.0]) and np.allclose(rec["pos"][0, 1], [1, 2, 3])
+    assert np.allclose(rec["dir"][0, 1], [0, 0]) and np.allclose(rec["normal"][0, 1], [0, 0])
+    assert np.allclose(rec["wo"][0, 1], [math.pi / 2, math.pi / 2]) and rec["wi_pdf"][0, 1] == 0.7
+    assert rec["mis_weight"][0, 1] == 0.6 and rec["delta"][0, 1] == 1 and rec["miss"][0, 1] == 0
+    assert np.allclose(rec["bsdf_val"][0, 1], [0.1, 0.2, 0.4])     # product with |cos| = 0.5
+    assert rec["roughness"][0, 1] == 0.1 and np.allclose(rec["diffuse"][0, 1], 0.4)
+    # pixel 3 ended (not alive): the placeholder from its shading data; not smooth -> delta
+    assert np.allclose(rec["pos"][0, 3], [4, 5, 6]) and rec["delta"][0, 3] == 1 and rec["miss"][0, 3] == 0
+    assert np.allclose(rec["normal"][0, 3], [math.pi / 2, 0]) and np.allclose(rec["wo"][0, 3], [math.pi, 0])
+    assert np.allclose(rec["thp"][0, 3], 0) and rec["wi_pdf"][0, 3] == 0     # dir / pdf / mis left as they were
+    # pixel 2 is not a training pixel: untouched
+    assert rec["cur_depth"][2] == 0 and np.allclose(rec["pos"][:, 2], 0)
+    # mis_aware off -> misWeight 1; a third vertex does not fit the 2-deep buffer
+    O.record_scatter_f64(rec, sd, np.array([1]), np.array([1]), True, 5, 1, 2, False, False, alive=np.array([1]),
+                         wi_dir=wi_dir[2:], thp=np.array([[2.0, 2.0, 2.0]]), scatter_pdf=np.array([0.9]),
+                         mis=np.array([0.3]), bsdf_val=np.array([[1.0, 0.5, 0.25]]), cos_theta=np.array([0.5]))
+    assert rec["cur_depth"][1] == 2 and rec["mis_weight"][1, 1] == 1.0 and np.allclose(rec["bsdf_val"][1, 1], [1, 0.5, 0.25])
+    assert np.allclose(rec["dir"][1, 1], [math.pi / 2, 0])   # +x
+    before = {k: v.copy() for k, v in rec.items()}
+    O.record_scatter_f64(rec, sd, np.array([1]), np.array([1]), True, 5, 1, 2, True, False, alive=np.array([1]),
+                         wi_dir=wi_dir[:1], thp=thp[:1], scatter_pdf=np.array([1.0]), mis=np.array([1.0]),
+                         bsdf_val=np.array([[1.0, 1, 1]]), cos_theta=np.array([1.0]))
+    assert all(np.array_equal(before[k], rec[k]) for k in rec)
+    # depth at max_train_depth: no record
+    O.record_scatter_f64(rec, sd, np.array([3]), np.array([5]), True, 5, 1, 2, True, False)
+    assert rec["cur_depth"][3] == 1
+    # radiance: added to every vertex below cur_depth, Le onto the last one
+    O.record_radiance_f64(rec, np.array([1, 3, 2]), np.array([[1.0, 2, 3], [0.5, 0.5, 0.5], [7, 7, 7]]), True, 5, 1, 2,
+                          Le=np.array([[4.0, 4, 4], [8, 8, 8], [1, 1, 1]]))
+    assert np.allclose(rec["L"][0, 1], [1, 2, 3]) and np.allclose(rec["L"][1, 1], [1, 2, 3])
+    assert np.allclose(rec["Le"][1, 1], 4) and np.allclose(rec["Le"][0, 1], 0)
+    assert np.allclose(rec["L"][0, 3], 0.5) and np.allclose(rec["Le"][0, 3], 8) and np.allclose(rec["L"][:, 2], 0)
+    # a miss at depth <= max_train_depth adds the placeholder (empty ShadingData)
+    O.record_radiance_f64(rec, np.array([3]), np.array([[0.0, 0, 0]]), True, 1, 1, 2, miss=np.array([1]),
+                          depth=np.array([1]))
+    assert rec["cur_depth"][3] == 2 and rec["miss"][1, 3] == 1 and rec["delta"][1, 3] == 0
+    assert rec["roughness"][1, 3] == 1.0 and np.allclose(rec["pos"][1, 3], 0)
+    O.record_radiance_f64(rec, np.array([3]), np.array([[0.0, 0, 0]]), True, 1, 1, 2, miss=np.array([1]),
+                          depth=np.array([2]))
+    assert rec["cur_depth"][3] == 2
+
+
+def test_recorded_path_yields_arriving_radiance():
+    """A simulated path: vertex k scatters with throughput T_k (product of
+    the per-bounce factors), then an emitter of radiance E is hit.  The
+    producers store L_k = E T_path at every vertex, so the DF training target
+    L_k / T_k * mis * bsdfVal (train.cu:6-96) is the radiance arriving at
+    vertex k, E prod_{j>k} factor_j, times bsdfVal and mis."""
+    D, N = 4, 1
+    rec = _empty_records(D, N)
+    rng = np.random.default_rng(61)
+    factors = rng.uniform(0.3, 0.9, (3, 3))
+    T = np.ones(3)
+    sd = {"pos": rng.uniform(-1, 1, (1, 3)), "normal": np.array([[0.0, 0, 1]]), "wo": np.array([[0.0, 1, 1]]),
+          "roughness": np.array([0.5]), "diffuse": np.full((1, 3), 0.5), "specular": np.zeros((1, 3)),
+          "smooth": np.array([1], np.uint8)}
+    for k in range(3):
+        T = T * factors[k]
+        O.record_scatter_f64(rec, sd, np.array([0]), np.array([k]), True, 5, 0, 1, True, False, alive=np.array([1]),
+                             wi_dir=rng.normal(0, 1, (1, 3)), thp=T[None], scatter_pdf=np.array([0.4]),
+                             mis=np.array([0.75]), bsdf_val=np.array([[0.3, 0.3, 0.3]]), cos_theta=np.array([0.5]))
+    E = np.array([2.0, 1.0, 0.5])
+    O.record_radiance_f64(rec, np.array([0]), (E * T)[None], True, 5, 0, 1, Le=E[None])
+    rec["cur_depth"] = rec["cur_depth"].astype(np.uint32)
+    aabb = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    f, t = O.training_targets_f64(rec, 1, 0, 1, aabb)
+    assert f.shape == (3, 9)
+    for k in range(3):
+        arriving = E * np.prod(factors[k + 1:], axis=0)
+        assert np.allclose(t[k, 2:5], arriving * 0.75 * 0.3, rtol=1e-12), k
+        assert t[k, 5] == 0.4

@@ -885,3 +885,75 @@ def test_full_size_training_properties(torch_cuda):
         assert r <= 1e-5, "net %d grid grad bucketed vs atomic rel err %.3g" % (k, r)
     for f in fs + [fb, fa]:
         f.close()
+
+
+def test_initialisation_bounds_statistics_and_seed(torch_cuda):
+    """Trainer::initialize_params (trainer.h:71-93): every MLP matrix Xavier-
+    uniform over its padded shape (gpu_matrix.h:291-306), U(-b, b) with
+    b = sqrt(6 / (fan_in + fan_out)): inside the bound, mean ~0, variance
+    ~b^2/3, reaching close to the bound; grid entries U[-1e-4, 1e-4]
+    (grid.h:1364-1369).  The same seed gives the same weights; another seed
+    and the other net give others.  (tcnn's exact pcg32 stream is not
+    reproduced: weights are fed to the oracle instead.)"""
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=14)
+    f1, f2 = G.GuidingField(cfg), G.GuidingField(cfg)
+    cfg3 = G.hash_config(with_nrc=True, log2_hashmap_size=14, seed=1234)
+    f3 = G.GuidingField(cfg3)
+    layouts = O.config_layouts(cfg)
+    for net, L in enumerate(layouts):
+        p = f1.get_params(net, G.DFG_PARAMS_MASTER).astype(np.float64)
+        assert np.array_equal(p, f2.get_params(net, G.DFG_PARAMS_MASTER))
+        assert not np.array_equal(p, f3.get_params(net, G.DFG_PARAMS_MASTER))
+        offs = list(L["layer_off"]) + [L["n_matrix"]]
+        for l in range(len(offs) - 1):
+            w = p[offs[l]:offs[l + 1]]
+            if w.size == 0:
+                continue
+            fan_in = L["enc_width"] if l == 0 else 64
+            fan_out = w.size // fan_in
+            b = math.sqrt(6.0 / (fan_in + fan_out))
+            assert np.abs(w).max() <= b * (1 + 1e-6)
+            assert np.abs(w).max() >= b * 0.98
+            assert abs(w.mean()) <= 4 * b / math.sqrt(3 * w.size)
+            assert abs(w.var() / (b * b / 3) - 1) <= 0.1
+        if L["grid"]:
+            g = p[L["n_matrix"]:]
+            assert np.abs(g).max() <= 1e-4 * (1 + 1e-6) and np.abs(g).max() >= 0.99e-4
+            assert abs(g.var() / (1e-8 / 3) - 1) <= 0.01
+    a, b_ = f1.get_params(0, G.DFG_PARAMS_MASTER), f1.get_params(1, G.DFG_PARAMS_MASTER)
+    assert not np.array_equal(a[:1000], b_[:1000])
+    for f in (f1, f2, f3):
+        f.close()
+
+
+@pytest.mark.parametrize("weights", ["seeded", "trained_scale"])
+def test_query_with_theta_phi_grids_of_different_structure(torch_cuda, weights):
+    """theta and phi grids that differ (phi: 12 levels, T = 2^17) take the
+    per-net pre-pass and per-net tables instead of the interleaved pair:
+    query parity against FP64 on both kinds of weights, at a size that runs
+    the pre-pass (40,000 rows) and one that does not (2,000)."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False)
+    ph = cfg["nn"]["phi_encoding"]["nested"][0] = dict(cfg["nn"]["phi_encoding"]["nested"][0])
+    ph["n_levels"], ph["log2_hashmap_size"] = 12, 17
+    f = G.GuidingField(cfg)
+    th, phl, _ = O.config_layouts(cfg)
+    assert th["grid"]["levels"] == 16 and phl["grid"]["levels"] == 12
+    if weights == "trained_scale":
+        rng = np.random.default_rng(31)
+        f.set_params(0, trained_scale_params(th, rng))
+        f.set_params(1, trained_scale_params(phl, rng))
+    pt, pp = f.get_params(0, G.DFG_PARAMS_INFERENCE), f.get_params(1, G.DFG_PARAMS_INFERENCE)
+    for n in (40000, 2000):
+        feat, u = df_features(n, 32), uniforms(n, 33)
+        d_wi = dev(torch, wi_init(n))
+        f.query(dev(torch, feat), dev(torch, u), d_wi)
+        torch.cuda.synchronize()
+        g = host(d_wi).astype(np.float64)
+        k = min(2048, n)
+        at = wi_init(k)
+        at[:, :2] = g[:k, :2]
+        o = O.query_f64(th, phl, pt, pp, feat[:k], 9, u[:k], at, 1, guided=np.zeros(k, np.uint8))
+        rel = _rel(g[:k, 2] * g[:k, 3], o[:, 2].astype(np.float64) * o[:, 3])
+        assert np.median(rel) <= 5e-6 and np.percentile(rel, 99) <= 5e-4, (n, np.median(rel))
+    f.close()

@@ -392,6 +392,7 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         if (c->query_chunks < 1 || c->query_chunks > 64) throw Error(DFG_ERR_CONFIG, "query_chunks must be in 1..64");
         if (c->prepass_blocks_per_cu < 0 || c->prepass_blocks_per_cu > 64)
             throw Error(DFG_ERR_CONFIG, "prepass_blocks_per_cu must be in 0..64");
+        c->fused_training = d.number_or("fused_training", 1.0) != 0.0;
         c->wgrad_chunk_rows = (int)d.number_or("wgrad_chunk_rows", 1024);
         if (c->wgrad_chunk_rows < 64 || c->wgrad_chunk_rows > 65536 || c->wgrad_chunk_rows % 64)
             throw Error(DFG_ERR_CONFIG, "wgrad_chunk_rows must be a multiple of 64 in 64..65536");
@@ -1233,8 +1234,38 @@ int dfg_nrc_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat,
     const int chunk = ctx->wgrad_chunk_rows;
     const int n_chunks = (int)((ld + chunk - 1) / chunk);
     NetState& s = ctx->net[2];
-    ensure_train_ws(s, ld, n_chunks);
     ctx->loss.ensure((size_t)ld);
+    if (ctx->fused_training) {
+        // forward, loss, backward and weight gradients in one pass over
+        // 64-row tiles kept in LDS, two workgroups per CU
+        const int64_t n_tiles = ld / 64;
+        const int blocks = (int)std::min<int64_t>(n_tiles, 2 * ctx->n_cus);
+        s.partial.ensure((size_t)blocks * s.spec.n_matrix_params);
+        TrainFusedArgs f;
+        std::memset(&f, 0, sizeof(f));
+        f.ns = s.d_spec.p;
+        f.pack = s.pack_train.p;
+        f.tab = s.tab_train();
+        f.feat = feat;
+        f.stride = feat_stride;
+        f.target = target_rgb;
+        f.loss = ctx->loss.p;
+        f.partial = s.partial.p;
+        f.n = B;
+        f.n_tiles = n_tiles;
+        f.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE_NRC, parameters.h:59
+        if (launch_train_fused_nrc(f, s.spec, blocks, st)) {
+            launch_wgrad_reduce(s.partial.p, blocks, (int64_t)s.spec.n_matrix_params, s.grad.p, st);
+            net_update(ctx, s, 1024.0f, st);
+            if (loss_out) {
+                ctx->mean_ws.ensure(256);
+                launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+            }
+            check_launch();
+            return DFG_OK;
+        }
+    }
+    ensure_train_ws(s, ld, n_chunks);
     net_forward_train(ctx, s, feat, feat_stride, nullptr, 1, B, ld, st);
     NrcLossArgs l;
     std::memset(&l, 0, sizeof(l));

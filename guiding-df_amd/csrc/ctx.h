@@ -102,6 +102,7 @@ struct dfg_ctx {
     int prepass_blocks_per_cu = 0;     // persistent pre-pass grid per CU in chunked mode (0: one block per item)
     int64_t pipeline_min_rows = 262144;
     int wgrad_chunk_rows = 1024;       // batch rows per weight-gradient partial sum (multiple of 64)
+    bool fused_training = true;        // one-kernel forward/loss/backward/weight-gradient pass where available
     int n_cus = 256;
     int device = 0;                    // the device current at dfg_create (weights and workspace live there)
     void* noalloc_scratch = nullptr;   // that device's address of the allocation-free query's scratch array

@@ -907,28 +907,38 @@ __device__ __forceinline__ void tile_store(const _Float16* tile, _Float16* __res
 }
 
 
-// fp32 -> (hi, lo) fp16 pair with hi + lo == v to ~2^-21 relative: the MFMA
-// then sees the activation at fp32-level precision through two products.  hi
-// keeps the sign, exponent and top 10 mantissa bits of v (a value fp16 holds
-// exactly over its normal range), so v - hi is exact in fp32 and only lo is
-// rounded; below the fp16 normal range the pair is exact to ~2^-25 absolute.
-__device__ __forceinline__ void split_h(float v, _Float16& hi, _Float16& lo) {
-    const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & 0xffffe000u);
-    hi = (_Float16)h;
-    lo = (_Float16)(v - h);
+// fp32 -> (hi, lo) fp16 pairs for two values at once, hi + lo == v to
+// ~2^-23 relative (2^-25 absolute below the fp16 normal range): the MFMA then
+// sees the activation at fp32-level precision through two products.
+// hi = fp16(v) rounded to nearest (v_cvt_pk_f16_f32); v - hi is exact in
+// fp32 (hi is within half an fp16 ulp of v), and v_fma_mix{lo,hi}_f16
+// computes v * 1 + (-hi) with an fp16 operand and rounds that exact
+// remainder to fp16 in one instruction, into the low or high half of `lo`
+// (the mix instructions leave the other half alone; the second fills it).
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, h2{(_Float16)a, (_Float16)b});
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(lo) : "v"(a), "v"(hi));
+    asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(lo) : "v"(b), "v"(hi));
+}
+
+// Eight values -> their (hi, lo) operands.
+__device__ __forceinline__ void split8(const float (&v)[8], h8& hi, h8& lo) {
+    uint32_t h[4], l[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) split_pair(v[2 * k], v[2 * k + 1], h[k], l[k]);
+    hi = __builtin_bit_cast(h8, h);
+    lo = __builtin_bit_cast(h8, l);
 }
 
 // relu(C) of 4 M-tiles -> (hi, lo) operands of the next layer.
 __device__ __forceinline__ void relu_to_act2(const f4 (&c)[4], h8 (&hi)[2], h8 (&lo)[2]) {
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
+        float v[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            _Float16 h, l;
-            split_h(relu(c[2 * s + (j >> 2)][j & 3]), h, l);
-            hi[s][j] = h;
-            lo[s][j] = l;
-        }
+        for (int j = 0; j < 8; ++j) v[j] = relu(c[2 * s + (j >> 2)][j & 3]);
+        split8(v, hi[s], lo[s]);
+    }
 }
 
 // One layer's activations (lane: rows q*16 + i16, neurons 16*(2s + j/4) +
@@ -1015,16 +1025,11 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                         v[0] = v0.x; v[1] = v0.y; v[2] = v0.z; v[3] = v0.w;
                         v[4] = v1.x; v[5] = v1.y; v[6] = v1.z; v[7] = v1.w;
                     }
+                    if constexpr (SPLIT_IN) {
+                        split8(v, b[q], bl[q]);
+                    } else {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (SPLIT_IN) {
-                            _Float16 h, l;
-                            split_h(v[j], h, l);
-                            b[q][j] = h;
-                            bl[q][j] = l;
-                        } else {
-                            b[q][j] = (_Float16)v[j];
-                        }
+                        for (int j = 0; j < 8; ++j) b[q][j] = (_Float16)v[j];
                     }
                 }
             }

@@ -157,6 +157,24 @@ struct WgradArgs {
 };
 void launch_wgrad(const WgradArgs& a, const NetSpec& hs, hipStream_t s);
 
+// Fused training pass of one net (forward, loss, backward, weight gradients
+// per 64-row tile in LDS); partial weight gradients per workgroup.
+struct TrainFusedArgs {
+    const NetSpec* ns;
+    const _Float16* pack;     // training packs (forward at fwd_off, backward at bwd_off)
+    const _Float16* tab;      // training grid table or NULL
+    const float* feat;
+    int stride;
+    const float* target;      // radiance cache: target RGB [B][3]
+    float* loss;              // [B]
+    float* partial;           // [workgroups][n_matrix_params]
+    int64_t n, n_tiles;
+    float loss_scale;
+};
+size_t train_fused_lds_bytes(const NetSpec& hs);
+bool launch_train_fused_nrc(const TrainFusedArgs& a, const NetSpec& hs, int blocks, hipStream_t s);
+void launch_wgrad_reduce(const float* partial, int n_chunks, int64_t n_matrix, float* grad, hipStream_t s);
+
 struct DfLossArgs {
     const float* target;    // [B][9]
     const _Float16* bins_th;  // [ld][nt_pad]

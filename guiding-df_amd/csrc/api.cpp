@@ -1174,9 +1174,76 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     const int n_chunks = (int)((ld + chunk - 1) / chunk);
     NetState& th = ctx->net[0];
     NetState& ph = ctx->net[1];
+    ctx->loss.ensure((size_t)ld);
+    if (ctx->fused_training && ctx->pair_ok && train_fused_df_ok(th.spec, ctx->n_theta) &&
+        train_fused_df_ok(ph.spec, ctx->n_phi)) {
+        // per net one pass over 64-row tiles kept in LDS (forward, loss,
+        // backward, weight gradients); the loss couples the nets through each
+        // row's |pdf|: phi's forward alone first, then theta's and phi's passes
+        const int64_t n_tiles = ld / 64;
+        const int blocks = (int)std::min<int64_t>(n_tiles, 2 * ctx->n_cus);
+        const size_t cols = (size_t)th.spec.grid.n_levels * th.spec.grid.n_feat;
+        th.dx.ensure(cols * ld);
+        ph.dx.ensure(cols * ld);
+        th.partial.ensure((size_t)blocks * th.spec.n_matrix_params);
+        ph.partial.ensure((size_t)blocks * ph.spec.n_matrix_params);
+        ctx->train_pdf.ensure((size_t)2 * ld);
+        float* pdf_th = ctx->train_pdf.p;
+        float* pdf_ph = ctx->train_pdf.p + ld;
+        train_grid_pair(ctx, feat, feat_stride, B, ld, st);
+        auto args = [&](NetState& s, int theta_in) {
+            TrainFusedArgs f;
+            std::memset(&f, 0, sizeof(f));
+            f.ns = s.d_spec.p;
+            f.pack = s.pack_train.p;
+            f.feat = feat;
+            f.stride = feat_stride;
+            f.target = target;
+            f.loss = ctx->loss.p;
+            f.partial = s.partial.p;
+            f.n = B;
+            f.n_tiles = n_tiles;
+            f.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE, parameters.h:45
+            f.gfeat = s.gfeat.p;
+            f.ld = ld;
+            f.theta_in = theta_in;
+            f.theta_bins = ctx->n_theta;
+            f.linear = ctx->linear;
+            const int gp = grid_part_index(s.spec);
+            f.dx = s.dx.p;
+            f.dx_begin = s.spec.parts[gp].out_begin;
+            f.dx_n = (int)cols;
+            return f;
+        };
+        TrainFusedArgs fp = args(ph, 1);
+        fp.pdf_out = pdf_ph;
+        TrainFusedArgs ft = args(th, 0);
+        ft.pdf_in = pdf_ph;
+        ft.pdf_out = pdf_th;
+        TrainFusedArgs fq = args(ph, 1);
+        fq.pdf_in = pdf_th;
+        // the forward-only pass holds no activation slots: three workgroups per CU
+        const int blocks_pdf = (int)std::min<int64_t>(n_tiles, 3 * ctx->n_cus);
+        if (!launch_train_fused_df(fp, ph.spec, ctx->n_phi, true, blocks_pdf, st) ||
+            !launch_train_fused_df(ft, th.spec, ctx->n_theta, false, blocks, st) ||
+            !launch_train_fused_df(fq, ph.spec, ctx->n_phi, false, blocks, st))
+            throw Error(DFG_ERR_HIP, "fused DF training launch failed");
+        launch_wgrad_reduce(th.partial.p, blocks, (int64_t)th.spec.n_matrix_params, th.grad.p, st);
+        launch_wgrad_reduce(ph.partial.p, blocks, (int64_t)ph.spec.n_matrix_params, ph.grad.p, st);
+        if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
+            net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
+            net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+        }
+        nets_update_pair(ctx, 1024.0f, st);
+        if (loss_out) {
+            ctx->mean_ws.ensure(256);
+            launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
+        }
+        check_launch();
+        return DFG_OK;
+    }
     ensure_train_ws(th, ld, n_chunks);
     ensure_train_ws(ph, ld, n_chunks);
-    ctx->loss.ensure((size_t)ld);
     const bool paired = train_grid_pair(ctx, feat, feat_stride, B, ld, st);
     net_forward_train(ctx, th, feat, feat_stride, nullptr, 0, B, ld, st, paired);
     net_forward_train(ctx, ph, feat, feat_stride, target, 0, B, ld, st, paired);

@@ -513,11 +513,14 @@ template <typename In, typename Out>
 // reach the MLP straight from the pre-pass buffer).
 // pad_k: also write zeros from enc_width up to the K-step boundary (the MLP
 // loader treats those columns as zero either way).
+// part_first, part_step: encode only the nested parts part_first,
+// part_first + part_step, ... (several lanes sharing one row's encoding).
 __device__ __forceinline__ void encode_row(const NetSpec& ns, const _Float16* __restrict__ table, In in, Out out,
                                            const float* __restrict__ gfeat = nullptr, int64_t gstride = 0,
-                                           bool skip_grid = false, bool pad_k = true) {
+                                           bool skip_grid = false, bool pad_k = true, int part_first = 0,
+                                           int part_step = 1) {
 #pragma clang fp contract(off)
-    for (int pi = 0; pi < ns.n_parts; ++pi) {
+    for (int pi = part_first; pi < ns.n_parts; pi += part_step) {
         const EncPart& P = ns.parts[pi];
         const int ob = P.out_begin + (P.pad_front ? P.n_pad : 0);
         const int pb = P.pad_front ? P.out_begin : P.out_begin + P.n_out;
@@ -780,36 +783,70 @@ __device__ __forceinline__ float theta_input(float theta, int n_theta, int linea
 // dL/dz_j for one net's logits (compute_gradients_specialized_linear_interpolation
 // / _nearest, AR.h:50-104): multiplier * d pdf(x) / d z_j through the softmax
 // Jacobian, where P(k) is the softmax probability of bin k (bins[k] / N).
-template <int N, typename Prob>
-__device__ __forceinline__ void df_logit_grad(float x, float multiplier, int linear, Prob P, float (&g)[N]) {
+// Per-row terms of combine_AR_pdf_and_dl_doutput (train.h:948-961) from the
+// record t ({theta, phi, Li rgb, wiPdf}) and the nets' |pdf| at the sample:
+// the guided pdf pdf() * pdfTransformInv() + eps as one fused multiply-add
+// (the contraction CUDA applies to that expression), the gradient prefix and
+// the row's loss.  ls_over_n: the loss scale over the batch size.
+struct DfRowTerms {
+    float prefix, loss;
+};
+__device__ __forceinline__ DfRowTerms df_row_terms(const float* t, float apt, float app, float ls_over_n) {
 #pragma clang fp contract(off)
-    if (linear) {
-        const float idx = x * (float)N - 0.5f;
-        int i0 = min((int)floorf(idx), N - 1);
-        int i1 = i0 + 1;
-        float w = idx - (float)i0;
-        if (i0 < 0) { i0 = N - 1; w = 1.0f; }
-        if (i1 > N - 1) { i1 = 0; w = 0.0f; }
-        const float p0 = P(i0);
-        const float p1 = P(i1);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float pi = P(i);
+    const float Li = (t[2] + t[3] + t[4]) / 3.0f;
+    const float wiPdf = t[5] + DFG_EPS;
+    const float guided = __builtin_fmaf((apt * app) * DFG_INV_4PI, 1.0f / DFG_INV_4PI, DFG_EPS);
+    DfRowTerms r;
+    r.prefix = -Li / wiPdf / guided * ls_over_n;
+    r.loss = -Li / wiPdf * det_logf(guided);
+    return r;
+}
+
+// The same gradient one logit at a time: the constructor reads the (one or two)
+// bins the pdf at x interpolates, operator() gives dL/dz_i from P(i).
+template <int N>
+struct DfLogitGrad {
+    int i0, i1;
+    float w, p0, p1;
+    int linear;
+    template <typename Prob>
+    __device__ __forceinline__ DfLogitGrad(float x, int lin, Prob P) : linear(lin) {
+#pragma clang fp contract(off)
+        if (linear) {
+            const float idx = x * (float)N - 0.5f;
+            i0 = min((int)floorf(idx), N - 1);
+            i1 = i0 + 1;
+            w = idx - (float)i0;
+            if (i0 < 0) { i0 = N - 1; w = 1.0f; }
+            if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+            p0 = P(i0);
+            p1 = P(i1);
+        } else {
+            i0 = min((int)(x * (float)N), N - 1);
+            i0 = max(i0, 0);
+            i1 = 0;
+            w = 0.0f;
+            p0 = P(i0);
+            p1 = 0.0f;
+        }
+    }
+    __device__ __forceinline__ float operator()(int i, float pi, float multiplier) const {
+#pragma clang fp contract(off)
+        if (linear) {
             const float j0 = i0 == i ? p0 * (1.0f - p0) : -p0 * pi;
             const float j1 = i1 == i ? p1 * (1.0f - p1) : -p1 * pi;
             const float j = (1.0f - w) * j0 + w * j1;
-            g[i] = multiplier * j;
+            return multiplier * j;
         }
-    } else {
-        int i0 = min((int)(x * (float)N), N - 1);
-        i0 = max(i0, 0);
-        const float p0 = P(i0);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float pi = P(i);
-            g[i] = multiplier * (i0 == i ? p0 * (1.0f - p0) : -p0 * pi);
-        }
+        return multiplier * (i0 == i ? p0 * (1.0f - p0) : -p0 * pi);
     }
+};
+
+template <int N, typename Prob>
+__device__ __forceinline__ void df_logit_grad(float x, float multiplier, int linear, Prob P, float (&g)[N]) {
+    const DfLogitGrad<N> d(x, linear, P);
+#pragma unroll
+    for (int i = 0; i < N; ++i) g[i] = d(i, P(i), multiplier);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,6 +863,17 @@ __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
 
 __device__ __forceinline__ h8 ld_frag(const _Float16* __restrict__ pack, int t, int s, int ks, int lane) {
     return *(const h8*)(pack + ((size_t)((t * ks + s) * 64 + lane) << 3));
+}
+
+// Rounds a computed fp32 value to fp16 (to nearest even), as storing the
+// finished fp32 result to half does in the reference.  The empty asm makes the
+// value opaque: without it the compiler may fold the multiply that produced
+// it into a mixed-precision multiply-add (v_fma_mixlo_f16) that rounds the
+// exact product to fp16 once, which differs at and near fp16 rounding ties,
+// and differs between kernels that inline the same encoder.
+__device__ __forceinline__ _Float16 round_half(float v) {
+    asm("" : "+v"(v));
+    return (_Float16)v;
 }
 
 // fp16 conversion that saturates at +-65504 instead of overflowing to inf.

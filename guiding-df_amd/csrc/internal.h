@@ -165,14 +165,28 @@ struct TrainFusedArgs {
     const _Float16* tab;      // training grid table or NULL
     const float* feat;
     int stride;
-    const float* target;      // radiance cache: target RGB [B][3]
-    float* loss;              // [B]
+    const float* target;      // radiance cache: target RGB [B][3]; DF: training records [B][9]
+    float* loss;              // [B] (DF: written by the theta pass)
     float* partial;           // [workgroups][n_matrix_params]
     int64_t n, n_tiles;
     float loss_scale;
+    // DF nets
+    const float* gfeat;       // grid features from the pre-pass, [L*F][ld] fp32, or NULL
+    int64_t ld;               // padded batch (column stride of gfeat and dx)
+    int theta_in;             // 1: the phi net (theta input from the record, phi coordinate)
+    int theta_bins, linear;
+    const float* pdf_in;      // [B] the other net's |pdf| at the sample (training pass)
+    float* pdf_out;           // [B] this net's |pdf| (pdf pass; theta's training pass)
+    float* dx;                // [dx_n][ld] dL/d(grid features) or NULL
+    int dx_begin, dx_n;       // the grid features' columns in the encoded input (and dx's columns)
 };
-size_t train_fused_lds_bytes(const NetSpec& hs);
+size_t train_fused_lds_bytes(const NetSpec& hs, bool df, bool fwd_only);
 bool launch_train_fused_nrc(const TrainFusedArgs& a, const NetSpec& hs, int blocks, hipStream_t s);
+// DF nets of 3 hidden layers and 16/32/64 bins: the forward-only |pdf| pass
+// (pdf_only) or the training pass; false when the shape is not instantiated.
+bool train_fused_df_ok(const NetSpec& hs, int n_bins);
+bool launch_train_fused_df(const TrainFusedArgs& a, const NetSpec& hs, int n_bins, bool pdf_only, int blocks,
+                           hipStream_t s);
 void launch_wgrad_reduce(const float* partial, int n_chunks, int64_t n_matrix, float* grad, hipStream_t s);
 
 struct DfLossArgs {

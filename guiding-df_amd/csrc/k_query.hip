@@ -131,7 +131,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         // radiance, not a pdf, and feeds training targets only.
         _Float16* const xh = (_Float16*)smem;
         _Float16* const xhrow = xh + lane * a.xs_nrc;
-        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = (_Float16)v; },
+        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = round_half(v); },
                    nullptr, 0, false, false);
         if constexpr (LOCKSTEP) __syncthreads();
         mlp_forward_tile<1, false, 4>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);

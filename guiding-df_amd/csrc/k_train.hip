@@ -43,7 +43,7 @@ __global__ __launch_bounds__(64, 2) void train_fwd_kernel(const TrainFwdArgs a) 
     }
     _Float16* const xrow = xt + lane * a.xs;
     // the training forward feeds fp16 activations, like the reference's
-    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = (_Float16)v; },
+    encode_row(ns, a.tab, RowInT{frow, base, th_in}, [xrow](int c, float v) { xrow[c] = round_half(v); },
                a.gfeat ? a.gfeat + r : nullptr, a.ld);
     const int kc = ns.k_steps * 32;
     for (int c = 0; c < kc; ++c) a.x[(size_t)c * a.ld + row] = xrow[c];
@@ -143,14 +143,10 @@ __global__ void df_loss_kernel(const DfLossArgs a) {
     const float pp = a.linear ? pdf_linear<NPH>(x1, Pp) : pdf_nearest<NPH>(x1, Pp);
     // AutoRegressive(guidingData): mPdfTheta = |p_theta|, mPdfPhi = |p_phi| (AR.h:25-35)
     const float apt = fabsf(pt), app = fabsf(pp);
-    const float Li = (t[2] + t[3] + t[4]) / 3.0f;
-    const float wiPdf = t[5] + DFG_EPS;
-    const float guided = ((apt * app) * DFG_INV_4PI) * (1.0f / DFG_INV_4PI) + DFG_EPS;
-    const float ls = a.loss_scale / (float)a.n;
-    const float prefix = -Li / wiPdf / guided * ls;
-    a.loss[row] = -Li / wiPdf * det_logf(guided);
-    df_grad<NTH, NTH>(x0, app, prefix, bt, a.linear, a.gout_th_rm, a.gout_th_fm, a.ld, row);
-    df_grad<NPH, NPH>(x1, apt, prefix, bp, a.linear, a.gout_ph_rm, a.gout_ph_fm, a.ld, row);
+    const DfRowTerms rt = df_row_terms(t, apt, app, a.loss_scale / (float)a.n);
+    a.loss[row] = rt.loss;
+    df_grad<NTH, NTH>(x0, app, rt.prefix, bt, a.linear, a.gout_th_rm, a.gout_th_fm, a.ld, row);
+    df_grad<NPH, NPH>(x1, apt, rt.prefix, bp, a.linear, a.gout_ph_rm, a.gout_ph_fm, a.ld, row);
 }
 
 void launch_df_loss(const DfLossArgs& a, int nt, int np, hipStream_t s) {
@@ -1167,14 +1163,14 @@ __device__ __forceinline__ void adam_ema_param(const AdamArgs& a, int64_t i, _Fl
         const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
         const float nw = wf - elr * m;
         a.master[i] = nw;
-        w16 = (_Float16)nw;
+        w16 = round_half(nw);
         a.w16[i] = w16;
     } else {
         w16 = a.w16[i];
     }
     const float e = (float)a.ema16[i];
     const float w = (float)w16;
-    const _Float16 ne = (_Float16)((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
+    const _Float16 ne = round_half((e * a.ema_decay * a.ema_debias_old + w * (1.0f - a.ema_decay)) * a.ema_debias_new);
     a.ema16[i] = ne;
     w16_out = w16;
     ema_out = ne;
@@ -1230,11 +1226,11 @@ __device__ __forceinline__ void adam_grid_update(const AdamArgs& a, float g, flo
         lr *= sqrtf(1.0f - exp2f((float)st * a.log2_beta2)) / (1.0f - exp2f((float)st * a.log2_beta1));
         const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
         w = w - elr * m;
-        w16 = (_Float16)w;
+        w16 = round_half(w);
     }
     const float e = (float)ema;
     const float wv = (float)w16;
-    ema = (_Float16)((e * a.ema_decay * a.ema_debias_old + wv * (1.0f - a.ema_decay)) * a.ema_debias_new);
+    ema = round_half((e * a.ema_decay * a.ema_debias_old + wv * (1.0f - a.ema_decay)) * a.ema_debias_new);
 }
 
 __device__ __forceinline__ void adam_pair_step(const AdamArgs& a, AdamPair& p) {

@@ -538,6 +538,62 @@ def test_nrc_fused_training_matches_separate_kernels(torch_cuda, B):
     fb.close()
 
 
+@pytest.mark.parametrize("bins,interp,B", [((32, 32), "linear", 4096), ((32, 32), "linear", 262144),
+                                           ((32, 32), "linear", 1000), ((16, 32), "nearest", 4096),
+                                           ((64, 64), "linear", 3000)])
+def test_df_fused_training_matches_separate_kernels(torch_cuda, bins, interp, B):
+    """The per-net fused DF training passes (phi's forward-only |pdf| pass,
+    then theta's and phi's passes with the activations in LDS) against the
+    separate forward / loss / backward / weight-gradient kernels on identical
+    fields: the same per-row losses and the same grid gradients bit for bit
+    (same forward, softmax, pdf, logit-gradient and data-gradient arithmetic,
+    and the deterministic grid backward), the same MLP weight gradients up to
+    the summation order of their fp32 partial sums, and weights that stay
+    within that order's drift over three steps."""
+    torch = torch_cuda
+    cfgs = []
+    for fused in (1, 0):
+        c = G.hash_config(n_bins_theta=bins[0], n_bins_phi=bins[1], with_nrc=False, log2_hashmap_size=14)
+        c["dfg"]["interpolation"] = interp
+        c["dfg"]["fused_training"] = fused
+        cfgs.append(c)
+    fa, fb = G.GuidingField(cfgs[0]), G.GuidingField(cfgs[1])
+    th, ph, _ = O.config_layouts(cfgs[0])
+    feat = dev(torch, df_features(B, 46))
+    tgt = dev(torch, train_targets(B, 47))
+    la = torch.zeros(3, dtype=torch.float32, device="cuda")
+    lb = torch.zeros(3, dtype=torch.float32, device="cuda")
+    for k in range(3):
+        fa.train_step(feat, tgt, la[k:k + 1])
+        fb.train_step(feat, tgt, lb[k:k + 1])
+        torch.cuda.synchronize()
+        if k == 0:
+            assert host(la)[0] == host(lb)[0], (host(la)[0], host(lb)[0])
+            for net, L in ((0, th), (1, ph)):
+                ga, gb = fa.get_grad(net).astype(np.float64), fb.get_grad(net).astype(np.float64)
+                nm = L["n_matrix"]
+                assert np.array_equal(ga[nm:], gb[nm:]), "net %d grid gradient differs" % net
+                assert np.isfinite(ga).all() and np.abs(gb[:nm]).max() > 0
+                assert _grad_rel(ga[:nm], gb[:nm]) <= 1e-6, _grad_rel(ga[:nm], gb[:nm])
+                assert np.allclose(ga[:nm], gb[:nm], rtol=1e-4, atol=1e-6 * np.abs(gb[:nm]).max())
+    # later steps: Adam's first step moves every parameter by +-lr (1e-2) by
+    # the sign of its gradient, so a parameter whose gradient is ~0 can step
+    # the other way when the fp32 summation order differs, and every later
+    # forward then differs a little.  Measured after three steps: losses
+    # within 5e-5; weights identical to 1e-5 (relative norm) except the
+    # 262,144-row phi net, where a sign flipped at step 1 (relative norm
+    # 1.0e-2, median |dw| 1.5e-5, 1.7 % of weights off by more than lr / 10)
+    assert np.allclose(host(la), host(lb), rtol=1e-4), (host(la), host(lb))
+    for net in (0, 1):
+        wa, wb = fa.get_params(net, G.DFG_PARAMS_MASTER), fb.get_params(net, G.DFG_PARAMS_MASTER)
+        d = np.abs(wa - wb)
+        assert np.median(d) <= 1e-4, np.median(d)
+        assert np.linalg.norm(wa - wb) <= 3e-2 * np.linalg.norm(wb), np.linalg.norm(wa - wb) / np.linalg.norm(wb)
+        assert (d > 1e-3).mean() <= 0.05, (d > 1e-3).mean()
+    fa.close()
+    fb.close()
+
+
 @pytest.mark.parametrize("cfgname", ["plain", "hash", "reference"])
 def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
     """The plain FP32 training comparator: fp32 bins, activations and

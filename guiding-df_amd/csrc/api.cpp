@@ -1175,22 +1175,29 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
     NetState& th = ctx->net[0];
     NetState& ph = ctx->net[1];
     ctx->loss.ensure((size_t)ld);
-    if (ctx->fused_training && ctx->pair_ok && train_fused_df_ok(th.spec, ctx->n_theta) &&
-        train_fused_df_ok(ph.spec, ctx->n_phi)) {
+    if (ctx->fused_training && train_fused_df_ok(th.spec, ctx->n_theta) && train_fused_df_ok(ph.spec, ctx->n_phi)) {
         // per net one pass over 64-row tiles kept in LDS (forward, loss,
         // backward, weight gradients); the loss couples the nets through each
         // row's |pdf|: phi's forward alone first, then theta's and phi's passes
         const int64_t n_tiles = ld / 64;
         const int blocks = (int)std::min<int64_t>(n_tiles, 2 * ctx->n_cus);
-        const size_t cols = (size_t)th.spec.grid.n_levels * th.spec.grid.n_feat;
-        th.dx.ensure(cols * ld);
-        ph.dx.ensure(cols * ld);
         th.partial.ensure((size_t)blocks * th.spec.n_matrix_params);
         ph.partial.ensure((size_t)blocks * ph.spec.n_matrix_params);
         ctx->train_pdf.ensure((size_t)2 * ld);
         float* pdf_th = ctx->train_pdf.p;
         float* pdf_ph = ctx->train_pdf.p + ld;
-        train_grid_pair(ctx, feat, feat_stride, B, ld, st);
+        // the grid features of the batch: one pass over the interleaved table
+        // for grids of identical structure, else one pre-pass per net
+        for (NetState* s : {&th, &ph})
+            if (s->spec.grid.present) s->dx.ensure((size_t)s->spec.grid.n_levels * s->spec.grid.n_feat * ld);
+        if (!train_grid_pair(ctx, feat, feat_stride, B, ld, st)) {
+            for (NetState* s : {&th, &ph}) {
+                if (!s->spec.grid.present) continue;
+                s->gfeat.ensure((size_t)s->spec.grid.n_levels * s->spec.grid.n_feat * ld);
+                const float* pos = compact_pos(ctx, feat, feat_stride, grid_pos_dim(s->spec), B, st);
+                launch_grid_encode(s->d_spec.p, s->spec, s->tab_train(), pos, B, ld, s->gfeat.p, st);
+            }
+        }
         auto args = [&](NetState& s, int theta_in) {
             TrainFusedArgs f;
             std::memset(&f, 0, sizeof(f));
@@ -1204,15 +1211,17 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
             f.n = B;
             f.n_tiles = n_tiles;
             f.loss_scale = 1024.0f;  // TRAIN_LOSS_SCALE, parameters.h:45
-            f.gfeat = s.gfeat.p;
             f.ld = ld;
             f.theta_in = theta_in;
             f.theta_bins = ctx->n_theta;
             f.linear = ctx->linear;
             const int gp = grid_part_index(s.spec);
-            f.dx = s.dx.p;
-            f.dx_begin = s.spec.parts[gp].out_begin;
-            f.dx_n = (int)cols;
+            if (gp >= 0) {
+                f.gfeat = s.gfeat.p;
+                f.dx = s.dx.p;
+                f.dx_begin = s.spec.parts[gp].out_begin;
+                f.dx_n = s.spec.grid.n_levels * s.spec.grid.n_feat;
+            }
             return f;
         };
         TrainFusedArgs fp = args(ph, 1);

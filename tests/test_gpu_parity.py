@@ -369,12 +369,15 @@ def _grad_rel(g, o):
     return np.linalg.norm(g - o) / max(np.linalg.norm(o), 1e-30)
 
 
-@pytest.mark.parametrize("cfgname", ["plain", "hash"])
+@pytest.mark.parametrize("cfgname", ["plain", "hash", "reference"])
 def test_train_step_vs_fp64(torch_cuda, cfgname):
     torch = torch_cuda
     if cfgname == "plain":
         cfg = G.plain_head_config()
         D, feat = 8, plain_features(1000, 20)
+    elif cfgname == "reference":
+        cfg = G.reference_config(with_nrc=False)
+        D, feat = 9, df_features(1000, 20)
     else:
         cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
         D, feat = 9, df_features(1000, 20)
@@ -393,8 +396,14 @@ def test_train_step_vs_fp64(torch_cuda, cfgname):
     # the reference keeps training bins in fp16; with that storage emulated the
     # loss agrees tightly (log q near q = 1 amplifies the fp16 rounding)
     loss_h, _, _ = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1, grads=False, half_bins=True)
-    assert abs(loss_g - loss_h) <= 2e-3 * abs(loss_h) + 1e-6, (loss_g, loss_h, loss_o)
-    assert abs(loss_g - loss_o) <= 2e-2 * abs(loss_o), (loss_g, loss_o)
+    # the batch loss is a mean of per-row terms of both signs (the reference
+    # configuration's nearly cancels: 0.125 from terms of O(1)), so the
+    # tolerance scales with the mean |term|
+    scale = np.mean([abs(O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat[i:i + 1], D, tgt[i:i + 1], 1,
+                                        grads=False)[0]) for i in range(B)])
+    print("loss %.7g, fp16-bin FP64 %.7g, FP64 %.7g, mean |row loss| %.4g" % (loss_g, loss_h, loss_o, scale))
+    assert abs(loss_g - loss_h) <= 2e-3 * max(abs(loss_h), scale) + 1e-6, (loss_g, loss_h, loss_o, scale)
+    assert abs(loss_g - loss_o) <= 2e-2 * max(abs(loss_o), scale), (loss_g, loss_o, scale)
     # gradients against FP64 and against the fp16-bin emulation (the bins the
     # reference differentiates through are fp16, train.h:874-893)
     _, gt_h, gp_h = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, D, tgt, 1, half_bins=True)
@@ -538,10 +547,14 @@ def test_nrc_fused_training_matches_separate_kernels(torch_cuda, B):
     fb.close()
 
 
-@pytest.mark.parametrize("bins,interp,B", [((32, 32), "linear", 4096), ((32, 32), "linear", 262144),
-                                           ((32, 32), "linear", 1000), ((16, 32), "nearest", 4096),
-                                           ((64, 64), "linear", 3000)])
-def test_df_fused_training_matches_separate_kernels(torch_cuda, bins, interp, B):
+@pytest.mark.parametrize("cfgname,bins,interp,B", [("hash", (32, 32), "linear", 4096),
+                                                   ("hash", (32, 32), "linear", 262144),
+                                                   ("hash", (32, 32), "linear", 1000),
+                                                   ("hash", (16, 32), "nearest", 4096),
+                                                   ("hash", (64, 64), "linear", 3000),
+                                                   ("reference", (16, 32), "linear", 4096),
+                                                   ("plain", (32, 32), "linear", 2000)])
+def test_df_fused_training_matches_separate_kernels(torch_cuda, cfgname, bins, interp, B):
     """The per-net fused DF training passes (phi's forward-only |pdf| pass,
     then theta's and phi's passes with the activations in LDS) against the
     separate forward / loss / backward / weight-gradient kernels on identical
@@ -549,17 +562,25 @@ def test_df_fused_training_matches_separate_kernels(torch_cuda, bins, interp, B)
     (same forward, softmax, pdf, logit-gradient and data-gradient arithmetic,
     and the deterministic grid backward), the same MLP weight gradients up to
     the summation order of their fp32 partial sums, and weights that stay
-    within that order's drift over three steps."""
+    within that order's drift over three steps.  Paired hash grids (one pass
+    over the interleaved table), the reference's default DenseGrid F = 4
+    smoothstep configuration (a grid pre-pass per net) and grid-less identity
+    encodings."""
     torch = torch_cuda
     cfgs = []
     for fused in (1, 0):
-        c = G.hash_config(n_bins_theta=bins[0], n_bins_phi=bins[1], with_nrc=False, log2_hashmap_size=14)
+        if cfgname == "reference":
+            c = G.reference_config(with_nrc=False)
+        elif cfgname == "plain":
+            c = G.plain_head_config(n_bins=bins[0])
+        else:
+            c = G.hash_config(n_bins_theta=bins[0], n_bins_phi=bins[1], with_nrc=False, log2_hashmap_size=14)
         c["dfg"]["interpolation"] = interp
         c["dfg"]["fused_training"] = fused
         cfgs.append(c)
     fa, fb = G.GuidingField(cfgs[0]), G.GuidingField(cfgs[1])
     th, ph, _ = O.config_layouts(cfgs[0])
-    feat = dev(torch, df_features(B, 46))
+    feat = dev(torch, plain_features(B, 46) if cfgname == "plain" else df_features(B, 46))
     tgt = dev(torch, train_targets(B, 47))
     la = torch.zeros(3, dtype=torch.float32, device="cuda")
     lb = torch.zeros(3, dtype=torch.float32, device="cuda")
@@ -572,7 +593,8 @@ def test_df_fused_training_matches_separate_kernels(torch_cuda, bins, interp, B)
             for net, L in ((0, th), (1, ph)):
                 ga, gb = fa.get_grad(net).astype(np.float64), fb.get_grad(net).astype(np.float64)
                 nm = L["n_matrix"]
-                assert np.array_equal(ga[nm:], gb[nm:]), "net %d grid gradient differs" % net
+                if L["grid"]:
+                    assert np.array_equal(ga[nm:], gb[nm:]), "net %d grid gradient differs" % net
                 assert np.isfinite(ga).all() and np.abs(gb[:nm]).max() > 0
                 assert _grad_rel(ga[:nm], gb[:nm]) <= 1e-6, _grad_rel(ga[:nm], gb[:nm])
                 assert np.allclose(ga[:nm], gb[:nm], rtol=1e-4, atol=1e-6 * np.abs(gb[:nm]).max())

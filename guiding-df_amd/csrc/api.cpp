@@ -590,6 +590,34 @@ void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool w
     const int ls4 = (a.ls + 3) / 4 * 4;
     a.xs = with_df ? std::max((ks + 3) / 4 * 4 + 4, ls4) : 0;
     a.xs_nrc = with_nrc ? std::max((c->net[2].spec.enc_width + 7) / 8 * 8 + 8, 2 * ls4) : 0;
+    a.lo_off = 0;
+    a.ph_part0 = 0;
+    // phi's nested parts after the grid that equal theta's (the same inputs,
+    // widths and columns: SH and OneBlob of the same direction/normal) keep
+    // theta's columns: the logits then go to the row's tail, past them, and
+    // phi encodes only its remaining parts (TriangleWave of theta)
+    const NetSpec& th = c->net[0].spec;
+    const NetSpec& ph = c->net[1].spec;
+    if (with_df && a.gfeat_th && a.gfeat_ph && th.grid_ksteps > 0 && th.grid_ksteps == ph.grid_ksteps &&
+        th.n_parts >= 2 && ph.n_parts > th.n_parts) {
+        bool same = true;
+        for (int i = 1; i < th.n_parts && same; ++i) {
+            const EncPart& p = th.parts[i];
+            const EncPart& q = ph.parts[i];
+            same = p.type == q.type && p.dim_begin == q.dim_begin && p.n_dims == q.n_dims &&
+                   p.out_begin == q.out_begin && p.n_out == q.n_out && p.pad_front == q.pad_front &&
+                   p.param == q.param && p.id_scale == q.id_scale && p.id_offset == q.id_offset &&
+                   p.type != ENC_GRID && (i + 1 == th.n_parts || p.n_pad == q.n_pad) && (p.pad_front == 0 || p.n_pad == 0);
+        }
+        if (same) {
+            const EncPart& last = th.parts[th.n_parts - 1];
+            const int end = last.out_begin + last.n_out - 32 * th.grid_ksteps;  // shared columns of the row
+            a.lo_off = (end + 3) / 4 * 4;
+            a.xs = std::max(a.xs, a.lo_off + ls4 + 4);
+            a.ls = a.xs;  // the logits share the input row's stride
+            a.ph_part0 = th.n_parts;
+        }
+    }
 }
 
 QueryArgs dfg::base_args(dfg_ctx* c) {

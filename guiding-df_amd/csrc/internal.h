@@ -49,6 +49,9 @@ struct QueryArgs {
     int xs;              // LDS row strides: DF encoded input (floats),
     int xs_nrc;          // NRC encoded input (halves),
     int ls;              // logits (floats)
+    int lo_off;          // logits' first float within the row (0: they alias the row's start)
+    int ph_part0;        // phi's first nested part to encode: the earlier ones equal theta's,
+                         // whose columns phi reads as theta left them (0: encode every part)
 };
 
 // Arguments of the training forward kernel of one net.

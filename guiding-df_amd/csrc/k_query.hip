@@ -5,6 +5,9 @@
 // all in one launch with the activations in registers and the encoded inputs
 // and logits in LDS.  Replaces the ~10 launches and the device-wide sync per
 // bounce of LearnedDistribution::inferenceStep (learnedDistribution.cpp:167-206).
+#include <mutex>
+#include <vector>
+
 #include "device.h"
 #include "internal.h"
 
@@ -49,7 +52,7 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false>
 __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
     float* xt = (float*)smem;
-    float* lo = (float*)smem;
+    float* lo = (float*)smem + a.lo_off;
     const int lane = threadIdx.x & 63;
     const int64_t row = row0 + lane;
     const bool valid = row < io.n;
@@ -98,9 +101,10 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     // ---- phi net (its own encoding, theta appended as the last raw input)
     const int gk_ph = a.gfeat_ph ? ph.grid_ksteps : 0;
     if (gk_ph) {
+        // (with a.ph_part0 > 0 the leading parts are theta's, still in the row)
         const int o = 32 * gk_ph;
         encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
-                   nullptr, 0, true, false);
+                   nullptr, 0, true, false, a.ph_part0);
     } else {
         encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
                    false, false);
@@ -244,6 +248,24 @@ static size_t max_block_lds() {
     return (size_t)cached;
 }
 
+// Lifts a kernel's dynamic-LDS limit to `lds` bytes when that exceeds the
+// default 64 KB (once per kernel and size; host-side cache).  A refusal
+// leaves the limit as it was, and the launch that follows then fails, which
+// the caller's launch check reports.
+static void allow_lds(const void* fn, size_t lds) {
+    if (lds <= 65536) return;
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> done;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t* have = nullptr;
+    for (auto& d : done)
+        if (d.first == fn) have = &d.second;
+    if (have && *have >= lds) return;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return;
+    if (have) *have = lds;
+    else done.emplace_back(fn, lds);
+}
+
 template <int NTH, int NPH>
 static void launch_q(const QueryArgs& a, hipStream_t s) {
     const int64_t tiles = (a.n + 63) / 64;
@@ -253,12 +275,16 @@ static void launch_q(const QueryArgs& a, hipStream_t s) {
     const unsigned blocks = (unsigned)((tiles + tpb - 1) / tpb);
     const size_t lds = tl * tpb;
     const dim3 bd(64 * tpb);
-    if (a.th == nullptr)
+    if (a.th == nullptr) {
+        allow_lds((const void*)query_fused_kernel<16, 16, false, true>, lds);
         hipLaunchKernelGGL((query_fused_kernel<16, 16, false, true>), dim3(blocks), bd, lds, s, a);
-    else if (a.rgb)
+    } else if (a.rgb) {
+        allow_lds((const void*)query_fused_kernel<NTH, NPH, true, true>, lds);
         hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, true>), dim3(blocks), bd, lds, s, a);
-    else
+    } else {
+        allow_lds((const void*)query_fused_kernel<NTH, NPH, true, false>, lds);
         hipLaunchKernelGGL((query_fused_kernel<NTH, NPH, true, false>), dim3(blocks), bd, lds, s, a);
+    }
 }
 
 static size_t query_lds(const QueryArgs& a) {
@@ -274,8 +300,13 @@ template <int NTH, int NPH>
 static void launch_wl(const QueryArgs& a, const WorkList* wl, unsigned int* counter, int blocks, int tpf,
                       hipStream_t s) {
     const size_t lds = query_lds(a);
-    if (a.rgb) hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, true>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
-    else hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, false>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
+    if (a.rgb) {
+        allow_lds((const void*)query_worklist_kernel<NTH, NPH, true>, lds);
+        hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, true>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
+    } else {
+        allow_lds((const void*)query_worklist_kernel<NTH, NPH, false>, lds);
+        hipLaunchKernelGGL((query_worklist_kernel<NTH, NPH, false>), dim3(blocks), dim3(64), lds, s, a, wl, counter, tpf);
+    }
 }
 
 void launch_query_worklist(const QueryArgs& a, int nt, int np, const WorkList* wl, unsigned int* counter,

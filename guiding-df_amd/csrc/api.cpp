@@ -618,6 +618,14 @@ void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool w
             a.ph_part0 = th.n_parts;
         }
     }
+    // one row per lane: the lanes' same-column accesses (encoder stores, the
+    // heads' bin reads and writes) hit 64 different banks when the row stride
+    // in dwords is odd (the first layer then reads its operands as dwords)
+    if (a.xs) {
+        a.xs = a.xs | 1;
+        if (a.lo_off) a.ls = a.xs;
+        else a.ls = std::max(a.ls, 1) | 1;
+    }
 }
 
 QueryArgs dfg::base_args(dfg_ctx* c) {

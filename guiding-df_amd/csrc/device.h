@@ -910,13 +910,33 @@ __device__ __forceinline__ void c_to_act(const f4 (&c)[4], h8 (&a)[2]) {
 // The first `steps` K-steps of layer 0 read straight from the grid pre-pass
 // buffer (fp32, feature-major, column stride `stride`) instead of the LDS
 // tile; rows past `last` are clamped to it (their results are discarded).
+// The first K-step's grid features of a 64-row tile loaded ahead of the
+// layer (lane: rows 16q + i16 of the four groups, features 8g..8g+7), so the
+// loads overlap the work before it.
+struct GridPre {
+    float v[4][8];
+};
 struct GridDirect {
     const float* base;
     int64_t stride;
     int steps;
     int64_t row0;
     int64_t last;
+    const GridPre* pre = nullptr;  // K-step 0 already loaded (all four groups in one pass)
 };
+
+__device__ __forceinline__ void grid_pre_load(const GridDirect& gd, GridPre& p) {
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int64_t rr = gd.row0 + q * 16 + i16;
+        if (rr > gd.last) rr = gd.last;
+        const float* gsrc = gd.base + (size_t)(g * 8) * gd.stride + rr;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p.v[q][j] = gsrc[(size_t)j * gd.stride];
+    }
+}
 
 struct ActStore {
     _Float16* x;        // encoded input, [k_steps*32][ld] (NULL: not stored)
@@ -1055,10 +1075,23 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                 // columns from enc_width to the K-step boundary are zero and not in the tile
                 const bool beyond = s * 32 + g * 8 >= ns.enc_width;
                 if constexpr (sizeof(TT) == 2) {
-                    b[q] = beyond ? h8{} : *(const h8*)src;
+                    // rows 16-byte aligned: one vector read; rows of an odd
+                    // dword stride (per-lane rows without bank conflicts): 4 dwords
+                    if (beyond) {
+                        b[q] = h8{};
+                    } else if ((xs & 7) == 0) {
+                        b[q] = *(const h8*)src;
+                    } else {
+                        const uint32_t* s32 = (const uint32_t*)src;
+                        uint4 u4 = make_uint4(s32[0], s32[1], s32[2], s32[3]);
+                        b[q] = __builtin_bit_cast(h8, u4);
+                    }
                 } else {
                     float v[8];
-                    if (s < gd.steps) {
+                    if (s == 0 && gd.pre && QN == 4) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[j] = gd.pre->v[q][j];
+                    } else if (s < gd.steps) {
                         int64_t rr = gd.row0 + (q0 + q) * 16 + i16;
                         if (rr > gd.last) rr = gd.last;
                         const float* gsrc = gd.base + (size_t)(s * 32 + g * 8) * gd.stride + rr;
@@ -1067,11 +1100,14 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
                     } else if (beyond) {
 #pragma unroll
                         for (int j = 0; j < 8; ++j) v[j] = 0.0f;
-                    } else {
+                    } else if ((xs & 3) == 0) {
                         const float4 v0 = *(const float4*)src;
                         const float4 v1 = *(const float4*)(src + 4);
                         v[0] = v0.x; v[1] = v0.y; v[2] = v0.z; v[3] = v0.w;
                         v[4] = v1.x; v[5] = v1.y; v[6] = v1.z; v[7] = v1.w;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[j] = src[j];
                     }
                     if constexpr (SPLIT_IN) {
                         split8(v, b[q], bl[q]);

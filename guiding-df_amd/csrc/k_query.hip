@@ -49,7 +49,9 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 // One 64-row tile of a query: rows [row0, row0 + 64) of `io` (lane = row).
 // LOCKSTEP: a workgroup barrier before each MLP keeps the workgroup's waves on
 // the same net's weights (every wave of the workgroup must run the tile).
-template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false>
+// PREFETCH: load each net's single grid K-step ahead of its MLP (more live
+// registers; the work-list kernels, already near the register limit, skip it).
+template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false, bool PREFETCH = false>
 __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
     float* xt = (float*)smem;
     float* lo = (float*)smem + a.lo_off;
@@ -69,6 +71,16 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
 
     // ---- theta net (with the pre-pass, its leading grid K-steps skip the LDS tile)
     const int gk_th = a.gfeat_th ? th.grid_ksteps : 0;
+    const int gk_ph = a.gfeat_ph ? ph.grid_ksteps : 0;
+    // a single grid K-step (32 features) is loaded ahead: theta's before its
+    // encoding, phi's before theta's head
+    GridPre pre_th, pre_ph;
+    const bool pth = PREFETCH && gk_th == 1, pph = PREFETCH && gk_ph == 1;
+    const GridDirect gd_th{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1,
+                           pth ? &pre_th : nullptr};
+    const GridDirect gd_ph{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1,
+                           pph ? &pre_ph : nullptr};
+    if (pth) grid_pre_load(gd_th, pre_th);
     if (gk_th) {
         const int o = 32 * gk_th;
         encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
@@ -78,8 +90,8 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
                    false, false);
     }
     if constexpr (LOCKSTEP) __syncthreads();
-    mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr,
-                                     GridDirect{gk_th ? a.gfeat_th + io.gbase : nullptr, a.gld, gk_th, row0, io.n - 1});
+    mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr, gd_th);
+    if (pph) grid_pre_load(gd_ph, pre_ph);
     if (a.logits_th && valid) {
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
@@ -99,7 +111,6 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
 
     // ---- phi net (its own encoding, theta appended as the last raw input)
-    const int gk_ph = a.gfeat_ph ? ph.grid_ksteps : 0;
     if (gk_ph) {
         // (with a.ph_part0 > 0 the leading parts are theta's, still in the row)
         const int o = 32 * gk_ph;
@@ -110,8 +121,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
                    false, false);
     }
     if constexpr (LOCKSTEP) __syncthreads();
-    mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr,
-                                     GridDirect{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1});
+    mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr, gd_ph);
     if (a.logits_ph && valid) {
 #pragma unroll
         for (int i = 0; i < NPH; ++i) a.logits_ph[r * NPH + i] = lrow[i];
@@ -171,7 +181,7 @@ __global__ __launch_bounds__(64 * kQueryTilesPerBlock, 2) void query_fused_kerne
     const int w = threadIdx.x >> 6;
     const int tpb = blockDim.x >> 6;
     const RowsIO io{a.feat, a.stride, a.u, a.guided, a.wi, a.rgb, a.n, 0};
-    query_tile<NTH, NPH, WITH_DF, WITH_NRC, true>(a, io, ((int64_t)blockIdx.x * tpb + w) * 64,
+    query_tile<NTH, NPH, WITH_DF, WITH_NRC, true, true>(a, io, ((int64_t)blockIdx.x * tpb + w) * 64,
                                                   smem + w * query_tile_lds(a));
 }
 

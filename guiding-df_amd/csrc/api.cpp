@@ -393,6 +393,7 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
         if (c->prepass_blocks_per_cu < 0 || c->prepass_blocks_per_cu > 64)
             throw Error(DFG_ERR_CONFIG, "prepass_blocks_per_cu must be in 0..64");
         c->fused_training = d.number_or("fused_training", 1.0) != 0.0;
+        c->keep_gradients = d.number_or("keep_gradients", 0.0) != 0.0;
         c->wgrad_chunk_rows = (int)d.number_or("wgrad_chunk_rows", 1024);
         if (c->wgrad_chunk_rows < 64 || c->wgrad_chunk_rows > 65536 || c->wgrad_chunk_rows % 64)
             throw Error(DFG_ERR_CONFIG, "wgrad_chunk_rows must be a multiple of 64 in 64..65536");
@@ -512,6 +513,10 @@ int dfg_get_grad(dfg_ctx* ctx, int net, float* host_out, size_t n) {
     NetState& s = need_net(ctx, net);
     if (!host_out) throw Error(DFG_ERR_INVALID_ARG, "null output");
     if (n != s.spec.n_params) throw Error(DFG_ERR_BAD_SIZE, "parameter count mismatch");
+    if (!s.grad_kept)
+        throw Error(DFG_ERR_INVALID_ARG,
+                    "the grid part of this gradient went straight into the optimizer; create the context with "
+                    "\"keep_gradients\": 1 to keep it");
     HIPX(hipMemcpy(host_out, s.grad.p, n * sizeof(float), hipMemcpyDeviceToHost));
     API_END(ctx)
 }
@@ -1084,7 +1089,7 @@ static void net_grid_backward(dfg_ctx* c, NetState& s, const float* feat, int st
 // pos: the batch's compact grid positions [n][3] (from the step's paired
 // pre-pass), read 16 times (once per level) instead of the strided features.
 static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, const float* pos, int64_t n, int64_t ld,
-                               hipStream_t st) {
+                               hipStream_t st, const GridAdam* gadam = nullptr) {
     NetState& th = c->net[0];
     NetState& ph = c->net[1];
     if (!c->pair_ok || !c->grid_bwd_bucketed || th.spec.grid.interp == INTERP_NEAREST) return false;
@@ -1103,7 +1108,9 @@ static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, const 
     a.grad2 = ph.grad.p;
     a.param_begin2 = ph.spec.grid.param_begin;
     a.lvl_max2 = mx + DFG_MAX_LEVELS;
-    launch_grid_bwd_bucketed_pair(a, th.spec, plan, st);
+    GridAdam off;
+    std::memset(&off, 0, sizeof(off));
+    launch_grid_bwd_bucketed_pair(a, th.spec, plan, gadam ? *gadam : off, st);
     return true;
 }
 
@@ -1167,6 +1174,7 @@ static void nets_pack(std::initializer_list<NetState*> nets, hipStream_t st) {
 static void net_pack(NetState& s, hipStream_t st) { nets_pack({&s}, st); }
 
 static void net_update(dfg_ctx* c, NetState& s, float loss_scale, hipStream_t st) {
+    s.grad_kept = true;
     launch_adam_ema(adam_args(c, s, loss_scale), st);
     net_pack(s, st);
 }
@@ -1191,8 +1199,38 @@ static void nets_update_pair(dfg_ctx* c, float loss_scale, hipStream_t st) {
     }
     const AdamArgs a = adam_args(c, th, loss_scale);
     const AdamArgs b = adam_args(c, ph, loss_scale);
+    th.grad_kept = ph.grad_kept = true;
     launch_adam_ema_pair(a, b, entries, st);
     nets_pack({&th, &ph}, st);
+}
+
+// Grid backward of both DF nets with their grids' Adam + EMA inside it (each
+// slice's sums update its entries; no gradient round trip through HBM), then
+// the MLP parameters' Adam + EMA and the packs.  False (nothing launched, no
+// step counted) when the paired bucketed backward does not apply.
+static bool grid_backward_update_pair(dfg_ctx* c, const float* feat, int stride, const float* pos, int64_t n,
+                                      int64_t ld, float loss_scale, hipStream_t st) {
+    NetState& th = c->net[0];
+    NetState& ph = c->net[1];
+    if (!c->pair_ok || !c->grid_bwd_bucketed || th.spec.grid.interp == INTERP_NEAREST) return false;
+    const int64_t entries = (int64_t)th.spec.grid.offset[th.spec.grid.n_levels];
+    auto tail = [&](const NetState& s) {
+        const int64_t b = s.spec.grid.param_begin;
+        return b + 2 * entries == s.spec.n_params && b >= s.spec.n_matrix_params && b % 2 == 0;
+    };
+    if (!tail(th) || !tail(ph) || !grid_bucket_plan(th.spec, n, 4).ok) return false;
+    GridAdam ga;
+    std::memset(&ga, 0, sizeof(ga));
+    ga.a = adam_args(c, th, loss_scale);
+    ga.b = adam_args(c, ph, loss_scale);
+    ga.on = 1;
+    ga.keep_grad = c->keep_gradients ? 1 : 0;
+    if (!grid_backward_pair(c, feat, stride, pos, n, ld, st, &ga))
+        throw Error(DFG_ERR_HIP, "paired grid backward refused a plan it accepted");
+    launch_adam_ema_pair(ga.a, ga.b, 0, st);  // the MLP parameters of both nets
+    nets_pack({&th, &ph}, st);
+    th.grad_kept = ph.grad_kept = c->keep_gradients;
+    return true;
 }
 
 static const int kChunk = 1024;
@@ -1275,11 +1313,13 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
             throw Error(DFG_ERR_HIP, "fused DF training launch failed");
         launch_wgrad_reduce(th.partial.p, blocks, (int64_t)th.spec.n_matrix_params, th.grad.p, st);
         launch_wgrad_reduce(ph.partial.p, blocks, (int64_t)ph.spec.n_matrix_params, ph.grad.p, st);
-        if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
-            net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
-            net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, 1024.0f, st)) {
+            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
+                net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
+                net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+            }
+            nets_update_pair(ctx, 1024.0f, st);
         }
-        nets_update_pair(ctx, 1024.0f, st);
         if (loss_out) {
             ctx->mean_ws.ensure(256);
             launch_mean(ctx->loss.p, B, 1.0f / (float)B, loss_out, ctx->mean_ws.p, st);
@@ -1314,11 +1354,13 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
         net_backward_mlp(th, B, ld, n_chunks, chunk, st);
         net_backward_mlp(ph, B, ld, n_chunks, chunk, st);
         // the compact positions of train_grid_pair are still in ctx->pos
-        if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
-            net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
-            net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, 1024.0f, st)) {
+            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
+                net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
+                net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
+            }
+            nets_update_pair(ctx, 1024.0f, st);
         }
-        nets_update_pair(ctx, 1024.0f, st);
     } else {
         net_backward_update(ctx, th, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);
         net_backward_update(ctx, ph, feat, feat_stride, B, ld, n_chunks, chunk, 1024.0f, st);

@@ -65,6 +65,8 @@ struct NetState {
     NetSpec spec;
     OptCfg opt;
     uint32_t step = 0;
+    bool grad_kept = true;   // s.grad holds the whole gradient of the last step (the fused
+                             // grid optimizer stores its grid part only with keep_gradients)
     DevBuf<NetSpec> d_spec;
     DevBuf<float> master, m, v, grad;
     DevBuf<_Float16> w16, ema16, pack_train, pack_infer;
@@ -103,6 +105,8 @@ struct dfg_ctx {
     int64_t pipeline_min_rows = 262144;
     int wgrad_chunk_rows = 1024;       // batch rows per weight-gradient partial sum (multiple of 64)
     bool fused_training = true;        // one-kernel forward/loss/backward/weight-gradient pass where available
+    bool keep_gradients = false;       // store the paired grids' gradient although their optimizer step
+                                       // runs inside the grid backward (for dfg_get_grad)
     int n_cus = 256;
     int device = 0;                    // the device current at dfg_create (weights and workspace live there)
     void* noalloc_scratch = nullptr;   // that device's address of the allocation-free query's scratch array

@@ -283,8 +283,7 @@ struct GridBucketArgs {
 void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p, hipStream_t s);
 // Theta and phi together (F = 2 grids of identical structure, not nearest):
 // a.*2 describe phi; p from grid_bucket_plan(hs, n, 4).
-void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,
-                                   hipStream_t s);
+
 
 struct AdamArgs {
     int64_t n_params, n_matrix;
@@ -310,6 +309,18 @@ void launch_adam_ema(const AdamArgs& a, hipStream_t s);
 // grids share the interleaved tables (a: theta, b: phi), whole entries per
 // thread, and in the same launch their MLP parameters [0, grid_begin).
 void launch_adam_ema_pair(const AdamArgs& a, const AdamArgs& b, int64_t n_entries, hipStream_t s);
+
+// Adam + EMA of the paired grids' parameters fused into the paired grid
+// backward: each slice's fixed-point sums update the parameters of its
+// entries directly (both nets, whole interleaved-table entries); the gradient
+// is stored only with keep_grad.  on == 0: the reduce stores the gradient and
+// the optimizer runs separately.
+struct GridAdam {
+    AdamArgs a, b;
+    int on, keep_grad;
+};
+void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,
+                                   const GridAdam& ga, hipStream_t s);
 
 // Pack fragments from fp16 parameters. which: 1 forward, 2 backward, 3 both.
 void launch_pack(const NetSpec* dns, const NetSpec& hs, const _Float16* w16, _Float16* pack, int which, hipStream_t s);

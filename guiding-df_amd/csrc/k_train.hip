@@ -968,11 +968,16 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_kernel(const GridBucketA
     }
 }
 
+// Adam + EMA of grid entry e (F = 2 parameters) of both nets from its gradient
+// (defined with the optimizer below).
+__device__ void grid_adam_pair_entry(const GridAdam& ga, int64_t e, float2 gt, float2 gp);
+
 // Paired theta/phi reduce: one block per (slice, part) of the 18-byte records
 // (16-bit entry in rec_idx, rec_val float4); LDS sums [entry][4] (theta f0, f1, phi f0,
 // f1), each net at its own fixed-point scale; writes both nets' slices (single
-// part) or the part's partial sums.
-__global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBucketArgs a) {
+// part) or the part's partial sums.  With ga.on a single-part slice goes
+// straight into the optimizer instead of the gradient arrays.
+__global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBucketArgs a, const GridAdam ga) {
     extern __shared__ unsigned long long acc[];  // [4][NE] (value-major: no bank conflicts)
     constexpr int NE = kGridSliceFloats / 4;
     static_assert(NE <= 65536, "paired record entries are 16-bit");
@@ -1022,7 +1027,16 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
         }
     }
     __syncthreads();
-    if (item.z == 1) {
+    if (item.z == 1 && ga.on) {
+        const int64_t E0 = (int64_t)gs.offset[level] + e0;
+        const float nan = __builtin_nanf("");
+        for (int en = threadIdx.x; en < ne; en += blockDim.x) {
+            const float2 gt = fin_a ? make_float2(unfix64(acc[en], Sa), unfix64(acc[NE + en], Sa)) : make_float2(nan, nan);
+            const float2 gp = fin_b ? make_float2(unfix64(acc[2 * NE + en], Sb), unfix64(acc[3 * NE + en], Sb))
+                                    : make_float2(nan, nan);
+            grid_adam_pair_entry(ga, E0 + en, gt, gp);
+        }
+    } else if (item.z == 1) {
         float* ga = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * 2;
         float* gb = a.grad2 + a.param_begin2 + ((size_t)gs.offset[level] + e0) * 2;
         for (int i = threadIdx.x; i < nf2; i += blockDim.x) {
@@ -1037,8 +1051,9 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
 }
 
 // Multi-part slices of the paired records: block (k, y) adds, for slice k and
-// the y-th stripe of its values, the parts' partial sums in part order.
-__global__ __launch_bounds__(256) void grid_bwd_combine_pair_kernel(const GridBucketArgs a) {
+// the y-th stripe of its values, the parts' partial sums in part order (with
+// ga.on: the y-th stripe of its entries, into the optimizer).
+__global__ __launch_bounds__(256) void grid_bwd_combine_pair_kernel(const GridBucketArgs a, const GridAdam ga) {
     if ((int)blockIdx.x >= a.n_items[1]) return;
     const int4 m = a.multi[blockIdx.x];
     const GridSpec& gs = a.ns->grid;
@@ -1050,9 +1065,30 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_pair_kernel(const GridBu
     bool fin_a, fin_b;
     const int Sa = slice_scale(a, a.lvl_max, level, m.x, fin_a);
     const int Sb = slice_scale(a, a.lvl_max2, level, m.x, fin_b);
-    float* ga = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * 2;
-    float* gb = a.grad2 + a.param_begin2 + ((size_t)gs.offset[level] + e0) * 2;
     const unsigned long long* pp = a.partial + (size_t)m.y * kGridSliceFloats;
+    if (ga.on) {
+        // (integer sums: any part order gives the same value)
+        const int ne = nf2 / 2;
+        const int estripe = (ne + gridDim.y - 1) / gridDim.y;
+        const int en0 = blockIdx.y * estripe;
+        const int en1 = min(ne, en0 + estripe);
+        const int64_t E0 = (int64_t)gs.offset[level] + e0;
+        const float nan = __builtin_nanf("");
+        for (int en = en0 + threadIdx.x; en < en1; en += blockDim.x) {
+            unsigned long long q[4] = {0ull, 0ull, 0ull, 0ull};
+            for (int p = 0; p < m.z; ++p) {
+                const unsigned long long* pr = pp + (size_t)p * kGridSliceFloats + en;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) q[r] += pr[r * NE];
+            }
+            const float2 gt = fin_a ? make_float2(unfix64(q[0], Sa), unfix64(q[1], Sa)) : make_float2(nan, nan);
+            const float2 gp = fin_b ? make_float2(unfix64(q[2], Sb), unfix64(q[3], Sb)) : make_float2(nan, nan);
+            grid_adam_pair_entry(ga, E0 + en, gt, gp);
+        }
+        return;
+    }
+    float* ga_ = a.grad + gs.param_begin + ((size_t)gs.offset[level] + e0) * 2;
+    float* gb_ = a.grad2 + a.param_begin2 + ((size_t)gs.offset[level] + e0) * 2;
     const int stripe = (nv + gridDim.y - 1) / gridDim.y;
     const int i0 = blockIdx.y * stripe;
     const int i1 = min(nv, i0 + stripe);
@@ -1070,8 +1106,8 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_pair_kernel(const GridBu
         }
         for (; p < m.z; ++p) q0 += pp[(size_t)p * kGridSliceFloats + j];
         const unsigned long long q = q0 + q1 + q2 + q3;
-        if (!second) ga[o] = fin_a ? unfix64(q, Sa) : __builtin_nanf("");
-        else gb[o] = fin_b ? unfix64(q, Sb) : __builtin_nanf("");
+        if (!second) ga_[o] = fin_a ? unfix64(q, Sa) : __builtin_nanf("");
+        else gb_[o] = fin_b ? unfix64(q, Sb) : __builtin_nanf("");
     }
 }
 
@@ -1124,7 +1160,7 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
 // per corner for both nets), then one reduce and combine writing both
 // gradients.
 void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, const GridBucketPlan& p,
-                                   hipStream_t s) {
+                                   const GridAdam& ga, hipStream_t s) {
     if (a.n <= 0 || !p.ok) return;
     const dim3 g1((unsigned)p.n_chunks, (unsigned)hs.grid.n_levels);
     // int64 slice totals, then both nets' per-level maxima right after them
@@ -1133,8 +1169,8 @@ void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, c
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
     hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
-    hipLaunchKernelGGL(grid_bwd_reduce_pair_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a);
-    hipLaunchKernelGGL(grid_bwd_combine_pair_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_reduce_pair_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a, ga);
+    hipLaunchKernelGGL(grid_bwd_combine_pair_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a, ga);
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,9 +1236,9 @@ struct AdamPair {
     h2 w16, ema;
 };
 
-__device__ __forceinline__ AdamPair adam_pair_load(const AdamArgs& a, int64_t i) {
+__device__ __forceinline__ AdamPair adam_pair_load(const AdamArgs& a, int64_t i, bool with_grad = true) {
     AdamPair p;
-    p.g = *(const float2*)(a.grad + i);
+    p.g = with_grad ? *(const float2*)(a.grad + i) : make_float2(0.f, 0.f);
     p.w = *(const float2*)(a.master + i);
     p.m = *(const float2*)(a.m + i);
     p.v = *(const float2*)(a.v + i);
@@ -1248,6 +1284,24 @@ __device__ __forceinline__ void adam_pair_store(const AdamArgs& a, int64_t i, co
     *(uint2*)(a.steps + i) = p.st;
     *(h2*)(a.w16 + i) = p.w16;
     *(h2*)(a.ema16 + i) = p.ema;
+}
+
+__device__ void grid_adam_pair_entry(const GridAdam& ga, int64_t e, float2 gt, float2 gp) {
+    const int64_t ia = ga.a.grid_begin + 2 * e, ib = ga.b.grid_begin + 2 * e;
+    if (ga.keep_grad) {
+        *(float2*)(const_cast<float*>(ga.a.grad) + ia) = gt;
+        *(float2*)(const_cast<float*>(ga.b.grad) + ib) = gp;
+    }
+    AdamPair pa = adam_pair_load(ga.a, ia, false);
+    AdamPair pb = adam_pair_load(ga.b, ib, false);
+    pa.g = gt;
+    pb.g = gp;
+    adam_pair_step(ga.a, pa);
+    adam_pair_step(ga.b, pb);
+    adam_pair_store(ga.a, ia, pa);
+    adam_pair_store(ga.b, ib, pb);
+    *(h4*)(ga.a.pair_w16 + 4 * e) = h4{pa.w16[0], pa.w16[1], pb.w16[0], pb.w16[1]};
+    *(h4*)(ga.a.pair_ema16 + 4 * e) = h4{pa.ema[0], pa.ema[1], pb.ema[0], pb.ema[1]};
 }
 
 // Grid parameters of the theta (a) and phi (b) nets together, one table

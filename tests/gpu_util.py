@@ -113,3 +113,13 @@ def trained_scale_params(L, rng, gain=2.0, grid_amp=0.5):
         g0 = L["grid"]["param_begin"]
         p[g0:] = rng.uniform(-grid_amp, grid_amp, L["n_params"] - g0)
     return p.astype(np.float16).astype(np.float32)
+
+
+def keep_grads(cfg):
+    """A copy of the config whose context also stores the paired grids'
+    gradient: their optimizer step otherwise consumes it inside the grid
+    backward, and dfg_get_grad then refuses to return it."""
+    import copy
+    c = copy.deepcopy(cfg)
+    c.setdefault("dfg", {})["keep_gradients"] = 1
+    return c

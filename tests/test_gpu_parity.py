@@ -13,7 +13,8 @@ import pytest
 
 import dfguide as G
 import oracle as O
-from gpu_util import df_features, plain_features, uniforms, wi_init, train_targets, dev, host, trained_scale_params
+from gpu_util import (df_features, plain_features, uniforms, wi_init, train_targets, dev, host, trained_scale_params,
+                      keep_grads)
 
 pytestmark = pytest.mark.gpu
 
@@ -381,7 +382,7 @@ def test_train_step_vs_fp64(torch_cuda, cfgname):
     else:
         cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
         D, feat = 9, df_features(1000, 20)
-    f = G.GuidingField(cfg)
+    f = G.GuidingField(keep_grads(cfg))
     th, ph, _ = O.config_layouts(cfg)
     B = feat.shape[0]
     tgt = train_targets(B, 21)
@@ -446,7 +447,7 @@ def test_adam_and_ema_over_steps_vs_fp64(torch_cuda):
     previous EMA and new fp16 weights, to fp16 rounding."""
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
-    f = G.GuidingField(cfg)
+    f = G.GuidingField(keep_grads(cfg))
     th, ph, _ = O.config_layouts(cfg)
     opt = cfg["nn"]["optimizer"]
     ad = opt["nested"]
@@ -490,7 +491,7 @@ def test_adam_and_ema_over_steps_vs_fp64(torch_cuda):
 def test_nrc_train_step_vs_fp64(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
-    f = G.GuidingField(cfg)
+    f = G.GuidingField(keep_grads(cfg))
     _, _, L = O.config_layouts(cfg)
     # targets and batch keep the loss-scaled fp16 gradients inside the half range
     B = 4096
@@ -520,7 +521,7 @@ def test_nrc_fused_training_matches_separate_kernels(torch_cuda, B):
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
     cfg_sep = G.hash_config(with_nrc=True, log2_hashmap_size=12)
     cfg_sep["dfg"]["fused_training"] = 0
-    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg_sep)
+    fa, fb = G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg_sep))
     feat = dev(torch, df_features(B, 36, nrc=True))
     tgt = dev(torch, np.exp(np.random.default_rng(37).normal(0, 1, (B, 3))).astype(np.float32))
     la = torch.zeros(3, dtype=torch.float32, device="cuda")
@@ -578,7 +579,7 @@ def test_df_fused_training_matches_separate_kernels(torch_cuda, cfgname, bins, i
         c["dfg"]["interpolation"] = interp
         c["dfg"]["fused_training"] = fused
         cfgs.append(c)
-    fa, fb = G.GuidingField(cfgs[0]), G.GuidingField(cfgs[1])
+    fa, fb = G.GuidingField(keep_grads(cfgs[0])), G.GuidingField(keep_grads(cfgs[1]))
     th, ph, _ = O.config_layouts(cfgs[0])
     feat = dev(torch, plain_features(B, 46) if cfgname == "plain" else df_features(B, 46))
     tgt = dev(torch, train_targets(B, 47))
@@ -616,6 +617,43 @@ def test_df_fused_training_matches_separate_kernels(torch_cuda, cfgname, bins, i
     fb.close()
 
 
+@pytest.mark.parametrize("B", [4096, 262144])
+def test_grid_optimizer_in_the_backward_keeps_or_drops_only_the_gradient(torch_cuda, B):
+    """The paired grids' Adam + EMA runs inside the grid backward (each slice's
+    fixed-point sums update its entries).  Storing the gradient for
+    dfg_get_grad (keep_gradients) changes nothing else: master, training and
+    EMA weights are bit-identical over three steps with and without it, and
+    without it dfg_get_grad refuses the DF nets (their grid part was never
+    stored) while the radiance cache's gradient stays readable."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=16)
+    fa, fb = G.GuidingField(cfg), G.GuidingField(keep_grads(cfg))
+    feat = dev(torch, df_features(B, 56))
+    tgt = dev(torch, train_targets(B, 57))
+    nfeat = dev(torch, df_features(B, 58, nrc=True))
+    rgb = dev(torch, np.exp(np.random.default_rng(59).normal(0, 1, (B, 3))).astype(np.float32))
+    la = torch.zeros(3, dtype=torch.float32, device="cuda")
+    lb = torch.zeros(3, dtype=torch.float32, device="cuda")
+    for k in range(3):
+        fa.train_step(feat, tgt, la[k:k + 1])
+        fb.train_step(feat, tgt, lb[k:k + 1])
+        fa.nrc_train_step(nfeat, rgb)
+        fb.nrc_train_step(nfeat, rgb)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(la), host(lb))
+    for net in (0, 1, 2):
+        for which in (G.DFG_PARAMS_MASTER, G.DFG_PARAMS_TRAIN, G.DFG_PARAMS_INFERENCE):
+            assert np.array_equal(fa.get_params(net, which), fb.get_params(net, which)), (net, which)
+    for net in (0, 1):
+        with pytest.raises(G.DfgError):
+            fa.get_grad(net)
+        g = fb.get_grad(net)
+        assert np.isfinite(g).all() and np.abs(g).max() > 0
+    assert np.array_equal(fa.get_grad(2), fb.get_grad(2))
+    fa.close()
+    fb.close()
+
+
 @pytest.mark.parametrize("cfgname", ["plain", "hash", "reference"])
 def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
     """The plain FP32 training comparator: fp32 bins, activations and
@@ -631,7 +669,7 @@ def test_plain_train_step_vs_fp64(torch_cuda, cfgname):
     else:
         cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
         D, feat = 9, df_features(1000, 22)
-    f = G.GuidingField(cfg)
+    f = G.GuidingField(keep_grads(cfg))
     th, ph, _ = O.config_layouts(cfg)
     B = feat.shape[0]
     tgt = train_targets(B, 23)
@@ -697,7 +735,7 @@ def test_grid_backward_bucketed_equals_atomic(torch_cuda, cfgname, B):
     import copy
     cfg_a = copy.deepcopy(cfg)
     cfg_a["dfg"]["grid_backward"] = "atomic"
-    fb, fa = G.GuidingField(cfg), G.GuidingField(cfg_a)
+    fb, fa = G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg_a))
     feat, tgt = dev(torch, df_features(B, 60)), dev(torch, train_targets(B, 61))
     fb.train_step(feat, tgt)
     fa.train_step(feat, tgt)
@@ -724,7 +762,7 @@ def test_train_steps_are_deterministic(torch_cuda):
     on the same batches hold bit-identical weights and gradients."""
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=False, log2_hashmap_size=16)
-    fs = [G.GuidingField(cfg) for _ in range(2)]
+    fs = [G.GuidingField(keep_grads(cfg)) for _ in range(2)]
     B = 20000
     for step in range(3):
         feat, tgt = dev(torch, df_features(B, 80 + step)), dev(torch, train_targets(B, 90 + step))
@@ -750,7 +788,7 @@ def test_df_and_nrc_steps_on_two_streams_match_one_stream(torch_cuda, plain):
     and the DF losses are checked finite."""
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=16)
-    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg)
+    fa, fb = G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg))
     B = 20000
     main = torch.cuda.current_stream()
     side = torch.cuda.Stream()
@@ -793,7 +831,7 @@ def test_wgrad_chunk_rows_only_reorders_sums(torch_cuda):
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=15)
     cfg_b = copy.deepcopy(cfg)
     cfg_b["dfg"]["wgrad_chunk_rows"] = 192
-    fa, fb = G.GuidingField(cfg), G.GuidingField(cfg_b)
+    fa, fb = G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg_b))
     B = 5000  # the last 192-row chunk is ragged
     feat = dev(torch, df_features(B, 70, nrc=True))
     tgt = dev(torch, train_targets(B, 71))
@@ -815,7 +853,7 @@ def test_wgrad_chunk_rows_only_reorders_sums(torch_cuda):
 def test_plain_nrc_train_step_vs_fp64(torch_cuda):
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True, log2_hashmap_size=12)
-    f = G.GuidingField(cfg)
+    f = G.GuidingField(keep_grads(cfg))
     _, _, L = O.config_layouts(cfg)
     B = 4096
     feat = df_features(B, 32, nrc=True)
@@ -971,7 +1009,7 @@ def test_full_size_training_properties(torch_cuda):
     cfg = G.hash_config(with_nrc=True)
     cfg_a = copy.deepcopy(cfg)
     cfg_a["dfg"]["grid_backward"] = "atomic"
-    fs = [G.GuidingField(cfg), G.GuidingField(cfg)]
+    fs = [G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg))]
     B = 262144
     loss = torch.zeros((2, 2), dtype=torch.float32, device="cuda")
     for step in range(2):
@@ -989,7 +1027,7 @@ def test_full_size_training_properties(torch_cuda):
         assert np.array_equal(fs[0].get_grad(k), fs[1].get_grad(k)), "net %d gradients differ" % k
         assert np.array_equal(fs[0].get_params(k), fs[1].get_params(k)), "net %d weights differ" % k
     # the grid gradient against the atomic scatter, one step from the same seed
-    fb, fa = G.GuidingField(cfg), G.GuidingField(cfg_a)
+    fb, fa = G.GuidingField(keep_grads(cfg)), G.GuidingField(keep_grads(cfg_a))
     feat = dev(torch, df_features(B, 96, nrc=True))
     tgt = dev(torch, train_targets(B, 97))
     fb.train_step(feat, tgt)

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdfg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["k_query.hip", "k_train.hip", "k_plain.hip", "k_records.hip", "api.cpp", "serve.cpp", "records.cpp", "noalloc.hip", "k_train_fused.hip"]
+SOURCES = ["k_query.hip", "k_train.hip", "k_plain.hip", "k_records.hip", "api.cpp", "serve.cpp", "records.cpp", "noalloc.hip", "k_train_fused.hip", "k_serve.hip"]
 
 # -packed-fp32-ops: keep the compiler from forming packed fp32 VALU ops.
 FLAGS = [

@@ -128,6 +128,62 @@ void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2*
                              int max_blocks = 0);
 void launch_interleave_tables(const _Float16* th, const _Float16* ph, uint2* tp, int64_t n_entries, hipStream_t s);
 void launch_extract_pos_worklist(const WorkList* wl, int pos_dim, int64_t rows, float* pos, hipStream_t s);
+
+// ---- Request server: admission on the device --------------------------------
+// Queued requests live in a ring in host-visible pinned memory: the host
+// fills an entry, then advances `tail` (release).  Every round of the server
+// is one captured graph whose first kernel admits requests from the ring's
+// head on the device (FIFO, within the row budget), builds the round's work
+// list and publishes what it took in the slot's record; its last kernel
+// returns the round's rows to the budget.
+constexpr uint32_t kServerRing = 4096;   // ring entries (a power of two)
+constexpr int kServerMaxSlots = 16;      // rounds in flight at once
+struct ServerReq {                       // one queued request
+    int64_t id;
+    int64_t n;
+    const float* feat;
+    const float* u;
+    const uint8_t* guided;
+    float* wi;
+    float* rgb;
+    int64_t stride;
+};
+struct ServerRecord {                    // per slot: what the slot's latest round admitted
+    uint32_t seq;                        // admissions on this slot so far (published last)
+    uint32_t h0, h1;                     // ring positions [h0, h1) (mod 2^32)
+    uint32_t pad;
+    int64_t rows;
+};
+struct ServerHostShared {                // pinned, mapped: written by both sides
+    uint32_t tail;                       // requests pushed so far (mod 2^32), host-written
+    uint32_t pad[31];
+    ServerRecord rec[kServerMaxSlots];   // device-written
+    ServerReq ring[kServerRing];         // host-written
+};
+struct ServerDevState {                  // device memory
+    unsigned long long word;             // (head << 32) | rows in flight (both < 2^32)
+    unsigned long long peak_rows;        // largest rows in flight right after an admission
+    uint32_t seq[kServerMaxSlots];
+};
+struct AdmitParams {
+    int64_t budget_rows;                 // rows in flight at once (< 2^31)
+    int64_t round_rows;                  // rows of one round (the slot's pre-pass capacity, < 2^31)
+    int max_req;                         // requests of one round (<= DFG_MAX_SEGS)
+    int slot;
+};
+// item_ctr: the round's pre-pass item counters (kXcdGroups, 32 uint32s apart), cleared here.
+void launch_server_admit(ServerHostShared* hs, ServerDevState* dv, WorkList* wl, unsigned int* counter,
+                         unsigned int* item_ctr, const AdmitParams& p, hipStream_t s);
+void launch_server_complete(ServerDevState* dv, const WorkList* wl, hipStream_t s);
+// Positions of a round's rows (device-side row count) into a compact [rows][3] array.
+void launch_extract_pos_round(const WorkList* wl, int pos_dim, int64_t cap_rows, float* pos, hipStream_t s);
+// Paired grid pre-pass over the first *n_dev of cap rows, with the work items
+// sized on the device from *n_dev and taken in order from per-XCD counters
+// (item_ctr[32 * kXcdGroups], every 32nd zero at launch) by a grid of `blocks`
+// workgroups.
+void launch_grid_encode_pair_dev(const NetSpec* dns, const NetSpec& hs, const uint2* tp, const float* pos,
+                                 int64_t cap, int64_t ld, float* out_th, float* out_ph, const int64_t* n_dev,
+                                 unsigned int* item_ctr, int blocks, hipStream_t s);
 void launch_extract_pos(const float* feat, int stride, int pos_dim, int64_t n, float* pos, hipStream_t s,
                         const int64_t* n_dev = nullptr);
 int grid_pos_dim(const NetSpec& hs);

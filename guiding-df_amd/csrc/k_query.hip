@@ -204,8 +204,14 @@ __global__ __launch_bounds__(64, 2) void query_worklist_kernel(const QueryArgs a
         for (int k = 0; k < tiles_per_fetch; ++k) {
             const int64_t tile = (int64_t)t0 + k;
             if (tile >= total) break;
-            int sg = 0;
-            while (sg + 1 < nseg && wl->seg[sg + 1].tile_begin <= tile) ++sg;
+            // the last segment starting at or before the tile (empty segments
+            // share their successor's tile_begin and are skipped)
+            int sg = 0, hi = nseg - 1;
+            while (sg < hi) {
+                const int mid = (sg + hi + 1) >> 1;
+                if (wl->seg[mid].tile_begin <= tile) sg = mid;
+                else hi = mid - 1;
+            }
             const WorkSeg& ws = wl->seg[sg];
             const RowsIO io{ws.feat, ws.stride, ws.u, ws.guided, ws.wi, ws.rgb, ws.n, ws.gbase};
             query_tile<NTH, NPH, true, WITH_NRC>(a, io, (tile - ws.tile_begin) * 64, smem);

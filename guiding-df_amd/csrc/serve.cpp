@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <deque>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "ctx.h"
@@ -245,8 +246,13 @@ int dfg_wavefront_autotune(dfg_wavefront* w, void* stream, int n_bounces, const 
 
 }  // extern "C"
 
+
 // ============================================================================
-// Request server
+// Request server.  Admission runs on the device (k_serve.hip): the host pushes
+// requests into a ring in pinned memory and launches rounds; the first kernel
+// of each round takes what the row budget allows from the ring's head and
+// builds the round's work list there.  The host's part is the ring, launching
+// rounds, the training interleave and the latency records.
 // ============================================================================
 namespace {
 
@@ -266,13 +272,29 @@ struct TrainJob {
     const float* nrc_target;
 };
 
-struct Launch {
-    std::vector<Pending> reqs;
-    int64_t rows = 0;
-    int64_t version = 0;
-    hipEvent_t done = nullptr;
+struct Round {
     int slot = 0;
+    uint32_t seq = 0;        // the slot's admission count this round publishes
+    bool published = false;
+    uint32_t h0 = 0, h1 = 0;  // ring positions the round admitted
+    int64_t rows = 0;
+    int64_t version = 0;      // training steps enqueued before the round
+    hipEvent_t done = nullptr;
 };
+
+// One round in flight at a time per slot; slot i launches on stream i % n_streams.
+struct Slot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t launched = 0;
+    bool busy = false;
+    DevBuf<float> pos, gth, gph;  // the round's positions and paired pre-pass output
+};
+
+constexpr int64_t kRowBytes = 9 * 4 + 16 + 16 + 12;  // feat(>=9 floats) + u + wi + rgb per row
+// per slot: the query's tile counter, then 8 pre-pass item counters, each on a
+// 128-byte line of its own
+constexpr size_t kSlotCounters = 32 + 32 * kXcdGroups;
 
 }  // namespace
 
@@ -280,85 +302,91 @@ struct dfg_server {
     dfg_ctx* ctx = nullptr;
     dfg_server_config cfg{};
     int64_t budget_rows = 0;
+    int64_t round_rows = 0;
     std::vector<hipStream_t> streams;
     hipStream_t train_stream = nullptr;
+    hipStream_t cap_stream = nullptr;
     hipEvent_t origin = nullptr;       // GPU time origin
     Clock::time_point t0;              // host time origin (recorded right after `origin`)
-    DevBuf<WorkList> wls;              // one work list per launch slot
-    DevBuf<unsigned int> counters;
-    WorkList* host_wl = nullptr;       // pinned staging, one per slot
-    std::vector<bool> slot_busy;
-    std::deque<Pending> queue;
+    ServerHostShared* hs = nullptr;    // pinned: ring, tail, admission records (host address)
+    ServerHostShared* hs_dev = nullptr;  // the same memory's device address
+    DevBuf<ServerDevState> dv;
+    DevBuf<WorkList> wls;              // one work list per slot
+    DevBuf<unsigned int> counters;     // per slot: the query's tile counter, then the pre-pass's per-XCD item counters
+    std::vector<Slot> slots;
+    std::deque<Pending> pend;          // submitted, not completed; pend[i] is ring position pend_base + i
+    uint32_t pend_base = 0;
+    uint32_t pushed = 0;               // ring positions written (the tail)
+    uint32_t known_head = 0;           // the head after the newest published admission
     std::deque<TrainJob> train_queue;
-    std::deque<Launch> inflight;
-    int64_t rows_inflight = 0;
-    int64_t launches = 0, train_steps = 0, submitted = 0, completed = 0, rows_done = 0;
-    int next_stream = 0;
+    std::deque<Round> inflight;        // in launch order
+    bool unpublished = false;          // the newest round has not published its admission yet
+    bool blocked = false;              // the newest admission took nothing while requests waited
+    int64_t launches = 0, rounds = 0, train_steps = 0, submitted = 0, completed = 0, rows_done = 0;
     int64_t since_train = 0;
-    hipEvent_t last_train = nullptr;   // queries launched after a train step wait on it
+    hipEvent_t last_train = nullptr;   // rounds launched after a train step wait on it
     std::vector<double> latencies;
     std::vector<int64_t> done_ids, done_versions;
     std::vector<double> done_lat;
     double first_submit = -1.0, last_done = 0.0;
     bool nrc = false;
-    int nslots = 0;
-    // per launch slot: compact positions and the paired grid pre-pass output
-    std::vector<DevBuf<float>> slot_pos, slot_gth, slot_gph;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
     ~dfg_server() {
         for (auto& l : inflight) (void)hipEventSynchronize(l.done);
         for (auto& l : inflight) (void)hipEventDestroy(l.done);
+        if (last_train) (void)hipEventSynchronize(last_train);
+        for (auto& sl : slots) {
+            if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+            if (sl.graph) (void)hipGraphDestroy(sl.graph);
+            sl.pos.release();
+            sl.gth.release();
+            sl.gph.release();
+        }
         for (auto s : streams) (void)hipStreamDestroy(s);
         if (train_stream) (void)hipStreamDestroy(train_stream);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         if (origin) (void)hipEventDestroy(origin);
         if (last_train) (void)hipEventDestroy(last_train);
-        if (host_wl) (void)hipHostFree(host_wl);
+        if (hs) (void)hipHostFree(hs);
+        dv.release();
         wls.release();
         counters.release();
-        for (auto& b : slot_pos) b.release();
-        for (auto& b : slot_gth) b.release();
-        for (auto& b : slot_gph) b.release();
     }
 };
 
 namespace {
 
-constexpr int64_t kRowBytes = 9 * 4 + 16 + 16 + 12;  // feat(>=9 floats) + u + wi + rgb per row
+uint32_t load_acquire(const uint32_t* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
 
-void server_retire(dfg_server* s, bool block) {
-    while (!s->inflight.empty()) {
-        Launch& l = s->inflight.front();
-        if (block) HIPX(hipEventSynchronize(l.done));
-        const hipError_t q = hipEventQuery(l.done);
-        if (q == hipErrorNotReady) return;
-        if (q != hipSuccess) throw Error(DFG_ERR_HIP, std::string("launch failed: ") + hipGetErrorString(q));
-        float ms = 0.0f;
-        HIPX(hipEventElapsedTime(&ms, s->origin, l.done));
-        const double t_done = ms * 1e-3;
-        for (auto& p : l.reqs) {
-            const double lat = (t_done - p.t_submit) * 1e3;
-            s->latencies.push_back(lat);
-            s->done_ids.push_back(p.r.id);
-            s->done_lat.push_back(lat);
-            s->done_versions.push_back(l.version);
-            ++s->completed;
-            s->rows_done += p.r.n;
-        }
-        s->last_done = std::max(s->last_done, t_done);
-        s->rows_inflight -= l.rows;
-        s->slot_busy[l.slot] = false;
-        (void)hipEventDestroy(l.done);
-        s->inflight.pop_front();
+// Writes queued requests into the ring while it has room: positions below the
+// newest published head have been copied into a work list and are free.
+void server_push(dfg_server* s) {
+    const uint32_t end = s->pend_base + (uint32_t)s->pend.size();
+    bool any = false;
+    while (s->pushed != end && s->pushed - s->known_head < kServerRing) {
+        const dfg_request& r = s->pend[s->pushed - s->pend_base].r;
+        ServerReq& q = s->hs->ring[s->pushed & (kServerRing - 1)];
+        q.id = r.id;
+        q.n = r.n;
+        q.feat = r.feat;
+        q.u = r.u;
+        q.guided = r.guided;
+        q.wi = r.wi;
+        q.rgb = r.nrc_rgb;
+        q.stride = r.feat_stride;
+        ++s->pushed;
+        any = true;
     }
+    if (any) __atomic_store_n(&s->hs->tail, s->pushed, __ATOMIC_RELEASE);
 }
 
 void server_train(dfg_server* s) {
     if (s->train_queue.empty()) return;
     TrainJob j = s->train_queue.front();
     s->train_queue.pop_front();
-    // ordered after every launch before it ...
+    // ordered after every round before it ...
     for (auto& l : s->inflight) HIPX(hipStreamWaitEvent(s->train_stream, l.done, 0));
     int rc = dfg_train_step(s->ctx, s->train_stream, j.B, j.feat, j.stride, j.target, nullptr);
     if (rc != DFG_OK) throw Error(rc, s->ctx->err);
@@ -366,92 +394,149 @@ void server_train(dfg_server* s) {
         rc = dfg_nrc_train_step(s->ctx, s->train_stream, j.B, j.nrc_feat, 15, j.nrc_target, nullptr);
         if (rc != DFG_OK) throw Error(rc, s->ctx->err);
     }
-    // ... and before every launch after it
+    // ... and before every round after it
     HIPX(hipEventRecord(s->last_train, s->train_stream));
     ++s->train_steps;
 }
 
-bool server_launch(dfg_server* s) {
-    if (s->queue.empty()) return false;
-    int slot = -1;
-    for (int i = 0; i < s->nslots; ++i)
-        if (!s->slot_busy[i]) { slot = i; break; }
-    if (slot < 0) return false;
-    Launch l;
-    l.slot = slot;
-    WorkList& wl = s->host_wl[slot];
-    wl.n_seg = 0;
-    int64_t tiles = 0;
-    while (!s->queue.empty() && (int)l.reqs.size() < s->cfg.max_batch_requests) {
-        const Pending& p = s->queue.front();
-        const bool fits = s->rows_inflight + l.rows + p.r.n <= s->budget_rows;
-        if (!fits && !(l.reqs.empty() && s->inflight.empty())) break;  // an oversize request runs alone
-        WorkSeg& g = wl.seg[wl.n_seg++];
-        g.feat = p.r.feat;
-        g.stride = p.r.feat_stride;
-        g.u = p.r.u;
-        g.guided = p.r.guided;
-        g.wi = p.r.wi;
-        g.rgb = p.r.nrc_rgb;
-        g.n = p.r.n;
-        g.tile_begin = tiles;
-        g.gbase = l.rows;  // row of the segment in the launch's pre-pass buffers
-        tiles += (p.r.n + 63) / 64;
-        l.rows += p.r.n;
-        l.reqs.push_back(p);
-        s->queue.pop_front();
+// Reads the newest round's admission record once the device has published it.
+void server_poll(dfg_server* s) {
+    if (!s->unpublished) return;
+    Round& l = s->inflight.back();
+    const ServerRecord& rec = s->hs->rec[l.slot];
+    if (load_acquire(&rec.seq) != l.seq) return;
+    l.h0 = rec.h0;
+    l.h1 = rec.h1;
+    l.rows = rec.rows;
+    l.published = true;
+    s->unpublished = false;
+    if (l.h0 != s->known_head) throw Error(DFG_ERR_HIP, "server admission out of order");
+    s->known_head = l.h1;
+    const uint32_t k = l.h1 - l.h0;
+    s->blocked = k == 0;
+    if (k) {
+        ++s->rounds;
+        // a training step is due after every `train_every` query batches; it
+        // is enqueued right behind the round that completes the count
+        s->since_train += k;
+        while (s->cfg.train_every > 0 && s->since_train >= s->cfg.train_every && !s->train_queue.empty()) {
+            server_train(s);
+            s->since_train -= s->cfg.train_every;
+        }
     }
-    if (l.reqs.empty()) return false;
-    wl.total_tiles = tiles;
-    wl.total_rows = l.rows;
-    const int64_t n_reqs = (int64_t)l.reqs.size();
-    hipStream_t st = s->streams[s->next_stream];
-    s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
-    if (s->train_steps > 0) HIPX(hipStreamWaitEvent(st, s->last_train, 0));
-    WorkList* dwl = s->wls.p + slot;
-    const size_t bytes = offsetof(WorkList, seg) + sizeof(WorkSeg) * (size_t)wl.n_seg;
-    HIPX(hipMemcpyAsync(dwl, &wl, bytes, hipMemcpyHostToDevice, st));
-    // counter reset by a fill kernel (graph-safe), then the persistent launch
-    unsigned int* ctr = s->counters.p + slot;
-    launch_fill_u32(ctr, 0u, 1, st);
-    QueryArgs a = base_args(s->ctx);
-    a.feat = l.reqs[0].r.feat;
-    a.stride = l.reqs[0].r.feat_stride;
-    a.n = l.rows;
-    a.rgb = s->nrc ? l.reqs[0].r.nrc_rgb : nullptr;
+    server_push(s);
+}
+
+// Waits (bounded) until the newest round has published its admission.
+void server_wait_published(dfg_server* s) {
+    const auto limit = Clock::now() + std::chrono::seconds(30);
+    while (s->unpublished) {
+        server_poll(s);
+        if (!s->unpublished) break;
+        if (Clock::now() > limit) throw Error(DFG_ERR_HIP, "server round did not publish its admission");
+        std::this_thread::yield();
+    }
+}
+
+void server_retire(dfg_server* s, bool block) {
+    while (!s->inflight.empty()) {
+        Round& l = s->inflight.front();
+        if (block) HIPX(hipEventSynchronize(l.done));
+        const hipError_t q = hipEventQuery(l.done);
+        if (q == hipErrorNotReady) return;
+        if (q != hipSuccess) throw Error(DFG_ERR_HIP, std::string("round failed: ") + hipGetErrorString(q));
+        if (!l.published) server_poll(s);  // finished rounds have published (their first kernel did)
+        if (!l.published) throw Error(DFG_ERR_HIP, "finished round without an admission record");
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, s->origin, l.done));
+        const double t_done = ms * 1e-3;
+        for (uint32_t pos = l.h0; pos != l.h1; ++pos) {
+            if (s->pend.empty() || s->pend_base != pos) throw Error(DFG_ERR_HIP, "server retired out of order");
+            const Pending& p = s->pend.front();
+            const double lat = (t_done - p.t_submit) * 1e3;
+            s->latencies.push_back(lat);
+            s->done_ids.push_back(p.r.id);
+            s->done_lat.push_back(lat);
+            s->done_versions.push_back(l.version);
+            ++s->completed;
+            s->rows_done += p.r.n;
+            s->pend.pop_front();
+            ++s->pend_base;
+        }
+        if (l.h1 != l.h0) s->last_done = std::max(s->last_done, t_done);
+        s->slots[l.slot].busy = false;
+        s->blocked = false;  // rows went back to the budget
+        (void)hipEventDestroy(l.done);
+        s->inflight.pop_front();
+    }
+}
+
+// Captures the slot's round: admission, positions, paired pre-pass, the
+// persistent work-list query and the completion, all sized on the device.
+void server_capture(dfg_server* s, int slot) {
     dfg_ctx* c = s->ctx;
-    if (c->pair_ok && l.rows >= c->prepass_min_rows) {
-        // the XCD-aware paired grid pre-pass over every row of the launch
-        const int64_t ld = round_up(l.rows, 64);
+    Slot& sl = s->slots[slot];
+    WorkList* dwl = s->wls.p + slot;
+    unsigned int* ctr = s->counters.p + (size_t)slot * kSlotCounters;
+    unsigned int* item_ctr = ctr + 32;
+    const int64_t R = s->round_rows;
+    const int64_t ld = round_up(R, 64);
+    QueryArgs a = base_args(c);
+    a.n = R;
+    a.rgb = s->nrc ? s->pend.front().r.nrc_rgb : nullptr;  // selects the NRC variant; rows come from the work list
+    if (c->pair_ok) {
         const size_t cols = (size_t)c->net[0].spec.grid.n_levels * 2;
-        s->slot_pos[slot].ensure((size_t)l.rows * 3);
-        s->slot_gth[slot].ensure(cols * ld);
-        s->slot_gph[slot].ensure(cols * ld);
-        launch_extract_pos_worklist(dwl, grid_pos_dim(c->net[0].spec), l.rows, s->slot_pos[slot].p, st);
-        launch_grid_encode_pair(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, s->slot_pos[slot].p, l.rows, ld,
-                                s->slot_gth[slot].p, s->slot_gph[slot].p, st);
-        a.gfeat_th = s->slot_gth[slot].p;
-        a.gfeat_ph = s->slot_gph[slot].p;
+        sl.pos.ensure((size_t)R * 3);
+        sl.gth.ensure(cols * ld);
+        sl.gph.ensure(cols * ld);
+        a.gfeat_th = sl.gth.p;
+        a.gfeat_ph = sl.gph.p;
         a.gld = ld;
     }
     fill_lds_strides(c, a, s->nrc, true);
-    const int64_t want_blocks = std::min<int64_t>(tiles, (int64_t)cu_count() * 4);
-    launch_query_worklist(a, s->ctx->n_theta, s->ctx->n_phi, dwl, ctr, (int)std::max<int64_t>(1, want_blocks), 1, st);
-    check_launch();
+    AdmitParams p{s->budget_rows, R, s->cfg.max_batch_requests, slot};
+    hipStream_t st = s->cap_stream;
+    HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_server_admit(s->hs_dev, s->dv.p, dwl, ctr, item_ctr, p, st);
+    if (c->pair_ok) {
+        launch_extract_pos_round(dwl, grid_pos_dim(c->net[0].spec), R, sl.pos.p, st);
+        launch_grid_encode_pair_dev(c->net[0].d_spec.p, c->net[0].spec, c->tab_pair.p, sl.pos.p, R, ld, sl.gth.p,
+                                    sl.gph.p, &dwl->total_rows, item_ctr, cu_count() * 8, st);
+    }
+    // 8 workgroups (waves) per CU pulling 2 tiles at a time: the autotuner's
+    // choice for full-size frames of the same work-list kernel
+    launch_query_worklist(a, c->n_theta, c->n_phi, dwl, ctr, cu_count() * 8, 2, st);
+    launch_server_complete(s->dv.p, dwl, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) throw Error(DFG_ERR_HIP, std::string("server graph capture: ") + hipGetErrorString(e));
+    sl.graph = g;
+    HIPX(hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0));
+}
+
+bool server_launch(dfg_server* s) {
+    if (s->unpublished || s->blocked) return false;
+    server_push(s);
+    if (s->pushed == s->known_head) return false;  // nothing waiting in the ring
+    int slot = -1;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+        if (!s->slots[i].busy) { slot = i; break; }
+    if (slot < 0) return false;
+    Slot& sl = s->slots[slot];
+    if (!sl.exec) server_capture(s, slot);
+    hipStream_t st = s->streams[slot % s->streams.size()];
+    if (s->train_steps > 0) HIPX(hipStreamWaitEvent(st, s->last_train, 0));
+    HIPX(hipGraphLaunch(sl.exec, st));
+    Round l;
+    l.slot = slot;
+    l.seq = ++sl.launched;
+    l.version = s->train_steps;
     HIPX(hipEventCreateWithFlags(&l.done, 0));
     HIPX(hipEventRecord(l.done, st));
-    l.version = s->train_steps;
-    s->rows_inflight += l.rows;
-    s->slot_busy[slot] = true;
-    s->inflight.push_back(std::move(l));
+    sl.busy = true;
+    s->unpublished = true;
+    s->inflight.push_back(l);
     ++s->launches;
-    // a training step is due after every `train_every` query batches; it is
-    // enqueued right behind the launch that completes the count
-    s->since_train += n_reqs;
-    while (s->cfg.train_every > 0 && s->since_train >= s->cfg.train_every && !s->train_queue.empty()) {
-        server_train(s);
-        s->since_train -= s->cfg.train_every;
-    }
     return true;
 }
 
@@ -468,24 +553,28 @@ int dfg_server_create(dfg_ctx* ctx, const dfg_server_config* cfg, dfg_server** o
         throw Error(DFG_ERR_BAD_SIZE, "max_batch_requests must be 1..256");
     if (cfg->n_streams < 1 || cfg->n_streams > 4) throw Error(DFG_ERR_BAD_SIZE, "n_streams must be 1..4");
     if (cfg->train_every < 0) throw Error(DFG_ERR_BAD_SIZE, "train_every < 0");
+    if (cfg->round_rows < 0 || cfg->round_rows > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "round_rows out of range");
     std::unique_ptr<dfg_server> s(new dfg_server());
     s->ctx = ctx;
     s->cfg = *cfg;
-    s->budget_rows = cfg->budget_bytes / kRowBytes;
+    s->budget_rows = std::min<int64_t>(cfg->budget_bytes / kRowBytes, ((int64_t)1 << 31) - 1);
+    s->round_rows = cfg->round_rows > 0 ? cfg->round_rows
+                                        : std::max<int64_t>((int64_t)1 << 20, std::min<int64_t>(s->budget_rows, (int64_t)1 << 22));
     for (int i = 0; i < cfg->n_streams; ++i) {
         hipStream_t st;
         HIPX(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         s->streams.push_back(st);
     }
     HIPX(hipStreamCreateWithFlags(&s->train_stream, hipStreamNonBlocking));
-    s->nslots = cfg->n_streams * 2;
-    s->slot_pos.resize(s->nslots);
-    s->slot_gth.resize(s->nslots);
-    s->slot_gph.resize(s->nslots);
-    s->wls.ensure(s->nslots);
-    s->counters.ensure(s->nslots);
-    HIPX(hipHostMalloc((void**)&s->host_wl, sizeof(WorkList) * s->nslots, hipHostMallocDefault));
-    s->slot_busy.assign(s->nslots, false);
+    HIPX(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking));
+    s->slots.resize((size_t)cfg->n_streams * 2);
+    s->wls.ensure(s->slots.size());
+    s->counters.ensure(s->slots.size() * kSlotCounters);
+    s->dv.ensure(1);
+    HIPX(hipMemset(s->dv.p, 0, sizeof(ServerDevState)));
+    HIPX(hipHostMalloc((void**)&s->hs, sizeof(ServerHostShared), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(s->hs, 0, sizeof(ServerHostShared));
+    HIPX(hipHostGetDevicePointer((void**)&s->hs_dev, s->hs, 0));
     HIPX(hipEventCreate(&s->last_train));
     HIPX(hipEventCreate(&s->origin));
     HIPX(hipDeviceSynchronize());
@@ -507,7 +596,7 @@ int dfg_server_submit(dfg_server* s, const dfg_request* r) {
     dfg_ctx* ctx = s->ctx;
     API_BEGIN
     if (!r) throw Error(DFG_ERR_INVALID_ARG, "null request");
-    if (r->n < 1 || r->n > ((int64_t)1 << 30)) throw Error(DFG_ERR_BAD_SIZE, "request size out of range");
+    if (r->n < 1 || r->n > s->round_rows) throw Error(DFG_ERR_BAD_SIZE, "request size out of range (1..round_rows)");
     if (!r->feat || !r->u || !r->wi) throw Error(DFG_ERR_INVALID_ARG, "null request buffer");
     if (r->feat_stride < ctx->D) throw Error(DFG_ERR_BAD_SIZE, "feat_stride below n_input_dims");
     const bool wants_nrc = r->nrc_rgb != nullptr;
@@ -517,8 +606,10 @@ int dfg_server_submit(dfg_server* s, const dfg_request* r) {
     s->nrc = wants_nrc;
     const double t = s->now();
     if (s->first_submit < 0) s->first_submit = t;
-    s->queue.push_back(Pending{*r, t});
+    s->pend.push_back(Pending{*r, t});
     ++s->submitted;
+    // visible to a round whose admission runs from now on
+    server_push(s);
     API_END(ctx)
 }
 
@@ -538,11 +629,11 @@ int dfg_server_progress(dfg_server* s, int wait) {
     dfg_ctx* ctx = s->ctx;
     API_BEGIN
     for (;;) {
+        server_poll(s);
         server_retire(s, false);
-        while (server_launch(s)) {
-        }
+        if (server_launch(s)) continue;
         if (!wait) break;
-        if (s->queue.empty() && s->inflight.empty()) {
+        if (s->pend.empty() && s->inflight.empty()) {
             if (s->train_steps > 0) {  // the busy span ends with the last training step too
                 HIPX(hipEventSynchronize(s->last_train));
                 float ms = 0.0f;
@@ -551,7 +642,14 @@ int dfg_server_progress(dfg_server* s, int wait) {
             }
             break;
         }
-        if (!s->inflight.empty()) server_retire(s, true);
+        if (s->unpublished) {
+            server_wait_published(s);
+        } else if (!s->inflight.empty()) {
+            server_retire(s, true);
+        } else {
+            // nothing in flight: the whole budget is free, so the next round admits
+            s->blocked = false;
+        }
     }
     API_END(ctx)
 }
@@ -562,7 +660,7 @@ int dfg_server_get_stats(dfg_server* s, dfg_server_stats* out) {
     std::memset(out, 0, sizeof(*out));
     out->submitted = s->submitted;
     out->completed = s->completed;
-    out->launches = s->launches;
+    out->launches = s->rounds;
     out->train_steps = s->train_steps;
     out->rows = s->rows_done;
     if (!s->latencies.empty()) {
@@ -581,6 +679,10 @@ int dfg_server_get_stats(dfg_server* s, dfg_server_stats* out) {
         out->max_ms = v.back();
     }
     out->busy_s = s->first_submit >= 0 ? std::max(0.0, s->last_done - s->first_submit) : 0.0;
+    unsigned long long peak = 0;
+    if (hipMemcpy(&peak, &s->dv.p->peak_rows, sizeof(peak), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(s->ctx, DFG_ERR_HIP, "server stats: device read failed");
+    out->peak_rows = (int64_t)peak;
     return DFG_OK;
 }
 

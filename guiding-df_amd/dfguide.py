@@ -70,7 +70,7 @@ class DfgWavefrontTuning(ctypes.Structure):
 
 class DfgServerConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("max_batch_requests", ctypes.c_int),
-                ("n_streams", ctypes.c_int), ("train_every", ctypes.c_int)]
+                ("n_streams", ctypes.c_int), ("train_every", ctypes.c_int), ("round_rows", ctypes.c_int64)]
 
 
 class DfgRequest(ctypes.Structure):
@@ -83,7 +83,7 @@ class DfgServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("train_steps", ctypes.c_int64), ("rows", ctypes.c_int64), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
-                ("busy_s", ctypes.c_double)]
+                ("busy_s", ctypes.c_double), ("peak_rows", ctypes.c_int64)]
 
 
 RECORD_FIELDS = ("cur_depth", "pos", "dir", "wo", "normal", "roughness", "L", "thp", "bsdf_val", "diffuse",
@@ -510,12 +510,14 @@ class Wavefront:
 
 
 class Server:
-    """Continuous-batching request server over one GuidingField."""
+    """Continuous-batching request server over one GuidingField (admission on
+    the device; round_rows 0 picks the library's default)."""
 
-    def __init__(self, field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8):
+    def __init__(self, field, budget_bytes=16 << 30, max_batch_requests=64, n_streams=2, train_every=8,
+                 round_rows=0):
         self.field = field
         self.lib = field.lib
-        cfg = DfgServerConfig(budget_bytes, max_batch_requests, n_streams, train_every)
+        cfg = DfgServerConfig(budget_bytes, max_batch_requests, n_streams, train_every, round_rows)
         h = ctypes.c_void_p()
         field._c(self.lib.dfg_server_create(field.ctx, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -415,15 +415,20 @@ int dfg_wavefront_autotune(dfg_wavefront* w, void* stream, int n_bounces, const 
                            dfg_wavefront_tuning* best);
 
 /* ---- Served loop: continuous batching of guidance queries ----------------
- * Requests (device-resident rows) are queued by dfg_server_submit; every
- * dfg_server_progress admits queued requests while the rows in flight stay
- * within the memory budget, packs all admitted ones into one persistent
- * work-list launch on the next of `n_streams` streams (so a batch that
- * arrives while another runs overlaps it on the device), interleaves a queued
- * training step after every `train_every` query batches (requests; enqueued
- * behind the launch that completes the count, ordered after the launches
- * before it and before the ones after it), and retires finished launches,
- * timing each request from submission to the completion event of its launch.
+ * dfg_server_submit puts a request (device-resident rows) into a request ring
+ * in pinned host memory.  dfg_server_progress launches rounds, each one
+ * captured graph on the next free slot (two per stream, `n_streams` streams),
+ * and retires finished ones.  Admission runs on the device: the first kernel
+ * of a round takes requests from the ring's head, in arrival order, while the
+ * rows in flight (a device counter that every round's last kernel gives back)
+ * stay within the memory budget and the round within `round_rows`, and builds
+ * the round's work list there; the round then runs the grid pre-pass and one
+ * persistent work-list query over every admitted row.  A round started while
+ * another runs overlaps it on the device and admits what arrived meanwhile.
+ * A queued training step is interleaved after every `train_every` query
+ * batches (requests; enqueued behind the round that completes the count,
+ * ordered after the rounds before it and before the ones after it).  Each
+ * request is timed from submission to the completion event of its round.
  * Each row's result is identical whether its request ran alone or in a mix.
  * No counterpart in the reference (its queries are one batch per bounce). */
 typedef struct dfg_server dfg_server;
@@ -433,6 +438,9 @@ typedef struct dfg_server_config {
     int max_batch_requests; /* requests packed into one launch (<= 256)             */
     int n_streams;          /* launch streams (1..4)                                */
     int train_every;        /* query batches between training steps (0: never)     */
+    int64_t round_rows;     /* rows of one round, the largest request accepted; also
+                               the per-slot pre-pass capacity (0: max(2^20,
+                               min(budget rows, 2^22)))                              */
 } dfg_server_config;
 
 typedef struct dfg_request {
@@ -450,6 +458,7 @@ typedef struct dfg_server_stats {
     int64_t submitted, completed, launches, train_steps, rows;
     double p50_ms, p99_ms, mean_ms, max_ms;  /* request latency, submit -> done */
     double busy_s;                            /* first submit -> last completion */
+    int64_t peak_rows;                        /* most rows in flight after an admission */
 } dfg_server_stats;
 
 int dfg_server_create(dfg_ctx* ctx, const dfg_server_config* cfg, dfg_server** out);

@@ -101,9 +101,9 @@ def _requests(torch, n_req, seed, nrc=False, lo=1024, hi=40000):
                                                                 (3, 120000, 1 << 30), (2, 1 << 22, 1024),
                                                                 (3, 120000, 1024)])
 def test_server_alone_equals_mix(torch_cuda, n_streams, budget_rows, prepass_min):
-    """Every request equals the same request queried alone, with in-kernel grid
-    gathers (prepass_min huge) and with the paired grid pre-pass over the
-    packed launch (prepass_min 1024: alone and packed both use it)."""
+    """Every request equals the same request queried alone (alone: in-kernel
+    grid gathers with prepass_min huge, the paired pre-pass with prepass_min
+    1024; the server's rounds always run the pre-pass over their rows)."""
     torch = torch_cuda
     cfg = G.hash_config(with_nrc=True)
     cfg["dfg"]["grid_prepass_min_rows"] = prepass_min
@@ -133,6 +133,49 @@ def test_server_alone_equals_mix(torch_cuda, n_streams, budget_rows, prepass_min
     for i in alone:
         assert np.array_equal(host(alone[i][0]), host(mixed[i][0])), "request %d differs alone vs in the mix" % i
         assert np.array_equal(host(alone[i][1]), host(mixed[i][1]))
+    srv.close()
+    f.close()
+
+
+def test_server_ring_wraps_and_budget_holds(torch_cuda):
+    """5,000 small requests (more than the 4,096-entry request ring) through
+    device-side admission under a 3,000-row budget: every request completes,
+    each row equals one dfg_query over all rows (requests are slices of one
+    buffer), and the rows in flight after any admission stay within the budget
+    except for a single request larger than it, which runs alone."""
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False, log2_hashmap_size=14))
+    rng = np.random.default_rng(21)
+    sizes = rng.integers(1, 200, 5000)
+    sizes[1234] = 4000                       # above the budget: admitted alone
+    total = int(sizes.sum())
+    feat, u = df_features(total, 22), uniforms(total, 23)
+    d_feat, d_u = dev(torch, feat), dev(torch, u)
+    ref = dev(torch, wi_init(total))
+    f.query(d_feat, d_u, ref)
+    budget_rows = 3000
+    srv = G.Server(f, budget_bytes=budget_rows * 80, max_batch_requests=256, n_streams=2, train_every=0,
+                   round_rows=1 << 16)
+    out = dev(torch, wi_init(total))
+    row = 0
+    for i, n in enumerate(sizes.tolist()):
+        srv.submit(i, d_feat[row:row + n], d_u[row:row + n], out[row:row + n])
+        row += n
+        if i % 500 == 0:
+            srv.progress()
+    srv.progress(wait=True)
+    st = srv.stats()
+    assert st["completed"] == len(sizes) and st["rows"] == total
+    assert 0 < st["peak_rows"] <= max(budget_rows, int(sizes.max())), st["peak_rows"]
+    ids, lat, _ = srv.drain(max_records=len(sizes))
+    assert sorted(ids.tolist()) == list(range(len(sizes)))
+    torch.cuda.synchronize()
+    assert np.array_equal(host(out), host(ref))
+    # a request above round_rows is refused at submission
+    big = 1 << 16
+    with pytest.raises(G.DfgError):
+        srv.submit(99999, dev(torch, df_features(big + 1, 24)), dev(torch, uniforms(big + 1, 25)),
+                   dev(torch, wi_init(big + 1)))
     srv.close()
     f.close()
 

@@ -149,6 +149,15 @@ __device__ __forceinline__ uint32_t grid_index3(int type, uint32_t hsize, uint32
     return index % hsize;
 }
 
+// floor(v) as an int for any float: NaN -> 0, beyond the int range saturated
+// (what v_cvt_i32_f32 does; written out so that no value is undefined in C++
+// and the host restatement takes the same one).
+__device__ __forceinline__ int floor_to_int(float v) {
+    const float f = floorf(v);
+    if (!(f == f)) return 0;
+    return f >= 2147483648.0f ? 2147483647 : (f < -2147483648.0f ? (-2147483647 - 1) : (int)f);
+}
+
 // pos_fract, tcnn common_device.h:418-431 (identity or smoothstep), with the
 // position p = in * scale + 0.5 taken exactly instead of rounded to fp32.
 // Rounded, p carries up to half an ulp of error (2^-6 at scale 2^19), which
@@ -170,7 +179,7 @@ __device__ __forceinline__ void pos_fract(float in, float scale, int interp, flo
         t = t - 1.0f;
         p = __builtin_fmaf(in, scale, 0.5f - t);
     }
-    cell = (uint32_t)(int)t;
+    cell = (uint32_t)floor_to_int(t);  // t is integral (or not finite)
     if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0f - 2.0f * p);
     frac = p;
 }
@@ -748,22 +757,27 @@ __device__ __forceinline__ float sample_nearest(const Segment& sg, float u, floa
 }
 
 // sample_pdf, train.h:654-681 (x is the square coordinate; idx = x * N).
+// Total for every float x: on [0, 1] (and above it) the reference's indices;
+// below the first bin centre's half-bin the reference's i1 runs negative and
+// reads out of bounds, here it stays 0 (the constant end half-bin extends);
+// NaN gives bin 0 with a NaN weight, i.e. a NaN pdf.
 template <int N, typename Bin>
 __device__ __forceinline__ float pdf_linear(float x, Bin P) {
 #pragma clang fp contract(off)
     float idx = x * (float)N;
     idx -= 0.5f;
-    int i0 = min((int)floorf(idx), N - 1);
+    int i0 = min(floor_to_int(idx), N - 1);
     int i1 = i0 + 1;
     float w = idx - (float)i0;
     if (i0 < 0) { i0 = N - 1; w = 1.0f; }
     if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+    i1 = max(i1, 0);
     return (1.0f - w) * P(i0) + w * P(i1);
 }
 
 template <int N, typename Bin>
 __device__ __forceinline__ float pdf_nearest(float x, Bin P) {
-    int i = min((int)floorf(x * (float)N), N - 1);
+    int i = min(floor_to_int(x * (float)N), N - 1);
     i = max(i, 0);  // the reference reads out of bounds for x < 0; clamp instead
     return P(i);
 }
@@ -772,7 +786,7 @@ __device__ __forceinline__ float pdf_nearest(float x, Bin P) {
 __device__ __forceinline__ float theta_input(float theta, int n_theta, int linear) {
 #pragma clang fp contract(off)
     const float t = theta * (float)n_theta;
-    const int fi = min((int)floorf(t), n_theta - 1);
+    const int fi = min(floor_to_int(t), n_theta - 1);
     if (linear) {
         const float w = t - (float)fi;
         return ((float)fi + w) / (float)n_theta;
@@ -812,17 +826,18 @@ struct DfLogitGrad {
     template <typename Prob>
     __device__ __forceinline__ DfLogitGrad(float x, int lin, Prob P) : linear(lin) {
 #pragma clang fp contract(off)
-        if (linear) {
+        if (linear) {  // the indices of pdf_linear (total for every float x)
             const float idx = x * (float)N - 0.5f;
-            i0 = min((int)floorf(idx), N - 1);
+            i0 = min(floor_to_int(idx), N - 1);
             i1 = i0 + 1;
             w = idx - (float)i0;
             if (i0 < 0) { i0 = N - 1; w = 1.0f; }
             if (i1 > N - 1) { i1 = 0; w = 0.0f; }
+            i1 = max(i1, 0);
             p0 = P(i0);
             p1 = P(i1);
         } else {
-            i0 = min((int)(x * (float)N), N - 1);
+            i0 = min(floor_to_int(x * (float)N), N - 1);
             i0 = max(i0, 0);
             i1 = 0;
             w = 0.0f;

@@ -185,6 +185,15 @@ uint32_t grid_index3(int type, uint32_t hsize, uint32_t res, uint32_t x, uint32_
     return index % hsize;
 }
 
+// floor(v) as an int for every value: NaN -> 0, beyond the int range
+// saturated (the device's conversion; the plain cast would be undefined).
+template <typename R>
+int floor_to_int(R v) {
+    const R f = std::floor(v);
+    if (!(f == f)) return 0;
+    return f >= (R)2147483648.0 ? 2147483647 : (f < (R)-2147483648.0 ? (-2147483647 - 1) : (int)f);
+}
+
 // common_device.h:418-431.  The position p = in * scale + 0.5 is taken
 // exactly in both modes (the reference rounds it to fp32, which costs up to
 // half an ulp, 2^-6 at scale 2^19, of the fine levels' weights):
@@ -200,7 +209,7 @@ template <>
 void pos_fract<double>(double in, double scale, int interp, double& frac, uint32_t& cell) {
     double p = in * scale + 0.5;
     const double t = std::floor(p);
-    cell = (uint32_t)(int)t;
+    cell = (uint32_t)floor_to_int(t);
     p = p - t;
     if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0 - 2.0 * p);
     frac = p;
@@ -214,7 +223,7 @@ void pos_fract<float>(float in, float scale, int interp, float& frac, uint32_t& 
         t = t - 1.0f;
         p = fmaf(in, scale, 0.5f - t);
     }
-    cell = (uint32_t)(int)t;
+    cell = (uint32_t)floor_to_int(t);
     if (interp == INTERP_SMOOTHSTEP) p = p * p * (3.0f - 2.0f * p);
     frac = p;
 }
@@ -477,21 +486,25 @@ R sample_nearest(int N, const Seg<R>& sg, R u, R u_extra) {
     return std::fmin(((R)sg.i + w) / (R)N, (R)1 - eps);
 }
 
+// sample_pdf, train.h:654-681.  The reference's indices on [0, 1] and above;
+// below the first bin centre's half-bin its i1 runs negative (an out-of-bounds
+// read), here i1 stays 0; NaN x gives a NaN pdf.
 template <typename R>
-R pdf_linear(const R* p, int N, R x) {  // sample_pdf, train.h:654-681
+R pdf_linear(const R* p, int N, R x) {
     R idx = x * (R)N;
     idx -= (R)0.5;
-    int i0 = std::min((int)std::floor(idx), N - 1);
+    int i0 = std::min(floor_to_int(idx), N - 1);
     int i1 = i0 + 1;
     R w = idx - (R)i0;
     if (i0 < 0) { i0 = N - 1; w = 1; }
     if (i1 > N - 1) { i1 = 0; w = 0; }
+    if (i1 < 0) i1 = 0;
     return ((R)1 - w) * p[i0] + w * p[i1];
 }
 
 template <typename R>
 R pdf_nearest(const R* p, int N, R x) {
-    int i = std::min((int)std::floor(x * (R)N), N - 1);
+    int i = std::min(floor_to_int(x * (R)N), N - 1);
     if (i < 0) i = 0;
     return p[i];
 }
@@ -499,7 +512,7 @@ R pdf_nearest(const R* p, int N, R x) {
 template <typename R>
 R theta_input(R th, int N, int linear) {  // add_theta_to_input_F, train.h:793-812
     const R t = th * (R)N;
-    const int fi = std::min((int)std::floor(t), N - 1);
+    const int fi = std::min(floor_to_int(t), N - 1);
     if (linear) return ((R)fi + (t - (R)fi)) / (R)N;
     return (R)fi / (R)N;
 }
@@ -720,17 +733,18 @@ static void dpdf_dz(const double* p, int N, double x, int linear, double* out) {
     const double inv = 1.0 / N;
     if (linear) {
         const double idx = x * N - 0.5;
-        int i0 = std::min((int)std::floor(idx), N - 1), i1 = i0 + 1;
+        int i0 = std::min(floor_to_int(idx), N - 1), i1 = i0 + 1;
         double w = idx - i0;
         if (i0 < 0) { i0 = N - 1; w = 1; }
         if (i1 > N - 1) { i1 = 0; w = 0; }
+        if (i1 < 0) i1 = 0;
         const double s0 = p[i0] * inv, s1 = p[i1] * inv;
         for (int j = 0; j < N; ++j) {
             const double sj = p[j] * inv;
             out[j] = N * ((1 - w) * s0 * ((j == i0) - sj) + w * s1 * ((j == i1) - sj));
         }
     } else {
-        int i0 = std::min((int)std::floor(x * N), N - 1);
+        int i0 = std::min(floor_to_int(x * N), N - 1);
         if (i0 < 0) i0 = 0;
         const double s0 = p[i0] * inv;
         for (int j = 0; j < N; ++j) out[j] = N * s0 * ((j == i0) - p[j] * inv);

@@ -871,6 +871,108 @@ def test_plain_nrc_train_step_vs_fp64(torch_cuda):
 
 
 # ----------------------------------------------------------------------------
+# Inputs outside the unit interval and non-finite inputs
+# ----------------------------------------------------------------------------
+_ODD_COORDS = np.array([-np.inf, -1e30, -5.0, -0.7, -1e-6, 0.0, 0.5, 1.0, 1.3, 7.0, 1e30, np.inf, np.nan], np.float32)
+
+
+@pytest.mark.parametrize("nt,np_,linear", [(32, 32, 1), (64, 64, 1), (32, 32, 0)])
+def test_head_on_given_coordinates_outside_the_unit_interval(torch_cuda, nt, np_, linear):
+    """Unguided rows take the caller's square coordinates.  Every float gives
+    the oracle's FP32 pdf bit for bit: the end half-bins extend beyond [0, 1]
+    (the reference's indices would read outside the bins there), infinities
+    take the end bins, NaN gives NaN (linear) or bin 0 (nearest)."""
+    torch = torch_cuda
+    n = 4096
+    rng = np.random.default_rng(70 + nt + linear)
+    lt = rng.normal(0.0, 1.5, (n, nt)).astype(np.float32)
+    lp = rng.normal(0.0, 1.5, (n, np_)).astype(np.float32)
+    u = rng.random((n, 4), dtype=np.float32)
+    guided = (rng.random(n) < 0.3).astype(np.uint8)
+    wi = np.ones((n, 4), np.float32)
+    wi[:, 0] = rng.choice(_ODD_COORDS, n)
+    wi[:, 1] = rng.choice(_ODD_COORDS, n)
+    d_wi = dev(torch, wi)
+    G.df_head(dev(torch, lt), dev(torch, lp), dev(torch, u), d_wi, guided=dev(torch, guided), interpolation=linear)
+    torch.cuda.synchronize()
+    g = host(d_wi)
+    x0, p0, _, _ = O.head(lt, linear, u[:, 0], u[:, 2], sample=guided, x_given=wi[:, 0], fp32=True)
+    x1, p1, _, _ = O.head(lp, linear, u[:, 1], u[:, 3], sample=guided, x_given=wi[:, 1], fp32=True)
+    assert np.array_equal(g[:, 0], x0, equal_nan=True) and np.array_equal(g[:, 1], x1, equal_nan=True)
+    assert np.array_equal(g[:, 2], p0, equal_nan=True) and np.array_equal(g[:, 3], p1, equal_nan=True)
+    # only an unguided NaN coordinate gives a NaN pdf in linear mode (nearest
+    # mode reads bin 0 for it, as the reference's float-to-int conversion does)
+    nan_in = np.isnan(wi[:, 0]) & (guided == 0)
+    assert np.all(np.isfinite(g[~nan_in, 2]))
+    assert np.all(np.isnan(g[nan_in, 2])) if linear else np.all(np.isfinite(g[nan_in, 2]))
+
+
+def test_query_rows_outside_the_unit_cube_and_non_finite_rows(torch_cuda):
+    """Positions outside [0, 1]^3 (the grid cells wrap as the reference's
+    uint32 arithmetic does) match the FP64 oracle like in-range rows; rows with
+    NaN or infinite features run without touching memory outside the tables
+    and leave every other row bit-identical to a query without them."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True, log2_hashmap_size=16)
+    f = G.GuidingField(cfg)
+    th, ph, nrc = O.config_layouts(cfg)
+    n = 4096
+    rng = np.random.default_rng(91)
+    feat = df_features(n, 92, nrc=True)
+    u = uniforms(n, 93)
+    out = np.arange(0, n, 7)
+    feat[out, 0:3] = rng.choice(np.array([-2.5, -0.4, 1.6, 3.7, 15.0], np.float32), (len(out), 3))
+    bad = np.arange(3, n, 11)
+    bad = bad[~np.isin(bad, out)]
+    feat[bad, rng.integers(0, 15, len(bad))] = rng.choice(np.array([np.nan, np.inf, -np.inf], np.float32), len(bad))
+    clean = np.setdiff1d(np.arange(n), bad)
+    wi0 = wi_init(n)
+    d_wi, d_rgb = dev(torch, wi0), torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    f.query(dev(torch, feat), dev(torch, u), d_wi, nrc_rgb=d_rgb)
+    ref_wi, ref_rgb = dev(torch, wi0[clean]), torch.zeros((len(clean), 3), dtype=torch.float32, device="cuda")
+    f.query(dev(torch, feat[clean]), dev(torch, u[clean]), ref_wi, nrc_rgb=ref_rgb)
+    torch.cuda.synchronize()
+    g, grgb = host(d_wi), host(d_rgb)
+    assert np.array_equal(g[clean], host(ref_wi)) and np.array_equal(grgb[clean], host(ref_rgb))
+    o = O.query_f64(th, ph, f.get_params(0, G.DFG_PARAMS_INFERENCE), f.get_params(1, G.DFG_PARAMS_INFERENCE),
+                    feat[out], 9, u[out], wi0[out], 1)
+    rel = _rel(g[out, 2] * g[out, 3], o[:, 2] * o[:, 3])
+    assert np.all(np.isfinite(g[out])) and np.median(rel) <= 1e-3, np.median(rel)
+    f.close()
+
+
+def test_train_step_on_targets_outside_the_unit_square(torch_cuda):
+    """Training targets whose directions lie outside [0, 1]^2 (a caller's
+    records, not the library's own) train on the extended end half-bins: the
+    loss matches the FP64 oracle with the reference's fp16 bins."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+    f = G.GuidingField(cfg)
+    th, ph, _ = O.config_layouts(cfg)
+    B = 1000
+    feat = df_features(B, 94)
+    tgt = train_targets(B, 95)
+    odd = np.arange(0, B, 5)
+    tgt[odd, 0] = np.random.default_rng(96).choice(np.array([-3.0, -0.02, 1.01, 2.5, -1e30, 1e30], np.float32), len(odd))
+    tgt[odd, 1] = np.random.default_rng(97).choice(np.array([-0.5, 1.7, 40.0], np.float32), len(odd))
+    wtrain = {k: f.get_params(k, G.DFG_PARAMS_TRAIN) for k in (0, 1)}
+    d_loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    f.train_step(dev(torch, feat), dev(torch, tgt), d_loss)
+    torch.cuda.synchronize()
+    loss_g = float(host(d_loss)[0])
+    loss_h, _, _ = O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat, 9, tgt, 1, grads=False, half_bins=True)
+    # the batch loss is a mean of per-row terms of both signs: the tolerance
+    # scales with the mean |term| (of a sample of rows, the odd ones included)
+    rows = np.arange(0, B, 10)
+    scale = np.mean([abs(O.df_train_f64(th, ph, wtrain[0], wtrain[1], feat[i:i + 1], 9, tgt[i:i + 1], 1,
+                                        grads=False, half_bins=True)[0]) for i in rows])
+    assert np.isfinite(loss_g) and abs(loss_g - loss_h) <= 2e-3 * max(abs(loss_h), scale) + 1e-6, (loss_g, loss_h)
+    for k in (0, 1):
+        assert np.all(np.isfinite(f.get_params(k, G.DFG_PARAMS_MASTER)))
+    f.close()
+
+
+# ----------------------------------------------------------------------------
 # Sizes, errors, formats
 # ----------------------------------------------------------------------------
 @pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000])

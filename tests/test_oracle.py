@@ -359,6 +359,35 @@ def test_pdf_integrates_to_one_and_end_bins_constant():
         assert abs(f((3 + 0.5) / N) - bins[0, 3]) < 1e-12     # value at a bin centre
 
 
+def test_pdf_is_total_beyond_the_unit_interval():
+    """The pdf of a given square coordinate is defined for every float: the
+    constant end half-bins extend beyond [0, 1] (the reference's own indices
+    read out of bounds below the first bin centre's half-bin), infinities
+    take the end bins and NaN gives NaN; the nearest mode clamps too."""
+    rng = np.random.default_rng(31)
+    for N in (16, 32, 64):
+        _, _, bins, _ = O.head(rng.normal(0, 2, (1, N)), 1, np.array([0.5]), np.array([0.5]))
+        f = lambda x: O.lib().orc_pdf_linear_f64(bins[0].ctypes.data_as(O.ctypes.c_void_p), N, x)
+        for x in (-1e-9, -0.3 / N, -1.0 / N, -0.7, -5.0, -1e30, -np.inf):
+            assert f(x) == bins[0, 0], x
+        for x in (1.0, 1.0 + 0.3 / N, 1.7, 1e30, np.inf):
+            assert f(x) == bins[0, -1], x
+        assert np.isnan(f(np.nan))
+        # inside the interval: unchanged (interior linear interpolation)
+        x = (5 + 0.5 + 0.25) / N   # a quarter of the way from bin 5's centre to bin 6's
+        assert abs(f(x) - (0.75 * bins[0, 5] + 0.25 * bins[0, 6])) < 1e-12
+    # the head with given coordinates (unguided rows), FP64 and FP32 modes
+    N = 32
+    z = rng.normal(0, 1.5, (8, N))
+    xg = np.array([-2.0, -1e-6, 0.0, 0.5, 1.0, 3.0, np.inf, -np.inf])
+    for fp32 in (False, True):
+        for linear in (0, 1):
+            x, pdf, bins, _ = O.head(z, linear, np.zeros(8), np.zeros(8), sample=np.zeros(8, np.uint8),
+                                     x_given=xg, fp32=fp32)
+            assert np.all(np.isfinite(pdf)) and np.all(pdf > 0)
+            assert pdf[0] == bins[0, 0] and pdf[1] == bins[1, 0] and pdf[6] == bins[6, -1] and pdf[7] == bins[7, 0]
+
+
 def test_samples_follow_the_pdf_chi2():
     rng = np.random.default_rng(13)
     N = 16

@@ -653,10 +653,12 @@ QueryArgs dfg::base_args(dfg_ctx* c) {
 
 // XCD-aware grid pre-pass of both DF nets for n rows (when worth a launch).
 // Compact copy of the grid positions (dims pos_dim..pos_dim+2) for the pre-pass.
-static const float* compact_pos(dfg_ctx* c, const float* feat, int stride, int pos_dim, int64_t n, hipStream_t st) {
-    c->pos.ensure((size_t)n * 3);
-    launch_extract_pos(feat, stride, pos_dim, n, c->pos.p, st);
-    return c->pos.p;
+// buf: the context's query positions or the training step's own.
+static const float* compact_pos(dfg::DevBuf<float>& buf, const float* feat, int stride, int pos_dim, int64_t n,
+                                hipStream_t st) {
+    buf.ensure((size_t)n * 3);
+    launch_extract_pos(feat, stride, pos_dim, n, buf.p, st);
+    return buf.p;
 }
 
 // Rebuilds both interleaved tables from the nets' own (after parameters were
@@ -742,7 +744,7 @@ static bool query_chunked(dfg_ctx* c, const QueryArgs& base, int64_t n, hipStrea
     c->net[1].gfeat.ensure(cols * ld);
     float* gth = c->net[0].gfeat.p;
     float* gph = c->net[1].gfeat.p;
-    const float* pos = compact_pos(c, base.feat, base.stride, grid_pos_dim(c->net[0].spec), n, st);
+    const float* pos = compact_pos(c->pos, base.feat, base.stride, grid_pos_dim(c->net[0].spec), n, st);
     // the side stream starts after everything queued so far on the caller's
     // stream (an earlier query may still read the feature buffers)
     HIPX(hipEventRecord(c->pipe_start, st));
@@ -951,7 +953,7 @@ static void net_forward_train(dfg_ctx* c, NetState& s, const float* feat, int st
     if (s.spec.grid.present) {
         if (!grid_done) {
             s.gfeat.ensure((size_t)s.spec.grid.n_levels * s.spec.grid.n_feat * ld);
-            const float* pos = compact_pos(c, feat, stride, grid_pos_dim(s.spec), n, st);
+            const float* pos = compact_pos(c->train_pos, feat, stride, grid_pos_dim(s.spec), n, st);
             launch_grid_encode(s.d_spec.p, s.spec, s.tab_train(), pos, n, ld, s.gfeat.p, st);
         }
         a.gfeat = s.gfeat.p;
@@ -968,7 +970,7 @@ static bool train_grid_pair(dfg_ctx* c, const float* feat, int stride, int64_t n
     const size_t cols = (size_t)th.spec.grid.n_levels * 2;
     th.gfeat.ensure(cols * ld);
     ph.gfeat.ensure(cols * ld);
-    const float* pos = compact_pos(c, feat, stride, grid_pos_dim(th.spec), n, st);
+    const float* pos = compact_pos(c->train_pos, feat, stride, grid_pos_dim(th.spec), n, st);
     launch_grid_encode_pair(th.d_spec.p, th.spec, c->tab_pair_train.p, pos, n, ld, th.gfeat.p, ph.gfeat.p, st);
     return true;
 }
@@ -1268,7 +1270,7 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
             for (NetState* s : {&th, &ph}) {
                 if (!s->spec.grid.present) continue;
                 s->gfeat.ensure((size_t)s->spec.grid.n_levels * s->spec.grid.n_feat * ld);
-                const float* pos = compact_pos(ctx, feat, feat_stride, grid_pos_dim(s->spec), B, st);
+                const float* pos = compact_pos(ctx->train_pos, feat, feat_stride, grid_pos_dim(s->spec), B, st);
                 launch_grid_encode(s->d_spec.p, s->spec, s->tab_train(), pos, B, ld, s->gfeat.p, st);
             }
         }
@@ -1313,8 +1315,8 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
             throw Error(DFG_ERR_HIP, "fused DF training launch failed");
         launch_wgrad_reduce(th.partial.p, blocks, (int64_t)th.spec.n_matrix_params, th.grad.p, st);
         launch_wgrad_reduce(ph.partial.p, blocks, (int64_t)ph.spec.n_matrix_params, ph.grad.p, st);
-        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, 1024.0f, st)) {
-            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
+        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->train_pos.p, B, ld, 1024.0f, st)) {
+            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->train_pos.p, B, ld, st)) {
                 net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
                 net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
             }
@@ -1353,9 +1355,9 @@ int dfg_train_step(dfg_ctx* ctx, void* stream, int64_t B, const float* feat, int
         // both MLP backwards, then one grid backward for the two grids
         net_backward_mlp(th, B, ld, n_chunks, chunk, st);
         net_backward_mlp(ph, B, ld, n_chunks, chunk, st);
-        // the compact positions of train_grid_pair are still in ctx->pos
-        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, 1024.0f, st)) {
-            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->pos.p, B, ld, st)) {
+        // the compact positions of train_grid_pair are still in ctx->train_pos
+        if (!grid_backward_update_pair(ctx, feat, feat_stride, ctx->train_pos.p, B, ld, 1024.0f, st)) {
+            if (!grid_backward_pair(ctx, feat, feat_stride, ctx->train_pos.p, B, ld, st)) {
                 net_grid_backward(ctx, th, feat, feat_stride, B, ld, st);
                 net_grid_backward(ctx, ph, feat, feat_stride, B, ld, st);
             }

@@ -122,6 +122,11 @@ struct dfg_ctx {
     // plain-path and loss workspace
     DevBuf<float> p_enc, p_a, p_b, p_theta, loss;
     DevBuf<float> tmp, pos, mean_ws;
+    // The training step's compact grid positions: the bucketed grid backward
+    // sizes its slices from them in one kernel and scatters by them in
+    // another, so they must not be a buffer a query also writes (queries use
+    // `pos`).
+    DevBuf<float> train_pos;
     DevBuf<float> train_pdf;  // fused DF training: [2][ld] |pdf| of theta and phi per row
     // training rows generated from records, and their per-step losses
     DevBuf<float> rec_feat, rec_tgt, rec_loss;
@@ -163,7 +168,7 @@ struct dfg_ctx {
         gb_items.release(); gb_n_items.release(); gb_part.release(); gb_multi.release();
         for (auto& n : net) n.release_all();
         p_enc.release(); p_a.release(); p_b.release(); p_theta.release(); loss.release(); tmp.release();
-        pos.release(); mean_ws.release(); train_pdf.release();
+        pos.release(); train_pos.release(); mean_ws.release(); train_pdf.release();
         rec_feat.release(); rec_tgt.release(); rec_loss.release(); rec_local.release(); rec_blk.release();
         rec_count.release(); rec_vidx.release(); img_rows.release(); img_wi2.release();
         n_loss.release(); n_pos.release(); n_mean_ws.release(); n_gb_val.release(); n_gb_offs.release();

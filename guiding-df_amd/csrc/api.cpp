@@ -402,6 +402,10 @@ int dfg_create(const char* json_config, dfg_ctx** out_ctx) {
             HIPX(hipGetDevice(&dev));
             HIPX(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             if (cus > 0) c->n_cus = cus;
+            int lds = 0;
+            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+                lds > 0)
+                c->lds_per_cu = lds;
             c->device = dev;
             c->noalloc_scratch = noalloc_scratch_setup(dev);
         }
@@ -578,7 +582,7 @@ int dfg_load(dfg_ctx* ctx, const char* prefix) {
 }  // extern "C"
 
 // --- query family ---------------------------------------------------------------
-void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df) {
+void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df, int tiles_per_block) {
     // DF rows in LDS: the K-steps not read straight from a grid pre-pass buffer
     int ks = 0;
     for (int i = 0; i < 2; ++i) {
@@ -600,7 +604,23 @@ void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool w
     // phi's nested parts after the grid that equal theta's (the same inputs,
     // widths and columns: SH and OneBlob of the same direction/normal) keep
     // theta's columns: the logits then go to the row's tail, past them, and
-    // phi encodes only its remaining parts (TriangleWave of theta)
+    // phi encodes only its remaining parts (TriangleWave of theta).  The
+    // longer row is taken only while the CU still holds at least 8 tiles
+    // (2 waves per SIMD) of the kernel's workgroups: with 32 bins (row 69
+    // floats) the 4-tile query keeps 2 workgroups per CU and gains; with 64
+    // bins the row would grow to 101 floats and drop the query to 1 workgroup
+    // per CU (1M queries, 64/64 bins: 0.85 ms with the shared columns, 0.70
+    // without; 6-bounce frame 4.50 -> 4.22 ms).
+    auto tile_bytes = [&](int xs, int ls) {
+        int64_t b = (int64_t)64 * (xs | 1) * 4;
+        b = std::max<int64_t>(b, (int64_t)64 * (ls | 1) * 4);
+        b = std::max<int64_t>(b, (int64_t)64 * a.xs_nrc * 2);
+        return (b + 15) / 16 * 16;
+    };
+    auto tiles_per_cu = [&](int xs, int ls) {
+        const int tpb = std::max(1, tiles_per_block);
+        return (c->lds_per_cu / (tpb * tile_bytes(xs, ls))) * tpb;
+    };
     const NetSpec& th = c->net[0].spec;
     const NetSpec& ph = c->net[1].spec;
     if (with_df && a.gfeat_th && a.gfeat_ph && th.grid_ksteps > 0 && th.grid_ksteps == ph.grid_ksteps &&
@@ -617,10 +637,14 @@ void dfg::fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool w
         if (same) {
             const EncPart& last = th.parts[th.n_parts - 1];
             const int end = last.out_begin + last.n_out - 32 * th.grid_ksteps;  // shared columns of the row
-            a.lo_off = (end + 3) / 4 * 4;
-            a.xs = std::max(a.xs, a.lo_off + ls4 + 4);
-            a.ls = a.xs;  // the logits share the input row's stride
-            a.ph_part0 = th.n_parts;
+            const int lo_off = (end + 3) / 4 * 4;
+            const int xs = std::max(a.xs, lo_off + ls4 + 4);
+            if (tiles_per_cu(xs, xs) >= 8) {
+                a.lo_off = lo_off;
+                a.xs = xs;
+                a.ls = a.xs;  // the logits share the input row's stride
+                a.ph_part0 = th.n_parts;
+            }
         }
     }
     // one row per lane: the lanes' same-column accesses (encoder stores, the

@@ -108,6 +108,7 @@ struct dfg_ctx {
     bool keep_gradients = false;       // store the paired grids' gradient although their optimizer step
                                        // runs inside the grid backward (for dfg_get_grad)
     int n_cus = 256;
+    int64_t lds_per_cu = 160 * 1024;  // LDS of one CU (bytes), from the device
     int device = 0;                    // the device current at dfg_create (weights and workspace live there)
     void* noalloc_scratch = nullptr;   // that device's address of the allocation-free query's scratch array
     hipStream_t side = nullptr;
@@ -190,7 +191,9 @@ struct NrcScratch {
 namespace dfg {
 int fail(dfg_ctx* c, int code, const std::string& msg);
 QueryArgs base_args(dfg_ctx* c);
-void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df);
+// tiles_per_block: 64-row tiles per workgroup of the kernel the strides are for
+// (4: query_fused_kernel; 1: the work-list kernel).
+void fill_lds_strides(const dfg_ctx* c, QueryArgs& a, bool with_nrc, bool with_df, int tiles_per_block = 4);
 void grid_prepass(dfg_ctx* c, QueryArgs& a, int64_t n, const float* feat, int stride, hipStream_t st);
 // Element counts (floats) of the grid pre-pass buffers of an n-row query:
 // compact positions [n][3] and each DF net's features [columns][ld].

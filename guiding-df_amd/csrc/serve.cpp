@@ -97,7 +97,7 @@ static void wavefront_enqueue(dfg_wavefront* w, hipStream_t st, int n_bounces, c
         }
         a.gld = ld;
     }
-    fill_lds_strides(c, a, false, true);
+    fill_lds_strides(c, a, false, true, 1);
     const int blocks = std::max(1, cu_count() * std::max(1, w->tune.blocks_per_cu));
     launch_query_worklist(a, c->n_theta, c->n_phi, w->wl.p, w->counter.p, blocks, std::max(1, w->tune.tiles_per_fetch),
                           st);
@@ -493,7 +493,7 @@ void server_capture(dfg_server* s, int slot) {
         a.gfeat_ph = sl.gph.p;
         a.gld = ld;
     }
-    fill_lds_strides(c, a, s->nrc, true);
+    fill_lds_strides(c, a, s->nrc, true, 1);
     AdmitParams p{s->budget_rows, R, s->cfg.max_batch_requests, slot};
     hipStream_t st = s->cap_stream;
     HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

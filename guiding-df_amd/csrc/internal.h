@@ -127,7 +127,6 @@ void launch_grid_encode_pair(const NetSpec* dns, const NetSpec& hs, const uint2*
                              int64_t ld, float* out_th, float* out_ph, hipStream_t s, const int64_t* n_dev = nullptr,
                              int max_blocks = 0);
 void launch_interleave_tables(const _Float16* th, const _Float16* ph, uint2* tp, int64_t n_entries, hipStream_t s);
-void launch_extract_pos_worklist(const WorkList* wl, int pos_dim, int64_t rows, float* pos, hipStream_t s);
 
 // ---- Request server: admission on the device --------------------------------
 // Queued requests live in a ring in host-visible pinned memory: the host

@@ -395,29 +395,6 @@ __global__ void extract_pos_kernel(const float* __restrict__ feat, int stride, i
     pos[i] = feat[r * stride + pos_dim + (i - r * 3)];
 }
 
-// Positions of a packed work list (segments at row offsets gbase) into one
-// compact [rows][3] array for the grid pre-pass.
-__global__ void extract_pos_worklist_kernel(const WorkList* __restrict__ wl, int pos_dim, int64_t rows,
-                                            float* __restrict__ pos) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rows) return;
-    int s = 0;
-    while (s + 1 < wl->n_seg && wl->seg[s + 1].gbase <= g) ++s;
-    const WorkSeg& ws = wl->seg[s];
-    const int64_t r = g - ws.gbase;
-    if (r >= ws.n) return;
-    const float* f = ws.feat + r * ws.stride + pos_dim;
-    pos[3 * g + 0] = f[0];
-    pos[3 * g + 1] = f[1];
-    pos[3 * g + 2] = f[2];
-}
-
-void launch_extract_pos_worklist(const WorkList* wl, int pos_dim, int64_t rows, float* pos, hipStream_t s) {
-    if (rows <= 0) return;
-    hipLaunchKernelGGL(extract_pos_worklist_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, wl,
-                       pos_dim, rows, pos);
-}
-
 int grid_pos_dim(const NetSpec& hs) {
     for (int i = 0; i < hs.n_parts; ++i)
         if (hs.parts[i].type == ENC_GRID) return hs.parts[i].dim_begin;

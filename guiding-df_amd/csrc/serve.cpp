@@ -515,6 +515,9 @@ void server_capture(dfg_server* s, int slot) {
 }
 
 bool server_launch(dfg_server* s) {
+    // with nothing in flight the whole budget is free and a round admits at
+    // least its first request
+    if (s->inflight.empty()) s->blocked = false;
     if (s->unpublished || s->blocked) return false;
     server_push(s);
     if (s->pushed == s->known_head) return false;  // nothing waiting in the ring
@@ -647,8 +650,7 @@ int dfg_server_progress(dfg_server* s, int wait) {
         } else if (!s->inflight.empty()) {
             server_retire(s, true);
         } else {
-            // nothing in flight: the whole budget is free, so the next round admits
-            s->blocked = false;
+            break;  // unreachable: with nothing in flight server_launch takes the next request
         }
     }
     API_END(ctx)

@@ -51,6 +51,49 @@ def measured_traffic():
     return None
 
 
+def profiled_query_kernels():
+    """Average durations (us) of the three kernels of one 1,048,576-query
+    dfg_query call, from the rocprofv3 kernel-trace summary in profiles/ (the
+    library launches them internally, so events in this process see only the
+    whole call); None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "query_1M_and_train_kernel_trace.txt")
+    try:
+        with open(path) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    import re
+    m = re.search(r"extract_pos ([0-9.]+) us, grid_encode_pair \(theta\+phi\) ([0-9.]+) us, query_fused ([0-9.]+) us", text)
+    if not m:
+        return None
+    return {"extract_pos_kernel": float(m.group(1)), "grid_encode_pair_kernel": float(m.group(2)),
+            "query_fused_kernel": float(m.group(3))}
+
+
+def kernel_rooflines(n, fl_q):
+    """Per-kernel roofline fractions of one 1,048,576-query call from the
+    profiled kernel durations.  Algorithmic work per query: extract_pos reads
+    the 3 position floats of a 60-byte feature row and writes 12 B; the grid
+    pre-pass gathers 2 grids x 16 levels x 8 corners x 4 B = 1,024 B, reads
+    12 B of positions and writes 2 x 32 fp32 features (256 B); the fused
+    kernel performs the MLP FLOPs and moves 120 B of IO + 256 B of features."""
+    us = profiled_query_kernels()
+    if us is None or n != 1 << 20:
+        return None
+    out = []
+    for name, b in (("extract_pos_kernel", 60 + 12), ("grid_encode_pair_kernel", 1024 + 12 + 256)):
+        gbs = b * n / (us[name] * 1e-6) / 1e9
+        out.append({"kernel": name, "us": us[name], "bound": "hbm", "bytes_per_query": b, "achieved": gbs,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS})
+    tf = fl_q * n / (us["query_fused_kernel"] * 1e-6) / 1e12
+    out.append({"kernel": "query_fused_kernel", "us": us["query_fused_kernel"], "bound": "mfma",
+                "flops_per_query": fl_q, "achieved": tf, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / MFMA_F16_PEAK_TFLOPS,
+                "hbm_frac": (120 + 256) * n / (us["query_fused_kernel"] * 1e-6) / 1e9 / HBM_PEAK_GBS})
+    return {"source": "profiles/query_1M_and_train_kernel_trace.txt (rocprofv3 averages, same build)",
+            "kernels": out}
+
+
 def pdf_error(field, cfg, feat, u, feat_h, u_h, k=4096):
     """Accuracy of the fused and the plain FP32 query against the FP64 oracle
     on the first k queries, with the field's current inference weights.
@@ -398,7 +441,8 @@ def main():
                                "query_fused_kernel<32,32,true,true>",
                      "kernel_ms": kernel_ms,
                      "bytes_per_query": bytes_q, "flops_per_query": fl_q,
-                     "mfma_tflops": achieved_tf, "mfma_frac": achieved_tf / MFMA_F16_PEAK_TFLOPS},
+                     "mfma_tflops": achieved_tf, "mfma_frac": achieved_tf / MFMA_F16_PEAK_TFLOPS,
+                     "per_kernel": kernel_rooflines(n, fl_q)},
         "cpu_baseline": cpu,
         "wavefront_frame": wave,
         "served_loop": served,

@@ -309,7 +309,8 @@ static NetSpec build_net(const Json& enc, const Json& net, int in_dims, int n_ou
 
 using namespace dfg;
 
-static std::string g_create_error = "no error";
+// the error of the calling thread's last failed dfg_create (no context to hold it)
+static thread_local std::string g_create_error = "no error";
 
 int dfg::fail(dfg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;

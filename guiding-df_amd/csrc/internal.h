@@ -108,6 +108,9 @@ struct WorkList {
     int64_t total_rows;
     WorkSeg seg[DFG_MAX_SEGS];
 };
+// Raises a kernel's dynamic-LDS limit above the 64 KB default on the current
+// device (once per kernel, device and size; thread-safe); false if refused.
+bool allow_dynamic_lds(const void* fn, size_t lds);
 void launch_query_worklist(const QueryArgs& a, int n_theta, int n_phi, const WorkList* wl, unsigned int* counter,
                            int blocks, int tiles_per_fetch, hipStream_t s);
 // Frame set-up on the device: segment b = rows of bounce b packed back to back

@@ -250,37 +250,48 @@ __global__ void worklist_setup_kernel(WorkList* __restrict__ wl, const int64_t* 
 
 
 
-// Dynamic LDS available to one workgroup on the current device.
+// Dynamic LDS available to one workgroup on the current device (cached per
+// device).
 static size_t max_block_lds() {
-    static int cached = 0;
-    if (!cached) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-            cached = v;
-        else
-            cached = 65536;
-    }
-    return (size_t)cached;
+    static std::mutex mu;
+    static std::vector<std::pair<int, size_t>> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& c : cache)
+        if (c.first == dev) return c.second;
+    int v = 0;
+    const size_t lds =
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0 ? (size_t)v
+                                                                                                          : 65536;
+    cache.emplace_back(dev, lds);
+    return lds;
 }
 
 // Lifts a kernel's dynamic-LDS limit to `lds` bytes when that exceeds the
-// default 64 KB (once per kernel and size; host-side cache).  A refusal
-// leaves the limit as it was, and the launch that follows then fails, which
-// the caller's launch check reports.
-static void allow_lds(const void* fn, size_t lds) {
-    if (lds <= 65536) return;
+// default 64 KB.  The limit is an attribute of the kernel on one device, so
+// it is raised once per (kernel, device, size) (host-side cache, thread-safe).
+// Returns false when the runtime refuses; the limit then stays as it was and
+// a launch above it fails.
+bool allow_dynamic_lds(const void* fn, size_t lds) {
+    if (lds <= 65536) return true;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
     static std::mutex mu;
-    static std::vector<std::pair<const void*, size_t>> done;
+    struct Raised { const void* fn; int dev; size_t lds; };
+    static std::vector<Raised> done;
     std::lock_guard<std::mutex> lock(mu);
-    size_t* have = nullptr;
+    Raised* have = nullptr;
     for (auto& d : done)
-        if (d.first == fn) have = &d.second;
-    if (have && *have >= lds) return;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return;
-    if (have) *have = lds;
-    else done.emplace_back(fn, lds);
+        if (d.fn == fn && d.dev == dev) have = &d;
+    if (have && have->lds >= lds) return true;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return false;
+    if (have) have->lds = lds;
+    else done.push_back(Raised{fn, dev, lds});
+    return true;
 }
+
+static void allow_lds(const void* fn, size_t lds) { (void)allow_dynamic_lds(fn, lds); }
 
 template <int NTH, int NPH>
 static void launch_q(const QueryArgs& a, hipStream_t s) {

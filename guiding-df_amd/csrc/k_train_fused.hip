@@ -490,16 +490,10 @@ size_t train_fused_lds_bytes(const NetSpec& hs, bool df, bool fwd_only) {
 
 namespace {
 
-// one instantiation: raise its dynamic LDS limit once, then launch
+// one instantiation: raise its dynamic LDS limit (once per device), then launch
 template <int H, int NT_OUT, int LOSS, int N0>
 bool launch_fused(const TrainFusedArgs& a, size_t lds, int blocks, hipStream_t s) {
-    static size_t attr = 0;
-    if (attr < lds) {
-        if (hipFuncSetAttribute((const void*)train_fused_kernel<H, NT_OUT, LOSS, N0>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-            return false;
-        attr = lds;
-    }
+    if (!allow_dynamic_lds((const void*)train_fused_kernel<H, NT_OUT, LOSS, N0>, lds)) return false;
     hipLaunchKernelGGL((train_fused_kernel<H, NT_OUT, LOSS, N0>), dim3((unsigned)blocks), dim3(256), lds, s, a);
     return true;
 }

@@ -822,6 +822,45 @@ def test_df_and_nrc_steps_on_two_streams_match_one_stream(torch_cuda, plain):
     fb.close()
 
 
+def test_two_contexts_on_two_streams_match_sequential(torch_cuda):
+    """One context per stream is the supported concurrency: two contexts
+    (different seeds) with a training step of one and queries of the other
+    enqueued on two streams at once end bit-identical to the same calls made
+    one after the other (no device state is shared between contexts)."""
+    torch = torch_cuda
+    cfgs = [G.hash_config(with_nrc=True, log2_hashmap_size=16, seed=s) for s in (11, 12)]
+    B, n = 65536, 200000
+    feat = dev(torch, df_features(n, 140, nrc=True))
+    u = dev(torch, uniforms(n, 141))
+    tf = dev(torch, df_features(B, 142, nrc=True))
+    tt = dev(torch, train_targets(B, 143))
+
+    def run(concurrent):
+        fq, ft = G.GuidingField(cfgs[0]), G.GuidingField(cfgs[1])
+        sq = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
+        st = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
+        wis = [dev(torch, wi_init(n)) for _ in range(3)]
+        rgbs = [torch.zeros((n, 3), dtype=torch.float32, device="cuda") for _ in range(3)]
+        loss = torch.zeros(3, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        for k in range(3):
+            fq.query(feat, u, wis[k], nrc_rgb=rgbs[k], stream=sq)
+            ft.train_step(tf, tt, loss[k:k + 1], stream=st)
+        torch.cuda.synchronize()
+        out = ([host(w) for w in wis], [host(r) for r in rgbs], host(loss),
+               [ft.get_params(k, which) for k in (0, 1) for which in (0, 1, 2)])
+        fq.close()
+        ft.close()
+        return out
+
+    a, b = run(False), run(True)
+    for x, y in zip(a[0] + a[1], b[0] + b[1]):
+        assert np.array_equal(x, y), "query results differ"
+    assert np.array_equal(a[2], b[2]), "losses differ"
+    for x, y in zip(a[3], b[3]):
+        assert np.array_equal(x, y), "trained parameters differ"
+
+
 def test_wgrad_chunk_rows_only_reorders_sums(torch_cuda):
     """Another weight-gradient chunk size changes only the order of the fp32
     sums: the matrix gradients agree to 1e-5 (relative norm) and the grid

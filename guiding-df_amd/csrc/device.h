@@ -894,9 +894,22 @@ __device__ __forceinline__ _Float16 round_half(float v) {
 // fp16 conversion that saturates at +-65504 instead of overflowing to inf.
 // Loss-scaled gradients of small batches can exceed the half range; the
 // reference's plain conversion would turn them into inf and the backward into
-// NaN.  NaN inputs stay NaN.
+// NaN.  (A NaN input comes out as -65504, fmaxf dropping the NaN: the loss
+// heads pass their gradients through loss_grad first, so none arrives here
+// from a loss.)
 __device__ __forceinline__ _Float16 sat_h(float v) {
     return (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f);
+}
+
+// A loss head's gradient for one output: a value that is not finite (its
+// row's loss term is not: e.g. a row whose logits all lie below -104, where
+// the softmax with its maximum started at 0 (train.h:846) underflows to a zero
+// sum, log 0, +inf bins and a NaN pdf, as the reference's does) contributes
+// nothing.  The reference's NaN would reach every weight of the net through
+// the backward and Adam; the row's loss value keeps the reference's NaN/inf,
+// so the step's loss still shows it.
+__device__ __forceinline__ float loss_grad(float g) {
+    return fabsf(g) <= 3.402823466e+38f ? g : 0.0f;
 }
 
 // max(v, 0) as one integer max on the bit pattern: non-negative floats order

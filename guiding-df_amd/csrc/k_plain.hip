@@ -240,14 +240,14 @@ __global__ void plain_df_loss_kernel(const float* __restrict__ target, const flo
         float gt[NTH];
         df_logit_grad<NTH>(x0, prefix * app * (float)NTH, linear, [&](int k) { return zt[k] * inv_res; }, gt);
 #pragma unroll
-        for (int i = 0; i < NTH; ++i) g_th[(size_t)i * ld + row] = gt[i];
+        for (int i = 0; i < NTH; ++i) g_th[(size_t)i * ld + row] = loss_grad(gt[i]);
     }
     {
         const float inv_res = 1.0f / (float)NPH;
         float gp[NPH];
         df_logit_grad<NPH>(x1, prefix * apt * (float)NPH, linear, [&](int k) { return zp[k] * inv_res; }, gp);
 #pragma unroll
-        for (int i = 0; i < NPH; ++i) g_ph[(size_t)i * ld + row] = gp[i];
+        for (int i = 0; i < NPH; ++i) g_ph[(size_t)i * ld + row] = loss_grad(gp[i]);
     }
 }
 
@@ -285,7 +285,7 @@ __global__ void plain_nrc_loss_kernel(const float* __restrict__ pred, const floa
         if (c < 3) {
             const float diff = p[c] - target[row * 3 + c];
             lsum += diff * diff / den / 1.0f / n_total;
-            gv = loss_scale * (2.0f * diff / den / 1.0f) / n_total;
+            gv = loss_grad(loss_scale * (2.0f * diff / den / 1.0f) / n_total);
         }
         g[(size_t)c * ld + row] = gv;
     }

@@ -114,7 +114,7 @@ __device__ __forceinline__ void df_grad(float x, float pdf_other, float prefix, 
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int i = v * 8 + j;
-            hv[j] = i < N ? sat_h(g[i < N ? i : 0]) : (_Float16)0.0f;
+            hv[j] = i < N ? sat_h(loss_grad(g[i < N ? i : 0])) : (_Float16)0.0f;
             fm[(size_t)i * ld + row] = hv[j];
         }
         *(h8*)(rm + row * NPAD + v * 8) = hv;
@@ -185,7 +185,7 @@ __global__ void nrc_loss_kernel(const NrcLossArgs a) {
             const float diff = (float)p[c] - a.target[row * 3 + c];
             lsum += diff * diff / den / 1.0f / n_total;
             const float grad = 2.0f * diff / den / 1.0f;
-            gh = sat_h(a.loss_scale * grad / n_total);
+            gh = sat_h(loss_grad(a.loss_scale * grad / n_total));
             g0[c] = gh;
         }
         a.gout_fm[(size_t)c * a.ld + row] = gh;

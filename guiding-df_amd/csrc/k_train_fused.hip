@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256, 2) void train_fused_kernel(const TrainFusedArg
                             const float diff = pc[ch] - tg[ch];
                             lsum += diff * diff / den / 1.0f / n_total;
                             const float grad = 2.0f * diff / den / 1.0f;
-                            gv[ch] = (float)sat_h(a.loss_scale * grad / n_total);
+                            gv[ch] = (float)sat_h(loss_grad(a.loss_scale * grad / n_total));
                         }
                         a.loss[row] = lsum;
                     }
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256, 2) void train_fused_kernel(const TrainFusedArg
                             for (int r = 0; r < 4; ++r) {
                                 const int i = 16 * t + 4 * g + r;
                                 const float gi = dg(i, bins[t][r] * inv_res, multiplier);
-                                GO[i * RS + rw] = row < a.n ? sat_h(gi) : (_Float16)0.0f;
+                                GO[i * RS + rw] = row < a.n ? sat_h(loss_grad(gi)) : (_Float16)0.0f;
                             }
                     }
                 }

@@ -790,8 +790,11 @@ double orc_df_train_f64(const OrcNet* th, const OrcNet* ph, const double* pth, c
         const double pre = -Li / wiPdf / q / (double)B;
         dpdf_dz(bt.data(), th->n_out, x0, linear, dt.data());
         dpdf_dz(bp.data(), ph->n_out, x1, linear, dp.data());
-        for (int j = 0; j < th->n_out_pad; ++j) got[j] = j < th->n_out ? pre * std::fabs(pp) * dt[j] : 0;
-        for (int j = 0; j < ph->n_out_pad; ++j) gop[j] = j < ph->n_out ? pre * std::fabs(pt) * dp[j] : 0;
+        // a loss-head gradient that is not finite contributes nothing (the
+        // library's rule; the reference's NaN would reach every weight)
+        auto fin = [](double g) { return std::fabs(g) <= 1.7976931348623157e308 ? g : 0.0; };
+        for (int j = 0; j < th->n_out_pad; ++j) got[j] = j < th->n_out ? fin(pre * std::fabs(pp) * dt[j]) : 0;
+        for (int j = 0; j < ph->n_out_pad; ++j) gop[j] = j < ph->n_out ? fin(pre * std::fabs(pt) * dp[j]) : 0;
         if (gth) {
             mlp_backward(*th, pth, xt.data(), at.data(), got.data(), gth, dxt.data());
             grid_backward(*th, in.data() + grid_dim(*th), dxt.data(), gth);
@@ -823,7 +826,8 @@ double orc_nrc_train_f64(const OrcNet* nt, const double* P, const float* feat, i
         for (int c = 0; c < 3; ++c) {
             const double diff = o[c] - target[b * 3 + c];
             total += diff * diff / den / n_total;
-            go[c] = 2 * diff / den / n_total;
+            const double gc = 2 * diff / den / n_total;
+            go[c] = std::fabs(gc) <= 1.7976931348623157e308 ? gc : 0.0;  // the loss-head rule of the library
         }
         if (grad) mlp_backward(*nt, P, x.data(), a.data(), go.data(), grad, dx.data());
     }

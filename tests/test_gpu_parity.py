@@ -1183,6 +1183,87 @@ def test_full_size_training_properties(torch_cuda):
         f.close()
 
 
+def test_long_training_stays_finite_and_accurate(torch_cuda):
+    """400 DF + NRC steps at the full batch (four batches in turn, lr 1e-2):
+    every master and inference weight stays finite, the NRC loss and at least
+    95 % of the DF losses stay finite, the EMA weights stay within the range of
+    the master weights they average, and the query on the trained inference
+    weights still matches the FP64 oracle (median pdf error <= 1e-3)."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=True)
+    f = G.GuidingField(cfg)
+    B, steps = 262144, 400
+    batches = [(dev(torch, df_features(B, 300 + i, nrc=True)), dev(torch, train_targets(B, 310 + i)),
+                dev(torch, np.exp(np.random.default_rng(320 + i).normal(0, 2, (B, 3))).astype(np.float32)))
+               for i in range(4)]
+    loss = torch.zeros((steps, 2), dtype=torch.float32, device="cuda")
+    for k in range(steps):
+        feat, tgt, rgb = batches[k % 4]
+        f.train_step(feat, tgt, loss[k, 0:1])
+        f.nrc_train_step(feat, rgb, loss[k, 1:2])
+    torch.cuda.synchronize()
+    ls = host(loss)
+    # the DF loss of a step is not finite when a row's theta logits all fall
+    # below -104 (the reference's softmax, its maximum started at 0, underflows
+    # there); such rows add no gradient, so training goes on
+    print("steps with a non-finite DF loss: %d of %d" % ((~np.isfinite(ls[:, 0])).sum(), steps))
+    assert np.isfinite(ls[:, 1]).all() and np.isfinite(ls[:, 0]).mean() >= 0.95
+    for k in (0, 1, 2):
+        master = f.get_params(k, G.DFG_PARAMS_MASTER)
+        ema = f.get_params(k, G.DFG_PARAMS_INFERENCE)
+        assert np.all(np.isfinite(master)) and np.all(np.isfinite(ema)), "net %d: non-finite weights" % k
+        assert np.abs(ema).max() <= 1.01 * np.abs(master).max() + 1e-3, "net %d: EMA outside the weights' range" % k
+    th, ph, _ = O.config_layouts(cfg)
+    n = 4096
+    feat_h, u_h = df_features(n, 330), uniforms(n, 331)
+    w = dev(torch, wi_init(n))
+    f.query(dev(torch, feat_h), dev(torch, u_h), w)
+    torch.cuda.synchronize()
+    g = host(w)
+    o = O.query_f64(th, ph, f.get_params(0, G.DFG_PARAMS_INFERENCE), f.get_params(1, G.DFG_PARAMS_INFERENCE),
+                    feat_h, 9, u_h, wi_init(n), 1)
+    rel = _rel(g[:, 2] * g[:, 3], o[:, 2] * o[:, 3])
+    print("after %d steps: median rel pdf err %.3g, p99 %.3g" % (steps, np.median(rel), np.percentile(rel, 99)))
+    assert np.median(rel) <= 1e-3, np.median(rel)
+    f.close()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_rows_with_an_underflowing_softmax_add_no_gradient(torch_cuda, plain):
+    """theta's last layer set so that every row's logits lie below -104: the
+    softmax with its maximum started at 0 (train.h:846) underflows to a zero
+    sum, the reference's bins turn +inf and its pdf NaN.  The step's loss shows
+    it (not finite), the rows' loss-head gradients count as zero instead of
+    NaN, as in the oracle, so the step's gradient is zero and every weight
+    stays finite."""
+    torch = torch_cuda
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=14)
+    f = G.GuidingField(keep_grads(cfg))
+    Lt, Lp, _ = O.config_layouts(cfg)
+    rng = np.random.default_rng(5)
+    pt, pp = trained_scale_params(Lt, rng), trained_scale_params(Lp, rng)
+    off = Lt["layer_off"]
+    pt[off[1]:off[3]] = np.abs(pt[off[1]:off[3]])   # hidden layers 2-3: activations >= 0
+    pt[off[3]:off[4]] = -1000.0                      # theta logits = -1000 * sum(a3), all far below -104
+    f.set_params(0, pt)
+    f.set_params(1, pp)
+    B = 4096
+    feat, tgt = dev(torch, df_features(B, 150)), dev(torch, train_targets(B, 151))
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    (f.train_step_plain if plain else f.train_step)(feat, tgt, loss)
+    torch.cuda.synchronize()
+    ref_loss, _, _ = O.df_train_f64(Lt, Lp, pt, pp, df_features(B, 150), 9, train_targets(B, 151), True)
+    assert not np.isfinite(ref_loss)   # the FP64 sum underflows as well at these logits
+    assert not np.isfinite(host(loss)[0]), host(loss)
+    for k in (0, 1):
+        g = f.get_grad(k)
+        assert np.all(np.isfinite(g)), "net %d: non-finite gradient" % k
+        assert not np.any(g), "net %d: gradient from rows with no finite loss" % k
+        for which in (G.DFG_PARAMS_MASTER, G.DFG_PARAMS_INFERENCE):
+            assert np.all(np.isfinite(f.get_params(k, which))), "net %d: non-finite weights" % k
+    f.close()
+
+
 def test_initialisation_bounds_statistics_and_seed(torch_cuda):
     """Trainer::initialize_params (trainer.h:71-93): every MLP matrix Xavier-
     uniform over its padded shape (gpu_matrix.h:291-306), U(-b, b) with

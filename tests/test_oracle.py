@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 import dfguide as G
-from gpu_util import df_features, uniforms, wi_init, trained_scale_params
+from gpu_util import df_features, uniforms, wi_init, trained_scale_params, train_targets
 
 
 # ----------------------------------------------------------------------------
@@ -1055,3 +1055,26 @@ def test_recorded_path_yields_arriving_radiance():
         arriving = E * np.prod(factors[k + 1:], axis=0)
         assert np.allclose(t[k, 2:5], arriving * 0.75 * 0.3, rtol=1e-12), k
         assert t[k, 5] == 0.4
+
+
+def test_underflowing_softmax_rows_give_a_nan_loss_and_no_gradient():
+    """theta logits all near -1e6: the softmax whose maximum starts at 0
+    (train.h:846) sums to exactly zero, its bins and pdf are not finite, so the
+    batch loss is NaN; the loss-head gradient of such a row counts as zero, so
+    the whole gradient is zero.  With one finite row (theta's last layer left
+    as drawn) the gradient is finite and nonzero again."""
+    cfg = G.hash_config(with_nrc=False, log2_hashmap_size=12)
+    Lt, Lp, _ = O.config_layouts(cfg)
+    rng = np.random.default_rng(5)
+    pt, pp = trained_scale_params(Lt, rng), trained_scale_params(Lp, rng)
+    off = Lt["layer_off"]
+    bad = pt.copy()
+    bad[off[1]:off[3]] = np.abs(bad[off[1]:off[3]])
+    bad[off[3]:off[4]] = -1000.0
+    B = 64
+    feat, tgt = df_features(B, 150), train_targets(B, 151)
+    loss, gt, gp = O.df_train_f64(Lt, Lp, bad, pp, feat, 9, tgt, True)
+    assert math.isnan(loss)
+    assert not np.any(gt) and not np.any(gp)
+    loss, gt, gp = O.df_train_f64(Lt, Lp, pt, pp, feat, 9, tgt, True)
+    assert math.isfinite(loss) and np.all(np.isfinite(gt)) and np.any(gt) and np.any(gp)

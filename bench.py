@@ -29,6 +29,8 @@ sys.path.insert(0, os.path.join(ROOT, "guiding-df_amd"))
 METRIC = "guided queries/s + train samples/s vs plain FP32 path; p99 batch latency"
 HBM_PEAK_GBS = 8000.0
 MFMA_F16_PEAK_TFLOPS = 2500.0
+CLOCK_GHZ = 2.4   # MI355X peak engine clock
+CU_COUNT = 256
 
 
 def net_flops(enc_width, n_hidden, n_out_pad):
@@ -70,7 +72,60 @@ def profiled_query_kernels():
             "query_fused_kernel": float(m.group(3))}
 
 
-def kernel_rooflines(n, fl_q):
+def gather_ceiling():
+    """Random-gather rates (lane loads per clock per CU) measured by
+    guiding-df_amd/microbench/gather_rate for a 4 MB table (one level of the
+    interleaved theta/phi grid, the size of one XCD's L2), 16- and 8-byte
+    loads, from the summary in profiles/; None when it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "memory_rates_microbench.txt")
+    try:
+        with open(path) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    rates = {}
+    for w in (16, 8):
+        m = re.search(r"^\s*4096 KB\s+%d B\s+\d+\s+[0-9.]+\s+[0-9.]+\s+([0-9.]+)\s*$" % w, text, re.M)
+        if m:
+            rates["%dB" % w] = float(m.group(1))
+    return rates if len(rates) == 2 else None
+
+
+def prepass_lane_loads(pos, n_levels=16, base=16, log2_t=19, sample=65536):
+    """Random lane loads the paired grid pre-pass issues per query: on each
+    level one 16-byte load per (y, z) corner pair (the x pair's entries of
+    both nets), plus an 8-byte load where the x pair's two entries are not
+    adjacent in the table.  Dense levels (index x + y res + z res^2) always
+    have them adjacent; hashed levels (x ^ y 2654435761 ^ z 805459861 mod T)
+    for even x and some odd x.  Counted on the first `sample` positions of
+    the batch, for the 16-level, base-16, scale-2, T = 2^log2_t hash grid."""
+    import numpy as np
+    p = np.asarray(pos[:sample], dtype=np.float64)
+    T = 1 << log2_t
+    loads = 0.0
+    for lvl in range(n_levels):
+        scale = float(base) * 2.0 ** lvl - 1.0
+        res = base << lvl
+        size = min((res ** 3 + 7) // 8 * 8, T)
+        hashed = res ** 3 > size
+        c = np.floor(p * scale + 0.5).astype(np.uint64)
+        for pr in range(4):
+            y = c[:, 1] + np.uint64(pr & 1)
+            z = c[:, 2] + np.uint64(pr >> 1)
+            if hashed:
+                yz = (y * np.uint64(2654435761)) ^ (z * np.uint64(805459861))
+                idx = lambda x: ((x ^ yz) & np.uint64(0xFFFFFFFF)) % np.uint64(size)
+            else:
+                yz = y * np.uint64(res) + z * np.uint64(res * res)
+                idx = lambda x: ((x + yz) & np.uint64(0xFFFFFFFF)) % np.uint64(size)
+            i0, i1 = idx(c[:, 0]), idx(c[:, 0] + np.uint64(1))
+            adjacent = (i1 == i0 + np.uint64(1)) | (i0 == i1 + np.uint64(1))
+            loads += 1.0 + float(np.mean(~adjacent))
+    return loads
+
+
+def kernel_rooflines(n, fl_q, pos=None):
     """Per-kernel roofline fractions of one 1,048,576-query call from the
     profiled kernel durations.  Algorithmic work per query: extract_pos reads
     the 3 position floats of a 60-byte feature row and writes 12 B; the grid
@@ -85,6 +140,14 @@ def kernel_rooflines(n, fl_q):
         gbs = b * n / (us[name] * 1e-6) / 1e9
         out.append({"kernel": name, "us": us[name], "bound": "hbm", "bytes_per_query": b, "achieved": gbs,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS})
+    ceil = gather_ceiling()
+    if pos is not None and ceil is not None:
+        # the pre-pass's actual limit: the chip's random-gather rate
+        lq = prepass_lane_loads(pos)
+        rate = lq * n / (us["grid_encode_pair_kernel"] * 1e-6 * CLOCK_GHZ * 1e9) / CU_COUNT
+        out[1]["gather"] = {"lane_loads_per_query": lq, "lane_loads_per_clk_per_cu": rate,
+                            "measured_ceiling_4MB_table": ceil, "frac_of_16B_ceiling": rate / ceil["16B"],
+                            "source": "profiles/memory_rates_microbench.txt"}
     tf = fl_q * n / (us["query_fused_kernel"] * 1e-6) / 1e12
     out.append({"kernel": "query_fused_kernel", "us": us["query_fused_kernel"], "bound": "mfma",
                 "flops_per_query": fl_q, "achieved": tf, "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -442,7 +505,7 @@ def main():
                      "kernel_ms": kernel_ms,
                      "bytes_per_query": bytes_q, "flops_per_query": fl_q,
                      "mfma_tflops": achieved_tf, "mfma_frac": achieved_tf / MFMA_F16_PEAK_TFLOPS,
-                     "per_kernel": kernel_rooflines(n, fl_q)},
+                     "per_kernel": kernel_rooflines(n, fl_q, feat_h[:, :3])},
         "cpu_baseline": cpu,
         "wavefront_frame": wave,
         "served_loop": served,

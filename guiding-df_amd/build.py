@@ -3,7 +3,8 @@
     python guiding-df_amd/build.py            # compile every source, link
     python guiding-df_amd/build.py --verbose
 
-Objects go to guiding-df_amd/_build/, the library to guiding-df_amd/libdfg.so.
+Objects go to guiding-df_amd/_build/, the library to guiding-df_amd/libdfg.so,
+the memory-rate microbenchmark to guiding-df_amd/microbench/gather_rate.
 """
 import os
 import subprocess
@@ -65,6 +66,15 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    # the memory-rate microbenchmark (a standalone program, not part of the library)
+    mb = os.path.join(HERE, "microbench")
+    cmd = [HIPCC] + FLAGS + ["-x", "hip", os.path.join(mb, "gather_rate.hip"), "-o", os.path.join(mb, "gather_rate")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if p.returncode != 0:
+        print(p.stdout.decode(errors="replace"), file=sys.stderr)
+        raise RuntimeError("hipcc failed for microbench/gather_rate.hip")
     if verbose:
         print("built %s in %.1fs" % (LIB, time.time() - t0))
     return LIB

@@ -641,6 +641,15 @@ struct Segment {
     float c_hi;   // cdf at the segment end
 };
 
+// The searches' three selects per bin are kept in the bin's step: without
+// this (empty) barrier the compiler splits them into three chains over all
+// bins, keeps every bin's hit mask live in scalar registers for them (N mask
+// pairs: spilled at 64 bins) and turns the index chain into float selects
+// whose constants occupy vector registers.  No instruction is emitted.
+__device__ __forceinline__ void select_barrier(Segment& s) {
+    asm volatile("" : "+v"(s.i), "+v"(s.c_lo), "+v"(s.c_hi));
+}
+
 // search_sorted_cdf_linearinterpolation, train.h:361-397 — the sequential fp32
 // CDF of the piecewise-linear density; branch-free, same sums in the same order.
 template <int N>
@@ -662,6 +671,7 @@ __device__ __forceinline__ Segment search_linear(const float (&p)[N], float u) {
         s.c_hi = hit ? c : s.c_hi;
         found = found | hit;
         prev = c;
+        select_barrier(s);
     }
     const float end = prev + p[N - 1] * 0.5f / (float)N;
     s.i = found ? s.i : N;
@@ -687,6 +697,7 @@ __device__ __forceinline__ Segment search_nearest(const float (&p)[N], float u) 
         s.c_hi = hit ? c : s.c_hi;
         found = found | hit;
         prev = c;
+        select_barrier(s);
     }
     s.c_lo = found ? s.c_lo : prev;
     s.c_hi = found ? s.c_hi : c;

@@ -964,6 +964,19 @@ struct GridDirect {
     const GridPre* pre = nullptr;  // K-step 0 already loaded (all four groups in one pass)
 };
 
+// The first layer's weight fragments of K-step 0 (4 column tiles), loaded
+// ahead of the layer by the caller so their latency overlaps earlier work.
+struct W0Pre {
+    h8 w[4];
+};
+
+__device__ __forceinline__ void w0_pre_load(const NetSpec& ns, const _Float16* __restrict__ fwd_pack, W0Pre& p) {
+    const int lane = threadIdx.x & 63;
+    const _Float16* pk = fwd_pack + ns.fwd_off[0];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) p.w[t] = ld_frag(pk, t, 0, ns.k_steps, lane);
+}
+
 __device__ __forceinline__ void grid_pre_load(const GridDirect& gd, GridPre& p) {
     const int lane = threadIdx.x & 63;
     const int i16 = lane & 15, g = lane >> 4;
@@ -1091,7 +1104,8 @@ __device__ __forceinline__ void store_act(const ActStore& st, _Float16* dst, con
 template <int NT_OUT, bool SPLIT, int QN, typename TT>
 __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
                                                    const TT* xt, int xs, float* lo, int ls,
-                                                   const ActStore* st, int q0, const GridDirect& gd) {
+                                                   const ActStore* st, int q0, const GridDirect& gd,
+                                                   const W0Pre* w0 = nullptr) {
     // an fp16 input tile has no low part to add
     constexpr bool SPLIT_IN = SPLIT && sizeof(TT) == 4;
     const int lane = threadIdx.x & 63;
@@ -1106,7 +1120,23 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
         for (int q = 0; q < QN; ++q)
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+        // K-step s + 1's weight fragments are loaded while step s computes
+        // (step 0's may come prefetched by the caller)
+        h8 wc[4];
+        if (w0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) wc[t] = w0->w[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) wc[t] = ld_frag(pk, t, 0, ns.k_steps, lane);
+        }
         for (int s = 0; s < ns.k_steps; ++s) {
+            h8 wn[4];
+            const bool more = s + 1 < ns.k_steps;
+            if (more) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) wn[t] = ld_frag(pk, t, s + 1, ns.k_steps, lane);
+            }
             h8 b[QN], bl[QN];
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
@@ -1158,12 +1188,16 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const h8 a = ld_frag(pk, t, s, ns.k_steps, lane);
+                const h8 a = wc[t];
 #pragma unroll
                 for (int q = 0; q < QN; ++q) {
                     acc[q][t] = mfma16(a, b[q], acc[q][t]);
                     if (SPLIT_IN) acc[q][t] = mfma16(a, bl[q], acc[q][t]);
                 }
+            }
+            if (more) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) wc[t] = wn[t];
             }
         }
 #pragma unroll
@@ -1242,8 +1276,10 @@ __device__ __forceinline__ void mlp_forward_groups(const NetSpec& ns, const _Flo
 template <int NT_OUT, bool SPLIT, int QN = (SPLIT ? 2 : 4), typename TT>
 __device__ __forceinline__ void mlp_forward_tile(const NetSpec& ns, const _Float16* __restrict__ fwd_pack,
                                                  const TT* xt, int xs, float* lo, int ls,
-                                                 const ActStore* st, const GridDirect& gd = GridDirect{nullptr, 0, 0, 0, 0}) {
-    for (int q0 = 0; q0 < 4; q0 += QN) mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0, gd);
+                                                 const ActStore* st, const GridDirect& gd = GridDirect{nullptr, 0, 0, 0, 0},
+                                                 const W0Pre* w0 = nullptr) {
+    for (int q0 = 0; q0 < 4; q0 += QN)
+        mlp_forward_groups<NT_OUT, SPLIT, QN>(ns, fwd_pack, xt, xs, lo, ls, st, q0, gd, q0 == 0 ? w0 : nullptr);
 }
 
 // ---------------------------------------------------------------------------

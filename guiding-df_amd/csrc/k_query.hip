@@ -20,6 +20,15 @@ struct RowIn {
     __device__ float operator()(int d) const { return d < base_dims ? row[d] : extra; }
 };
 
+// The same inputs from registers (the row loaded at the tile's start); the
+// encoders index them with a wave-uniform run-time index.
+struct RowInReg {
+    const float (*v)[DFG_MAX_IN];
+    int base_dims;
+    float extra;
+    __device__ float operator()(int d) const { return d < base_dims && d < DFG_MAX_IN ? (*v)[d] : extra; }
+};
+
 // Head of one net for the lane's row: reads N logits from the LDS row, writes
 // the bins back (for the dynamic reads of the sampler/pdf), optionally samples.
 template <int N>
@@ -64,10 +73,41 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     auto wr = [xrow](int c, float v) { xrow[c] = v; };
     float* lrow = lo + lane * a.ls;
 
+    W0Pre w0_nrc;  // the radiance cache's first K-step weights, loaded during phi's head
+    // PREFETCH: the row's raw inputs into registers at the tile's start (all
+    // loads in flight together) instead of one dependent load per use
+    float rin[DFG_MAX_IN];
+    if constexpr (PREFETCH) {
+        const int nd = io.stride < DFG_MAX_IN ? io.stride : DFG_MAX_IN;
+#pragma unroll
+        for (int d = 0; d < DFG_MAX_IN; ++d) rin[d] = d < nd ? frow[d] : 0.0f;
+    }
+    auto row_in = [&](int base_dims, float extra) {
+        if constexpr (PREFETCH) return RowInReg{&rin, base_dims, extra};
+        else return RowIn{frow, base_dims, extra};
+    };
   if constexpr (WITH_DF) {
     const NetSpec& th = *a.th;
     const NetSpec& ph = *a.ph;
     const int D = th.in_dims;
+
+    // the row's direction state, uniforms and guided flag (PREFETCH: loaded
+    // here, so their latency overlaps theta's encoding and MLP)
+    bool guided = false;
+    float4 w = make_float4(0.f, 0.f, 1.f, 1.f);
+    float4 uu = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto load_io = [&]() {
+        guided = a.wi2 == nullptr && io.u != nullptr && (io.guided == nullptr || io.guided[r] != 0);
+        if (a.wi2) {
+            w.x = a.wi2[2 * r];
+            w.y = a.wi2[2 * r + 1];
+        } else if (io.wi) {
+            w = *(const float4*)(io.wi + 4 * r);
+        }
+        // (loaded for every row when present; only guided rows use it)
+        if (a.wi2 == nullptr && io.u != nullptr) uu = *(const float4*)(io.u + 4 * r);
+    };
+    if constexpr (PREFETCH) load_io();
 
     // ---- theta net (with the pre-pass, its leading grid K-steps skip the LDS tile)
     const int gk_th = a.gfeat_th ? th.grid_ksteps : 0;
@@ -81,31 +121,27 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     const GridDirect gd_ph{gk_ph ? a.gfeat_ph + io.gbase : nullptr, a.gld, gk_ph, row0, io.n - 1,
                            pph ? &pre_ph : nullptr};
     if (pth) grid_pre_load(gd_th, pre_th);
+    // the first layer's K-step-0 weights of theta, loaded before its encoding
+    W0Pre w0_th, w0_ph;
+    if constexpr (PREFETCH) w0_pre_load(th, a.pack_th, w0_th);
     if (gk_th) {
         const int o = 32 * gk_th;
-        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
+        encode_row(th, a.tab_th, row_in(D, 0.0f), [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
                    nullptr, 0, true, false);
     } else {
-        encode_row(th, a.tab_th, RowIn{frow, D, 0.0f}, wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld,
+        encode_row(th, a.tab_th, row_in(D, 0.0f), wr, a.gfeat_th ? a.gfeat_th + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
     if constexpr (LOCKSTEP) __syncthreads();
-    mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr, gd_th);
+    mlp_forward_tile<NTH / 16, true, 4>(th, a.pack_th, xt, a.xs, lo, a.ls, nullptr, gd_th,
+                                         PREFETCH ? &w0_th : nullptr);
     if (pph) grid_pre_load(gd_ph, pre_ph);
+    if constexpr (PREFETCH) w0_pre_load(ph, a.pack_ph, w0_ph);
     if (a.logits_th && valid) {
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
     }
-    const bool guided = a.wi2 == nullptr && io.u != nullptr && (io.guided == nullptr || io.guided[r] != 0);
-    float4 w = make_float4(0.f, 0.f, 1.f, 1.f);
-    if (a.wi2) {
-        w.x = a.wi2[2 * r];
-        w.y = a.wi2[2 * r + 1];
-    } else if (io.wi) {
-        w = *(const float4*)(io.wi + 4 * r);
-    }
-    float4 uu = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (guided) uu = *(const float4*)(io.u + 4 * r);
+    if constexpr (!PREFETCH) load_io();
     float p_th;
     const float u0 = head_row<NTH>(lrow, guided, uu.x, uu.z, a.linear, w.x, p_th);
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
@@ -114,14 +150,16 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     if (gk_ph) {
         // (with a.ph_part0 > 0 the leading parts are theta's, still in the row)
         const int o = 32 * gk_ph;
-        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
+        encode_row(ph, a.tab_ph, row_in(D, th_in), [xrow, o](int c, float v) { if (c >= o) xrow[c - o] = v; },
                    nullptr, 0, true, false, a.ph_part0);
     } else {
-        encode_row(ph, a.tab_ph, RowIn{frow, D, th_in}, wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
+        encode_row(ph, a.tab_ph, row_in(D, th_in), wr, a.gfeat_ph ? a.gfeat_ph + io.gbase + r : nullptr, a.gld,
                    false, false);
     }
     if constexpr (LOCKSTEP) __syncthreads();
-    mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr, gd_ph);
+    mlp_forward_tile<NPH / 16, true, 4>(ph, a.pack_ph, xt, a.xs, lo, a.ls, nullptr, gd_ph,
+                                         PREFETCH ? &w0_ph : nullptr);
+    if constexpr (PREFETCH && WITH_NRC) w0_pre_load(*a.nrc, a.pack_nrc, w0_nrc);
     if (a.logits_ph && valid) {
 #pragma unroll
         for (int i = 0; i < NPH; ++i) a.logits_ph[r * NPH + i] = lrow[i];
@@ -145,10 +183,12 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         // radiance, not a pdf, and feeds training targets only.
         _Float16* const xh = (_Float16*)smem;
         _Float16* const xhrow = xh + lane * a.xs_nrc;
-        encode_row(*a.nrc, a.tab_nrc, RowIn{frow, 64, 0.0f}, [xhrow](int c, float v) { xhrow[c] = round_half(v); },
+        encode_row(*a.nrc, a.tab_nrc, row_in(64, 0.0f), [xhrow](int c, float v) { xhrow[c] = round_half(v); },
                    nullptr, 0, false, false);
         if constexpr (LOCKSTEP) __syncthreads();
-        mlp_forward_tile<1, false, 4>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr);
+        mlp_forward_tile<1, false, 4>(*a.nrc, a.pack_nrc, xh, a.xs_nrc, lo, a.ls, nullptr,
+                                      GridDirect{nullptr, 0, 0, 0, 0},
+                                      (PREFETCH && WITH_DF) ? &w0_nrc : nullptr);
         if (valid && io.rgb) {
             io.rgb[3 * r + 0] = lrow[0];
             io.rgb[3 * r + 1] = lrow[1];

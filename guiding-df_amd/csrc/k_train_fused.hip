@@ -122,8 +122,10 @@ __global__ __launch_bounds__(256, 2) void train_fused_kernel(const TrainFusedArg
         if constexpr (LOSS == FL_NRC) {
             if (lane < 16) {
                 const float* src = a.feat + r * a.stride;
+                int nd = in_dims;
+                asm volatile("" : "+s"(nd));  // as below: keeps the comparisons in the tile's step
 #pragma unroll
-                for (int d = 0; d < kMaxIn; ++d) fin[d] = d < in_dims ? src[d] : 0.0f;
+                for (int d = 0; d < kMaxIn; ++d) fin[d] = d < nd ? src[d] : 0.0f;
 #pragma unroll
                 for (int ch = 0; ch < 3; ++ch) tg[ch] = a.target[r * 3 + ch];
             }
@@ -142,7 +144,11 @@ __global__ __launch_bounds__(256, 2) void train_fused_kernel(const TrainFusedArg
             if (lane < 16) {
                 const float* src = a.feat + r * a.stride;
                 // the phi net's last input is theta from the record (add_theta_to_input_F)
-                const int base = a.theta_in ? in_dims - 1 : in_dims;
+                // (base is made opaque per tile: the compiler otherwise hoists
+                // the 2 x kMaxIn comparisons below out of the tile loop and
+                // keeps their masks live in scalar registers, which spill)
+                int base = a.theta_in ? in_dims - 1 : in_dims;
+                asm volatile("" : "+s"(base));
                 const float th = a.theta_in ? theta_input(tg[0], a.theta_bins, a.linear) : 0.0f;
 #pragma unroll
                 for (int d = 0; d < kMaxIn; ++d) fin[d] = d < base ? src[d] : (d == base ? th : 0.0f);

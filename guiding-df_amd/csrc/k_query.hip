@@ -60,7 +60,8 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 // the same net's weights (every wave of the workgroup must run the tile).
 // PREFETCH: load each net's single grid K-step ahead of its MLP (more live
 // registers; the work-list kernels, already near the register limit, skip it).
-template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false, bool PREFETCH = false>
+template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false, bool PREFETCH = false,
+          bool PREFETCH_IO = PREFETCH>
 __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
     float* xt = (float*)smem;
     float* lo = (float*)smem + a.lo_off;
@@ -77,13 +78,13 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     // PREFETCH: the row's raw inputs into registers at the tile's start (all
     // loads in flight together) instead of one dependent load per use
     float rin[DFG_MAX_IN];
-    if constexpr (PREFETCH) {
+    if constexpr (PREFETCH_IO) {
         const int nd = io.stride < DFG_MAX_IN ? io.stride : DFG_MAX_IN;
 #pragma unroll
         for (int d = 0; d < DFG_MAX_IN; ++d) rin[d] = d < nd ? frow[d] : 0.0f;
     }
     auto row_in = [&](int base_dims, float extra) {
-        if constexpr (PREFETCH) return RowInReg{&rin, base_dims, extra};
+        if constexpr (PREFETCH_IO) return RowInReg{&rin, base_dims, extra};
         else return RowIn{frow, base_dims, extra};
     };
   if constexpr (WITH_DF) {
@@ -107,7 +108,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
         // (loaded for every row when present; only guided rows use it)
         if (a.wi2 == nullptr && io.u != nullptr) uu = *(const float4*)(io.u + 4 * r);
     };
-    if constexpr (PREFETCH) load_io();
+    if constexpr (PREFETCH_IO) load_io();
 
     // ---- theta net (with the pre-pass, its leading grid K-steps skip the LDS tile)
     const int gk_th = a.gfeat_th ? th.grid_ksteps : 0;
@@ -141,7 +142,7 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
 #pragma unroll
         for (int i = 0; i < NTH; ++i) a.logits_th[r * NTH + i] = lrow[i];
     }
-    if constexpr (!PREFETCH) load_io();
+    if constexpr (!PREFETCH_IO) load_io();
     float p_th;
     const float u0 = head_row<NTH>(lrow, guided, uu.x, uu.z, a.linear, w.x, p_th);
     const float th_in = a.theta_in ? a.theta_in[r] : theta_input(u0, NTH, a.linear);
@@ -239,7 +240,9 @@ __global__ __launch_bounds__(64, 2) void query_worklist_kernel(const QueryArgs a
     for (;;) {
         unsigned int t0 = 0;
         if (lane == 0) t0 = atomicAdd(counter, (unsigned int)tiles_per_fetch);
-        t0 = __shfl(t0, 0);
+        // lane 0's value, as a wave-uniform (scalar) value: the segment search
+        // and the segment's pointers below then stay in scalar registers
+        t0 = (unsigned int)__builtin_amdgcn_readfirstlane((int)t0);
         if ((int64_t)t0 >= total) break;
         for (int k = 0; k < tiles_per_fetch; ++k) {
             const int64_t tile = (int64_t)t0 + k;
@@ -252,9 +255,10 @@ __global__ __launch_bounds__(64, 2) void query_worklist_kernel(const QueryArgs a
                 if (wl->seg[mid].tile_begin <= tile) sg = mid;
                 else hi = mid - 1;
             }
+            sg = __builtin_amdgcn_readfirstlane(sg);
             const WorkSeg& ws = wl->seg[sg];
             const RowsIO io{ws.feat, ws.stride, ws.u, ws.guided, ws.wi, ws.rgb, ws.n, ws.gbase};
-            query_tile<NTH, NPH, true, WITH_NRC>(a, io, (tile - ws.tile_begin) * 64, smem);
+            query_tile<NTH, NPH, true, WITH_NRC, false, false, true>(a, io, (tile - ws.tile_begin) * 64, smem);
         }
     }
 }

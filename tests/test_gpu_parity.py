@@ -32,6 +32,16 @@ def _adversarial_logits(rng, n, N):
     z[np.arange(2 * k, 3 * k), hot] = 25.0        # one-hot: empty segments (delta < eps)
     z[3 * k:4 * k] *= 8.0                         # peaked
     z[4 * k:5 * k] = np.round(z[4 * k:5 * k])     # equal neighbours
+    # non-finite and underflowing rows, 64 of each
+    e = [5 * k + 64 * j for j in range(8)]
+    z[e[0]:e[1]] = -120.0                         # every exp(z - 0) below the smallest subnormal: sum 0
+    z[e[1]:e[2]] = -60.0                          # max 50, the rest 110 below it: those bins underflow to 0
+    z[e[1]:e[2], 0] = 50.0
+    z[e[2]:e[3], N // 2] = np.nan                 # one NaN logit
+    z[e[3]:e[4], 1] = -np.inf                     # one -inf logit
+    z[e[4]:e[5], N - 1] = np.inf                  # one +inf logit
+    z[e[5]:e[6], 2] = -1e10                       # one logit far beyond the exponent's range
+    z[e[6]:e[7]] = -1e30
     return z
 
 
@@ -57,14 +67,18 @@ def test_df_head_bitexact(torch_cuda, nt, np_, linear):
     g_wi, g_bt, g_bp, g_seg = host(d_wi), host(d_bt), host(d_bp), host(d_seg)
     x0, p0, b0, s0 = O.head(lt, linear, u[:, 0], u[:, 2], sample=guided, x_given=wi[:, 0], fp32=True)
     x1, p1, b1, s1 = O.head(lp, linear, u[:, 1], u[:, 3], sample=guided, x_given=wi[:, 1], fp32=True)
-    assert np.array_equal(g_bt, b0), "theta bins differ in %d rows" % (g_bt != b0).any(1).sum()
-    assert np.array_equal(g_bp, b1), "phi bins differ in %d rows" % (g_bp != b1).any(1).sum()
+    # (rows with non-finite logits have non-finite bins, samples or pdfs: NaN counts as equal to NaN)
+    assert np.array_equal(g_bt, b0, equal_nan=True), "theta bins differ in %d rows" % (g_bt != b0).any(1).sum()
+    assert np.array_equal(g_bp, b1, equal_nan=True), "phi bins differ in %d rows" % (g_bp != b1).any(1).sum()
     s0 = np.where(guided == 1, s0, -1)
     s1 = np.where(guided == 1, s1, -1)
     assert np.array_equal(g_seg[:, 0], s0), "theta segments differ in %d rows" % (g_seg[:, 0] != s0).sum()
     assert np.array_equal(g_seg[:, 1], s1), "phi segments differ in %d rows" % (g_seg[:, 1] != s1).sum()
-    assert np.array_equal(g_wi[:, 0], x0) and np.array_equal(g_wi[:, 1], x1), "samples differ"
-    assert np.array_equal(g_wi[:, 2], p0) and np.array_equal(g_wi[:, 3], p1), "pdfs differ"
+    assert np.array_equal(g_wi[:, 0], x0, equal_nan=True) and np.array_equal(g_wi[:, 1], x1, equal_nan=True), \
+        "samples differ"
+    assert np.array_equal(g_wi[:, 2], p0, equal_nan=True) and np.array_equal(g_wi[:, 3], p1, equal_nan=True), \
+        "pdfs differ"
+    assert not np.isfinite(p0).all() and np.isinf(b0).any()   # the non-finite rows are there
 
 
 # ----------------------------------------------------------------------------

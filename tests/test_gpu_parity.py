@@ -180,6 +180,15 @@ def test_query_vs_fp64(torch_cuda, cfgname):
     assert np.percentile(rel, 99) <= 3e-2, "p99 rel pdf err %.3g" % np.percentile(rel, 99)
     du = np.abs(g[:, :2] - o[:, :2])
     assert np.median(du) <= 1e-3 and np.percentile(du, 99) <= 2e-2
+    # the pdf at the very direction the GPU sampled (the end-to-end figures
+    # above compose the sampler's rounding with the pdf's slope): FP64 there
+    at = wi_init(n)
+    at[:, :2] = g[:, :2]
+    o2 = O.query_f64(th, ph, pt, pp, feat, D, u, at, 1, guided=np.zeros(n, np.uint8))
+    at_sample = _rel(pdf_g.astype(np.float64), o2[:, 2].astype(np.float64) * o2[:, 3])
+    print("%s: pdf at sample median %.3g p99 %.3g" % (cfgname, np.median(at_sample), np.percentile(at_sample, 99)))
+    # measured on MI355X: median 1.1e-7..1.3e-7, p99 4e-7..1.4e-6
+    assert np.median(at_sample) <= 2e-6 and np.percentile(at_sample, 99) <= 5e-5
     # non-guided rows keep the caller's direction exactly
     ng = guided == 0
     assert np.array_equal(g[ng, :2], wi0[ng, :2])

@@ -1,6 +1,6 @@
 """CPU tests that pin the oracle itself (closed forms, invariants, brute force,
 finite differences).  The reference holds no golden vectors for this path, so
-these are what make the oracle trustworthy before it judges the GPU."""
+these are what make the oracle trustworthy before it checks the GPU."""
 import math
 
 import numpy as np

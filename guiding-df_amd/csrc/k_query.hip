@@ -58,8 +58,11 @@ __device__ __forceinline__ float head_row(float* lrow, bool do_sample, float u, 
 // One 64-row tile of a query: rows [row0, row0 + 64) of `io` (lane = row).
 // LOCKSTEP: a workgroup barrier before each MLP keeps the workgroup's waves on
 // the same net's weights (every wave of the workgroup must run the tile).
-// PREFETCH: load each net's single grid K-step ahead of its MLP (more live
-// registers; the work-list kernels, already near the register limit, skip it).
+// PREFETCH: load each net's single grid K-step and its first layer's K-step-0
+// weights ahead of its MLP.  PREFETCH_IO: load the row's raw inputs, direction
+// state, uniforms and guided flag at the tile's start.  Both hold registers
+// longer; the work-list kernels, nearer the register limit, take only the
+// second (with the first they spill).
 template <int NTH, int NPH, bool WITH_DF, bool WITH_NRC, bool LOCKSTEP = false, bool PREFETCH = false,
           bool PREFETCH_IO = PREFETCH>
 __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io, int64_t row0, unsigned char* smem) {
@@ -75,8 +78,8 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     float* lrow = lo + lane * a.ls;
 
     W0Pre w0_nrc;  // the radiance cache's first K-step weights, loaded during phi's head
-    // PREFETCH: the row's raw inputs into registers at the tile's start (all
-    // loads in flight together) instead of one dependent load per use
+    // PREFETCH_IO: the row's raw inputs into registers at the tile's start
+    // (all loads in flight together) instead of one dependent load per use
     float rin[DFG_MAX_IN];
     if constexpr (PREFETCH_IO) {
         const int nd = io.stride < DFG_MAX_IN ? io.stride : DFG_MAX_IN;
@@ -92,8 +95,8 @@ __device__ __forceinline__ void query_tile(const QueryArgs& a, const RowsIO& io,
     const NetSpec& ph = *a.ph;
     const int D = th.in_dims;
 
-    // the row's direction state, uniforms and guided flag (PREFETCH: loaded
-    // here, so their latency overlaps theta's encoding and MLP)
+    // the row's direction state, uniforms and guided flag (PREFETCH_IO:
+    // loaded here, so their latency overlaps theta's encoding and MLP)
     bool guided = false;
     float4 w = make_float4(0.f, 0.f, 1.f, 1.f);
     float4 uu = make_float4(0.f, 0.f, 0.f, 0.f);

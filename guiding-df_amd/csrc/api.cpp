@@ -355,13 +355,20 @@ static void alloc_net(dfg_ctx* c, int id, const NetSpec& ns, const OptCfg& opt) 
     HIPX(hipMemcpy(s.d_spec.p, &s.spec, sizeof(NetSpec), hipMemcpyHostToDevice));
     const size_t P = ns.n_params;
     s.master.ensure(P); s.m.ensure(P); s.v.ensure(P); s.grad.ensure(P);
-    s.w16.ensure(P); s.ema16.ensure(P); s.steps.ensure(P);
+    s.w16.ensure(P); s.ema16.ensure(P);
+    // 16-bit step counters when 1 - beta^65535 is exactly 1.0f for both betas
+    // (the device's exp2f(t log2 beta) <= 2^-26 there, with a bit to spare)
+    const float l1 = (float)std::log2((double)opt.beta1), l2 = (float)std::log2((double)opt.beta2);
+    s.short_steps = (float)kShortStepMax * l1 <= -26.0f && (float)kShortStepMax * l2 <= -26.0f;
+    if (s.short_steps) s.steps16.ensure(P);
+    else s.steps.ensure(P);
     s.pack_train.ensure(ns.pack_size);
     s.pack_infer.ensure(ns.fwd_off[ns.n_hidden + 1]);
     HIPX(hipMemset(s.m.p, 0, P * sizeof(float)));
     HIPX(hipMemset(s.v.p, 0, P * sizeof(float)));
     HIPX(hipMemset(s.grad.p, 0, P * sizeof(float)));
-    HIPX(hipMemset(s.steps.p, 0, P * sizeof(uint32_t)));
+    if (s.short_steps) HIPX(hipMemset(s.steps16.p, 0, P * sizeof(uint16_t)));
+    else HIPX(hipMemset(s.steps.p, 0, P * sizeof(uint32_t)));
     init_net_params(c, id);
     sync_copies(c, id, nullptr);
     HIPX(hipDeviceSynchronize());
@@ -1182,7 +1189,8 @@ static AdamArgs adam_args(dfg_ctx* c, NetState& s, float loss_scale) {
     o.grad = s.grad.p;
     o.m = s.m.p;
     o.v = s.v.p;
-    o.steps = s.steps.p;
+    o.steps = s.short_steps ? nullptr : s.steps.p;
+    o.steps16 = s.short_steps ? s.steps16.p : nullptr;
     return o;
 }
 

@@ -70,7 +70,12 @@ struct NetState {
     DevBuf<NetSpec> d_spec;
     DevBuf<float> master, m, v, grad;
     DevBuf<_Float16> w16, ema16, pack_train, pack_infer;
+    // per-parameter Adam step counters: 16-bit (saturating at 65535) when
+    // both betas make the bias corrections exactly 1 from that step on,
+    // else 32-bit; one of the two is allocated
     DevBuf<uint32_t> steps;
+    DevBuf<uint16_t> steps16;
+    bool short_steps = false;
     // training workspace
     DevBuf<_Float16> x, act, g, out, gout_rm, gout_fm;
     DevBuf<float> dx, partial, gfeat;
@@ -79,7 +84,7 @@ struct NetState {
     void release_all() {
         px.release(); pact.release(); pout.release(); pgout.release(); pg.release();
         d_spec.release(); master.release(); m.release(); v.release(); grad.release();
-        w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release();
+        w16.release(); ema16.release(); pack_train.release(); pack_infer.release(); steps.release(); steps16.release();
         x.release(); act.release(); g.release(); out.release(); gout_rm.release(); gout_fm.release();
         dx.release(); partial.release(); gfeat.release();
     }

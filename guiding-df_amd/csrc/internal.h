@@ -360,8 +360,13 @@ struct AdamArgs {
     const float* grad;
     float* m;
     float* v;
-    uint32_t* steps;
+    uint32_t* steps;     // per-parameter step counters, or NULL when steps16 is used
+    uint16_t* steps16;   // 16-bit counters saturating at kShortStepMax
 };
+// A 16-bit counter saturates here.  It is used only when beta^t rounds 1 - beta^t
+// to 1.0f for both betas at t = kShortStepMax (and so for every later t), so
+// the saturated counter gives the same bias corrections as the exact one.
+constexpr uint32_t kShortStepMax = 65535u;
 void launch_adam_ema(const AdamArgs& a, hipStream_t s);
 // Grid parameters [grid_begin, grid_begin + 2 n_entries) of two nets whose
 // grids share the interleaved tables (a: theta, b: phi), whole entries per

@@ -1190,8 +1190,14 @@ __device__ __forceinline__ void adam_ema_param(const AdamArgs& a, int64_t i, _Fl
         const float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * gsq;
         a.m[i] = m;
         a.v[i] = v;
-        const uint32_t step = a.steps[i] + 1;
-        a.steps[i] = step;
+        uint32_t step;
+        if (a.steps16) {
+            step = min((uint32_t)a.steps16[i] + 1u, kShortStepMax);
+            a.steps16[i] = (uint16_t)step;
+        } else {
+            step = a.steps[i] + 1;
+            a.steps[i] = step;
+        }
         // bias corrections beta^step as 2^(step log2 beta) (one hardware exp2
         // each instead of two powf per parameter)
         float lr = a.lr;
@@ -1236,14 +1242,21 @@ struct AdamPair {
     h2 w16, ema;
 };
 
+// The fp16 training copy is not read: every writer of a master weight also
+// writes its fp16 rounding, so the copy equals round_half(master).
 __device__ __forceinline__ AdamPair adam_pair_load(const AdamArgs& a, int64_t i, bool with_grad = true) {
     AdamPair p;
     p.g = with_grad ? *(const float2*)(a.grad + i) : make_float2(0.f, 0.f);
     p.w = *(const float2*)(a.master + i);
     p.m = *(const float2*)(a.m + i);
     p.v = *(const float2*)(a.v + i);
-    p.st = *(const uint2*)(a.steps + i);
-    p.w16 = *(const h2*)(a.w16 + i);
+    if (a.steps16) {
+        const ushort2 st = *(const ushort2*)(a.steps16 + i);
+        p.st = make_uint2(st.x, st.y);
+    } else {
+        p.st = *(const uint2*)(a.steps + i);
+    }
+    p.w16 = h2{round_half(p.w.x), round_half(p.w.y)};
     p.ema = *(const h2*)(a.ema16 + i);
     return p;
 }
@@ -1257,7 +1270,7 @@ __device__ __forceinline__ void adam_grid_update(const AdamArgs& a, float g, flo
         const float gsq = gr * gr;
         m = a.beta1 * m + (1.0f - a.beta1) * gr;
         v = a.beta2 * v + (1.0f - a.beta2) * gsq;
-        st += 1;
+        st = a.steps16 ? min(st + 1u, kShortStepMax) : st + 1u;
         float lr = a.lr;
         lr *= sqrtf(1.0f - exp2f((float)st * a.log2_beta2)) / (1.0f - exp2f((float)st * a.log2_beta1));
         const float elr = fminf(fmaxf(lr / (sqrtf(v) + a.eps), 0.0f), 3.402823466e+38f);
@@ -1281,7 +1294,8 @@ __device__ __forceinline__ void adam_pair_store(const AdamArgs& a, int64_t i, co
     *(float2*)(a.master + i) = p.w;
     *(float2*)(a.m + i) = p.m;
     *(float2*)(a.v + i) = p.v;
-    *(uint2*)(a.steps + i) = p.st;
+    if (a.steps16) *(ushort2*)(a.steps16 + i) = make_ushort2((unsigned short)p.st.x, (unsigned short)p.st.y);
+    else *(uint2*)(a.steps + i) = p.st;
     *(h2*)(a.w16 + i) = p.w16;
     *(h2*)(a.ema16 + i) = p.ema;
 }

@@ -180,6 +180,42 @@ def test_server_ring_wraps_and_budget_holds(torch_cuda):
     f.close()
 
 
+@pytest.mark.parametrize("n_streams,budget_rows,max_req", [(1, 20000, 64), (2, 3000, 256), (4, 5000, 16)])
+def test_server_many_requests_on_several_streams(torch_cuda, n_streams, budget_rows, max_req):
+    """20,000 requests of 1..300 rows (several passes through the 4,096-entry
+    ring) on 1, 2 or 4 launch streams, with rounds overlapping on the device
+    and admission racing for the shared head: every request completes once,
+    the rows in flight never exceed the budget, and every row equals one
+    dfg_query over all rows."""
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False, log2_hashmap_size=14))
+    rng = np.random.default_rng(31 + n_streams)
+    sizes = rng.integers(1, 300, 20000)
+    total = int(sizes.sum())
+    d_feat, d_u = dev(torch, df_features(total, 32)), dev(torch, uniforms(total, 33))
+    ref = dev(torch, wi_init(total))
+    f.query(d_feat, d_u, ref)
+    srv = G.Server(f, budget_bytes=budget_rows * 80, max_batch_requests=max_req, n_streams=n_streams, train_every=0,
+                   round_rows=1 << 16)
+    out = dev(torch, wi_init(total))
+    row = 0
+    for i, n in enumerate(sizes.tolist()):
+        srv.submit(i, d_feat[row:row + n], d_u[row:row + n], out[row:row + n])
+        row += n
+        if rng.random() < 0.01:
+            srv.progress()
+    srv.progress(wait=True)
+    st = srv.stats()
+    assert st["completed"] == len(sizes) and st["rows"] == total
+    assert 0 < st["peak_rows"] <= budget_rows, st["peak_rows"]
+    ids, _, _ = srv.drain(max_records=len(sizes))
+    assert sorted(ids.tolist()) == list(range(len(sizes)))
+    torch.cuda.synchronize()
+    assert np.array_equal(host(out), host(ref))
+    srv.close()
+    f.close()
+
+
 def test_server_training_interleave_is_ordered(torch_cuda):
     """Requests launched before the first training step see the initial weights;
     requests launched after the last one see the final weights: no torn reads."""

@@ -65,6 +65,41 @@ def test_wavefront_matches_per_bounce_queries(torch_cuda, tuning):
     f.close()
 
 
+def test_wavefront_graph_replays_with_random_sizes(torch_cuda):
+    """One captured frame graph replayed 12 times with random bounce sizes
+    (empty bounces, sizes that are not multiples of 64, 1 to 8 bounces) read
+    on the device: every replay equals the per-bounce queries, and rows past
+    the frame stay untouched."""
+    torch = torch_cuda
+    f = G.GuidingField(G.hash_config(with_nrc=False, log2_hashmap_size=14))
+    cap = 60000
+    feat, u = df_features(cap, 90), uniforms(cap, 91)
+    wi0 = wi_init(cap)
+    wf = G.Wavefront(f, 8, cap)
+    d_feat, d_u = dev(torch, feat), dev(torch, u)
+    d_wi = dev(torch, wi0)
+    d_sizes = torch.zeros(8, dtype=torch.int64, device="cuda")
+    rng = np.random.default_rng(92)
+    for rep in range(12):
+        nb = int(rng.integers(1, 9))
+        sizes = [int(x) for x in rng.integers(0, 7500, nb)]
+        for j in rng.choice(nb, size=min(nb, 2), replace=False):
+            if rng.random() < 0.5:
+                sizes[j] = 0
+        n = sum(sizes)
+        ref = _alone(torch, f, feat[:n], u[:n], wi0[:n], sizes, None)
+        d_sizes.zero_()
+        d_sizes[:nb].copy_(torch.tensor(sizes, dtype=torch.int64, device="cuda"))
+        d_wi.copy_(dev(torch, wi0))
+        wf.run(d_sizes[:nb], d_feat, d_u, d_wi)
+        torch.cuda.synchronize()
+        got = host(d_wi)
+        assert np.array_equal(got[:n], ref), "replay %d sizes %s" % (rep, sizes)
+        assert np.array_equal(got[n:], wi0[n:])
+    wf.close()
+    f.close()
+
+
 def test_wavefront_autotune(torch_cuda):
     torch = torch_cuda
     f = G.GuidingField(G.hash_config(with_nrc=False))

@@ -540,7 +540,7 @@ def test_nrc_train_step_vs_fp64(torch_cuda):
     f.close()
 
 
-@pytest.mark.parametrize("B", [4096, 262144, 1000])
+@pytest.mark.parametrize("B", [4096, 262144, 1000, 63, 1])
 def test_nrc_fused_training_matches_separate_kernels(torch_cuda, B):
     """The one-kernel radiance-cache training pass (activations kept in LDS)
     against the separate forward / loss / backward / weight-gradient kernels
@@ -582,6 +582,8 @@ def test_nrc_fused_training_matches_separate_kernels(torch_cuda, B):
 @pytest.mark.parametrize("cfgname,bins,interp,B", [("hash", (32, 32), "linear", 4096),
                                                    ("hash", (32, 32), "linear", 262144),
                                                    ("hash", (32, 32), "linear", 1000),
+                                                   ("hash", (32, 32), "linear", 65),
+                                                   ("hash", (32, 32), "linear", 1),
                                                    ("hash", (16, 32), "nearest", 4096),
                                                    ("hash", (64, 64), "linear", 3000),
                                                    ("reference", (16, 32), "linear", 4096),

@@ -1077,14 +1077,18 @@ static GridBucketArgs bucket_args(dfg_ctx* c, NetState& s, const float* feat, in
 
 // Shared bucket buffers for a plan with F values per record (offs carries
 // `maxima` per-level maxima arrays after the slice offsets).  F == 2: 12-byte
-// records in gb_val; otherwise an entry array and F values per record.
-static void ensure_bucket_ws(dfg_ctx* c, const GridBucketPlan& plan, int F, int maxima) {
+// records in gb_val; paired: 144-byte groups of 8 records in gb_val;
+// otherwise an entry array and F values per record.
+static void ensure_bucket_ws(dfg_ctx* c, const GridBucketPlan& plan, int F, int maxima, bool paired = false) {
     c->gb_offs.ensure((size_t)plan.n_slices + 1 + (size_t)maxima * DFG_MAX_LEVELS / 2);
     c->gb_part.ensure((size_t)plan.max_partials * kGridSliceFloats);
     c->gb_cursor.ensure((size_t)plan.n_slices);
     c->gb_cnt.ensure((size_t)plan.n_slices * plan.n_chunks);
-    if (F != 2) c->gb_idx.ensure((size_t)plan.records);
-    c->gb_val.ensure((size_t)plan.records * (F == 2 ? 3 : F));
+    if (F != 2 && !paired) c->gb_idx.ensure((size_t)plan.records);
+    if (paired)
+        c->gb_val.ensure((size_t)(plan.records + 7) / 8 * (kPairGroupBytes / 4));
+    else
+        c->gb_val.ensure((size_t)plan.records * (F == 2 ? 3 : F));
     c->gb_items.ensure((size_t)plan.max_items);
     c->gb_n_items.ensure(2);
     c->gb_multi.ensure((size_t)plan.max_multi);
@@ -1130,7 +1134,7 @@ static bool grid_backward_pair(dfg_ctx* c, const float* feat, int stride, const 
     // one record per corner for both nets: entry + float4 values, 2,048-entry slices
     const GridBucketPlan plan = grid_bucket_plan(th.spec, n, 4);
     if (!plan.ok) return false;
-    ensure_bucket_ws(c, plan, 4, 2);  // entry array + 4 values per record
+    ensure_bucket_ws(c, plan, 4, 2, true);  // groups of 8 records: 8 float4 values + 8 16-bit entries
     uint32_t* mx = (uint32_t*)(c->gb_offs.p + plan.n_slices + 1);
     GridBucketArgs a = bucket_args(c, th, feat, stride, n, ld, plan, c->gb_val.p, mx);
     if (pos) {

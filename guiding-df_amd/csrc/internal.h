@@ -295,6 +295,7 @@ void launch_grid_bwd(const GridBwdArgs& a, const NetSpec& hs, hipStream_t s);
 constexpr int kGridSliceFloats = 8192;        // 64 KB of int64 LDS accumulators per slice
 constexpr int kGridMaxSlicesPerLevel = 2048;
 constexpr int kGridChunkRows = 4096;
+constexpr int kPairGroupBytes = 144;          // paired records: 8 float4 values, then their 8 16-bit entries
 constexpr int kGridPartRecords = 65536;       // records reduced by one block
 struct GridBucketPlan {
     bool ok;                 // the configuration fits the bucketed kernels
@@ -321,9 +322,9 @@ struct GridBucketArgs {
     uint32_t* cnt;           // [n_slices][n_chunks] updates per slice and row chunk
     int64_t* offs;           // [n_slices + 1]: slice totals, then exclusive offsets
     int64_t* cursor;         // [n_slices] next free record of each slice
-    uint32_t* rec_idx;       // [records] entry within the slice (F != 2, and the paired records)
-    float* rec_val;          // F == 2: [records] 12-byte {entry, g0, g1}; paired: [records] float4;
-                             // else [records][F]
+    uint32_t* rec_idx;       // [records] entry within the slice (F != 2, unpaired)
+    float* rec_val;          // F == 2: [records] 12-byte {entry, g0, g1}; paired: groups of 8
+                             // records (kPairGroupBytes); else [records][F]
     int4* items;             // [max_items] (slice, part, parts of the slice, partial slot)
     int* n_items;            // [2]: work items, slices cut into several parts
     int4* multi;             // [max_multi] (slice, first partial slot, parts, -)
@@ -332,7 +333,8 @@ struct GridBucketArgs {
                                   // the parts of slices cut into several parts
     // second net of the paired backward (same grid structure): its dL/dy,
     // gradient, first grid parameter and level maxima; the records are then
-    // rec_idx as 16-bit entries in the slice + rec_val as float4 {g0, g1, g0', g1'}
+    // 18 bytes, in groups of 8 in rec_val: the 8 float4 {g0, g1, g0', g1'},
+    // then the 8 16-bit entries in the slice (one contiguous 144-byte run)
     const float* dx2;
     float* grad2;
     int64_t param_begin2;

@@ -730,8 +730,11 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 //
 // PAIR: the theta and phi grids of identical structure in one pass (same
 // positions, so the same corners, slices and record positions): one record
-// per corner, the 16-bit entry in rec_idx and {w dL/dy[0], w dL/dy[1], w dL/dy2[0],
-// w dL/dy2[1]} as a float4 in rec_val, and each net's level maxima.
+// per corner, {w dL/dy[0], w dL/dy[1], w dL/dy2[0], w dL/dy2[1]} as a float4
+// and the 16-bit entry, stored in groups of 8 records (the 8 float4, then the
+// 8 entries: 144 contiguous bytes, so a slice run is one contiguous store
+// range rather than a value run plus a separate 2-byte-per-record entry run),
+// and each net's level maxima.
 constexpr int kStageRows = 256;
 template <bool PAIR>
 __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
@@ -841,8 +844,9 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
             const int64_t pos = (int64_t)next[r.w] + (int64_t)(i - (int)loff[r.w]);
             if constexpr (PAIR) {
                 const uint2 q = stage2[i];
-                ((uint16_t*)a.rec_idx)[pos] = (uint16_t)r.x;  // entry in a slice of kGridSliceFloats / 4
-                ((uint4*)a.rec_val)[pos] = make_uint4(r.y, r.z, q.x, q.y);
+                unsigned char* grp = (unsigned char*)a.rec_val + (pos >> 3) * kPairGroupBytes;
+                *(uint4*)(grp + 16 * (pos & 7)) = make_uint4(r.y, r.z, q.x, q.y);
+                *(uint16_t*)(grp + 128 + 2 * (pos & 7)) = (uint16_t)r.x;  // entry in a slice of kGridSliceFloats / 4
             } else {
                 ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
             }
@@ -973,7 +977,7 @@ __global__ __launch_bounds__(256) void grid_bwd_combine_kernel(const GridBucketA
 __device__ void grid_adam_pair_entry(const GridAdam& ga, int64_t e, float2 gt, float2 gp);
 
 // Paired theta/phi reduce: one block per (slice, part) of the 18-byte records
-// (16-bit entry in rec_idx, rec_val float4); LDS sums [entry][4] (theta f0, f1, phi f0,
+// (float4 values and 16-bit entries, in groups of 8); LDS sums [entry][4] (theta f0, f1, phi f0,
 // f1), each net at its own fixed-point scale; writes both nets' slices (single
 // part) or the part's partial sums.  With ga.on a single-part slice goes
 // straight into the optimizer instead of the gradient arrays.
@@ -981,7 +985,7 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
     extern __shared__ unsigned long long acc[];  // [4][NE] (value-major: no bank conflicts)
     constexpr int NE = kGridSliceFloats / 4;
     static_assert(NE <= 65536, "paired record entries are 16-bit");
-    const uint16_t* rec_e = (const uint16_t*)a.rec_idx;
+    const unsigned char* recs = (const unsigned char*)a.rec_val;
     if ((int)blockIdx.x >= *a.n_items) return;
     const int4 item = a.items[blockIdx.x];
     const GridSpec& gs = a.ns->grid;
@@ -999,7 +1003,6 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
     const int64_t se = a.offs[s + 1];
     const int64_t pb = a.offs[s] + (int64_t)item.y * kGridPartRecords;
     const int64_t pe = pb + kGridPartRecords < se ? pb + kGridPartRecords : se;
-    const float4* r4 = (const float4*)a.rec_val;
     constexpr int U = 4;
     for (int64_t p0 = pb + threadIdx.x; p0 < pe; p0 += (int64_t)U * blockDim.x) {
         uint32_t e[U];
@@ -1008,8 +1011,9 @@ __global__ __launch_bounds__(1024) void grid_bwd_reduce_pair_kernel(const GridBu
         for (int j = 0; j < U; ++j) {
             const int64_t p = p0 + (int64_t)j * blockDim.x;
             const bool in = p < pe;
-            e[j] = in ? (uint32_t)rec_e[p] : 0u;
-            v[j] = in ? r4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const unsigned char* grp = recs + (p >> 3) * kPairGroupBytes;
+            e[j] = in ? (uint32_t) * (const uint16_t*)(grp + 128 + 2 * (p & 7)) : 0u;
+            v[j] = in ? *(const float4*)(grp + 16 * (p & 7)) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < U; ++j) {

@@ -302,6 +302,7 @@ struct GridBucketPlan {
     int slice_shift;         // log2 entries per slice
     int n_slices;            // all levels
     int n_chunks;            // row chunks of kGridChunkRows
+    int max_level_slices;    // largest slice count of one level
     int64_t records;         // n * levels * corners
     int64_t max_items;       // bound on (slice, part) work items
     int64_t max_partials;    // bound on the parts of slices cut into several parts

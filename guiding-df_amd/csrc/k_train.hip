@@ -721,12 +721,17 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
     if (threadIdx.x == 0 && bmax) atomicMax(&a.lvl_max[level], bmax);
 }
 
-// Staged scatter (F = 2, 8 corners): the records of 256 rows at a time (four
-// threads per row, one x pair of corners each) are sorted by slice in LDS and
-// then written as one contiguous run per slice, instead of one scattered
-// 12-byte store per record.  The same records land in the same slice ranges
-// (only their order inside a range differs, which the order-independent
-// fixed-point reduce does not see).
+// Staged scatter (F = 2, 8 corners): the records of 448 rows at a time (four
+// threads per row, one x pair of corners each; threads 0..767 take a second
+// row) are sorted by slice in LDS and then written as one contiguous run per
+// slice, instead of one scattered 12-byte store per record.  Longer runs
+// leave fewer partly written lines: 256-row sub-chunks took 308 us at 262k
+// samples, 128-row ones 408 us, 448-row ones 270 us.  Staged records are 20
+// bytes (float4 values + entry | slice << 16), the slice cursors sit in
+// dynamic LDS sized for the largest level, and 64 VGPRs keep two 1,024-thread
+// blocks (75 KB of LDS each) per CU.  The same records land in the same
+// slice ranges (only their order inside a range differs, which the
+// order-independent fixed-point reduce does not see).
 //
 // PAIR: the theta and phi grids of identical structure in one pass (same
 // positions, so the same corners, slices and record positions): one record
@@ -735,15 +740,14 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_kernel(const GridBucket
 // 8 entries: 144 contiguous bytes, so a slice run is one contiguous store
 // range rather than a value run plus a separate 2-byte-per-record entry run),
 // and each net's level maxima.
-constexpr int kStageRows = 256;
+constexpr int kStageRows = 448;  // rows per sub-chunk
 template <bool PAIR>
-__global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
-    // next record of each slice in this block's range (records < 2^31)
-    __shared__ uint32_t next[kGridMaxSlicesPerLevel];
-    __shared__ uint32_t lcnt[kGridMaxSlicesPerLevel];
-    __shared__ uint32_t loff[kGridMaxSlicesPerLevel];
-    __shared__ uint4 stage[kStageRows * 8];
-    __shared__ uint2 stage2[PAIR ? kStageRows * 8 : 1];
+__global__ __launch_bounds__(1024, 8) void grid_bwd_scatter_staged_kernel(const GridBucketArgs a) {
+    // next record of each slice in this block's range (records < 2^31), the
+    // sub-chunk's count and run offset of each slice: 3 x the level's slices
+    extern __shared__ uint32_t slice_lds[];
+    __shared__ float4 sval[kStageRows * 8];   // staged record values
+    __shared__ uint32_t skey[kStageRows * 8];  // staged entry | slice << 16
     __shared__ uint32_t bmax, bmax2;
     const GridSpec& gs = a.ns->grid;
     const int level = blockIdx.y;
@@ -751,6 +755,9 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
     const int t = threadIdx.x;
     const int nsl = level_slices(gs, level, a.slice_shift);
     const int sb = level_slice_base(gs, level, a.slice_shift);
+    uint32_t* const next = slice_lds;
+    uint32_t* const lcnt = slice_lds + nsl;
+    uint32_t* const loff = slice_lds + 2 * nsl;
     if (t == 0) bmax = bmax2 = 0u;
     for (int i = t; i < nsl; i += blockDim.x) {
         const uint32_t h = a.cnt[(size_t)(sb + i) * a.n_chunks + c];
@@ -760,8 +767,6 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
     const int64_t r1 = r0 + kGridChunkRows < a.n ? r0 + kGridChunkRows : a.n;
     const uint32_t emask = (1u << a.slice_shift) - 1u;
     uint32_t umax = 0u, umax2 = 0u;
-    // this thread's row inputs, loaded one sub-chunk ahead (the loads of the
-    // next sub-chunk are in flight while this one is written out)
     struct RowIn { float d0, d1, e0, e1, px, py, pz; };
     auto load_row = [&](int64_t row) {
         RowIn v{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -779,32 +784,30 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
         }
         return v;
     };
-    RowIn cur = load_row(r0 + (t >> 2));
+    // this thread's rows of a sub-chunk: s0 + t/4 and (threads below
+    // 4 (kStageRows - 256)) s0 + 256 + t/4; each thread one x pair of corners
+    const int rq = t >> 2;
+    const bool two = rq + 256 < kStageRows;
+    RowIn cur[2];
+    cur[0] = load_row(r0 + rq);
+    cur[1] = two ? load_row(r0 + 256 + rq) : RowIn{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int64_t s0 = r0; s0 < r1; s0 += kStageRows) {
         for (int i = t; i < nsl; i += blockDim.x) lcnt[i] = 0u;
         __syncthreads();
-        const int64_t row = s0 + (t >> 2);
-        const bool live = row < r1;
-        uint32_t sl[2] = {0u, 0u}, lp[2] = {0u, 0u};
-        uint4 rec[2];
-        uint2 rec2[2];
-        if (live) {
-            const float d0 = cur.d0, d1 = cur.d1, e0 = cur.e0, e1 = cur.e1;
-            uint32_t idx[2];
-            float w[2];
-            grid_corner_pair(gs, level, cur.px, cur.py, cur.pz, t & 3, idx, w);
+        bool live[2];
+        uint32_t sl[4] = {0u, 0u, 0u, 0u}, lp[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                sl[j] = idx[j] >> a.slice_shift;
-                const float v0 = w[j] * d0, v1 = w[j] * d1;
-                rec[j] = make_uint4(idx[j] & emask, __float_as_uint(v0), __float_as_uint(v1), sl[j]);
-                umax = max(umax, max(__float_as_uint(fabsf(v0)), __float_as_uint(fabsf(v1))));
-                if constexpr (PAIR) {
-                    const float u0 = w[j] * e0, u1 = w[j] * e1;
-                    rec2[j] = make_uint2(__float_as_uint(u0), __float_as_uint(u1));
-                    umax2 = max(umax2, max(__float_as_uint(fabsf(u0)), __float_as_uint(fabsf(u1))));
+        for (int h = 0; h < 2; ++h) {
+            live[h] = (h == 0 || two) && s0 + 256 * h + rq < r1;
+            if (live[h]) {
+                uint32_t idx[2];
+                float w[2];
+                grid_corner_pair(gs, level, cur[h].px, cur[h].py, cur[h].pz, t & 3, idx, w);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    sl[2 * h + j] = idx[j] >> a.slice_shift;
+                    lp[2 * h + j] = atomicAdd(&lcnt[sl[2 * h + j]], 1u);
                 }
-                lp[j] = atomicAdd(&lcnt[sl[j]], 1u);
             }
         }
         __syncthreads();
@@ -828,27 +831,46 @@ __global__ __launch_bounds__(1024) void grid_bwd_scatter_staged_kernel(const Gri
             }
         }
         __syncthreads();
-        if (live) {
+        // the records into their slice runs (corners recomputed: the same
+        // operations give the same indices and weights, and nothing stays
+        // live across the scan)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const uint32_t k = loff[sl[j]] + lp[j];
-                stage[k] = rec[j];
-                if constexpr (PAIR) stage2[k] = rec2[j];
+        for (int h = 0; h < 2; ++h) {
+            if (live[h]) {
+                uint32_t idx[2];
+                float w[2];
+                grid_corner_pair(gs, level, cur[h].px, cur[h].py, cur[h].pz, t & 3, idx, w);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const uint32_t k = loff[sl[2 * h + j]] + lp[2 * h + j];
+                    const float v0 = w[j] * cur[h].d0, v1 = w[j] * cur[h].d1;
+                    umax = max(umax, max(__float_as_uint(fabsf(v0)), __float_as_uint(fabsf(v1))));
+                    float u0 = 0.0f, u1 = 0.0f;
+                    if constexpr (PAIR) {
+                        u0 = w[j] * cur[h].e0;
+                        u1 = w[j] * cur[h].e1;
+                        umax2 = max(umax2, max(__float_as_uint(fabsf(u0)), __float_as_uint(fabsf(u1))));
+                    }
+                    sval[k] = make_float4(v0, v1, u0, u1);
+                    skey[k] = (idx[j] & emask) | (sl[2 * h + j] << 16);
+                }
             }
         }
         __syncthreads();
-        cur = load_row(s0 + kStageRows + (t >> 2));
+        cur[0] = load_row(s0 + kStageRows + rq);
+        if (two) cur[1] = load_row(s0 + kStageRows + 256 + rq);
         const int nrec = (int)((r1 - s0 < kStageRows ? r1 - s0 : kStageRows) * 8);
         for (int i = t; i < nrec; i += blockDim.x) {
-            const uint4 r = stage[i];
-            const int64_t pos = (int64_t)next[r.w] + (int64_t)(i - (int)loff[r.w]);
+            const uint32_t key = skey[i];
+            const uint32_t s = key >> 16;
+            const int64_t pos = (int64_t)next[s] + (int64_t)(i - (int)loff[s]);
+            const float4 v = sval[i];
             if constexpr (PAIR) {
-                const uint2 q = stage2[i];
                 unsigned char* grp = (unsigned char*)a.rec_val + (pos >> 3) * kPairGroupBytes;
-                *(uint4*)(grp + 16 * (pos & 7)) = make_uint4(r.y, r.z, q.x, q.y);
-                *(uint16_t*)(grp + 128 + 2 * (pos & 7)) = (uint16_t)r.x;  // entry in a slice of kGridSliceFloats / 4
+                *(float4*)(grp + 16 * (pos & 7)) = v;
+                *(uint16_t*)(grp + 128 + 2 * (pos & 7)) = (uint16_t)(key & 0xffffu);  // entry in a slice of kGridSliceFloats / 4
             } else {
-                ((uint3*)a.rec_val)[pos] = make_uint3(r.x, r.y, r.z);
+                ((uint3*)a.rec_val)[pos] = make_uint3(key & 0xffffu, __float_as_uint(v.x), __float_as_uint(v.y));
             }
         }
         __syncthreads();
@@ -1138,6 +1160,7 @@ GridBucketPlan grid_bucket_plan(const NetSpec& hs, int64_t n, int values_per_ent
     // a slice of c > P records has ceil(c / P) < 2c / P parts
     p.max_partials = 2 * (p.records / kGridPartRecords) + 1;
     p.max_multi = p.records / kGridPartRecords + 1;  // each holds more than P records
+    p.max_level_slices = maxl;
     p.ok = maxl <= kGridMaxSlicesPerLevel && p.records <= ((int64_t)1 << 31) && p.n_chunks <= 65535 &&
            p.max_items < ((int64_t)1 << 30);
     return p;
@@ -1151,7 +1174,8 @@ void launch_grid_bwd_bucketed(const GridBucketArgs& a, const NetSpec& hs, const 
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
     if (hs.grid.n_feat == 2 && hs.grid.interp != INTERP_NEAREST)
-        hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<false>, g1, dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<false>, g1, dim3(1024),
+                           (size_t)3 * p.max_level_slices * sizeof(uint32_t), s, a);
     else
         hipLaunchKernelGGL(grid_bwd_scatter_kernel, g1, dim3(1024), 0, s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
@@ -1171,7 +1195,8 @@ void launch_grid_bwd_bucketed_pair(const GridBucketArgs& a, const NetSpec& hs, c
     launch_fill_u32((unsigned int*)a.offs, 0u, 2 * (int64_t)(p.n_slices + 1) + 2 * DFG_MAX_LEVELS, s);
     hipLaunchKernelGGL(grid_bwd_count_kernel, g1, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(grid_bwd_scan_kernel, dim3(1), dim3(1024), 0, s, a, p.n_slices);
-    hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(grid_bwd_scatter_staged_kernel<true>, g1, dim3(1024),
+                       (size_t)3 * p.max_level_slices * sizeof(uint32_t), s, a);
     const size_t lds = (size_t)kGridSliceFloats * sizeof(unsigned long long);
     hipLaunchKernelGGL(grid_bwd_reduce_pair_kernel, dim3((unsigned)p.max_items), dim3(1024), lds, s, a, ga);
     hipLaunchKernelGGL(grid_bwd_combine_pair_kernel, dim3((unsigned)p.max_multi, 16), dim3(256), 0, s, a, ga);

@@ -206,19 +206,21 @@ __global__ __launch_bounds__(256, 2) void train_fused_kernel(const TrainFusedArg
 
     for (int64_t tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
         const int64_t row0 = tile * 64;
-        // ---- encode: lanes 0..15 of wave w encode rows 16w..16w+15 (lane = row)
+        // ---- encode: lanes 0..15 of wave w put the raw inputs of rows
+        // 16w..16w+15 into the row tile; each row's 4 lanes then encode its parts
         load_rows(tile);
         float* fr = FT + rw * kMaxIn;
         if (lane < 16)
 #pragma unroll
             for (int d = 0; d < kMaxIn; ++d) fr[d] = fin[d];
         if constexpr (LOSS == FL_NRC) {
-            if (lane < 16)
-                encode_row(ns, a.tab, RowInL{fr}, [X, rw](int c, float v) { X[c * RS + rw] = round_half(v); },
-                           nullptr, 0, false, true);
+            // the row's nested encodings spread over its 4 lanes (lane g:
+            // parts g, g + 4, ...), the raw inputs through the row tile
+            __syncthreads();
+            encode_row(ns, a.tab, RowInL{fr}, [X, rw](int c, float v) { X[c * RS + rw] = round_half(v); },
+                       nullptr, 0, false, g == 0, g, 4);
         } else {
-            // the row's nested encodings spread over its 4 lanes (lane g: parts
-            // g, g + 4, ...); the raw inputs reach them through the row tile
+            // (as above; a DF row's grid features come from the pre-pass)
             __syncthreads();
             const int64_t r = row0 + rw < a.n ? row0 + rw : a.n - 1;
             encode_row(ns, a.tab, RowInL{fr}, [X, rw](int c, float v) { X[c * RS + rw] = round_half(v); },
